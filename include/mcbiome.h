@@ -1,0 +1,157 @@
+/* mcbiome.h - C ABI of the MI355X (gfx950) biome engine: libmcbiome_hip.so
+ *
+ * Plain C types only (pointers, sizes, 64-bit seeds); no HIP or torch types.  Every entry point
+ * returns 0 on success and a nonzero MCB_E_* code on failure, with the message available from
+ * mcb_last_error() (thread-local).  The library never falls back to a CPU path: without a usable
+ * HIP device every compute entry point fails with MCB_E_DEVICE.
+ *
+ * Biome ids are u8 indices into the 67-name alphabetical table (mcg_biome_name), the ids of the
+ * reference engine (engine/gen/mc_biome_names.h:4-5).  Versions: 0 = 26.1, 1 = 26.2, 2 = 26.3.
+ * Dimensions: 0 = overworld, 1 = nether, 2 = end.  Search semantics are the reference CPU engine's:
+ * each sample's R-tree search starts from no candidate (engine/mc_biomes.h:58-62).
+ *
+ * Data: climate-<V>.txt, params/<V>/{overworld,nether}.tsv and rtree_<V>_{overworld,the_nether}.json
+ * are read from `data_dir`, or when that is NULL from $MCB_DATA, else $MCGEN_ROOT/data (the
+ * reference engine's convention, engine/mc_data.h:19), else the data/ folder beside the library.
+ */
+#ifndef MCBIOME_H
+#define MCBIOME_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { MCB_OK = 0, MCB_E_ARG = 1, MCB_E_IO = 2, MCB_E_DATA = 3, MCB_E_DEVICE = 4, MCB_E_NOMEM = 5, MCB_E_LIMIT = 6 };
+
+/* ================================================================================================
+ * Drop-in for the reference CPU engine API, engine/cuda/mcg_host.h:17-45.  Same names, argument
+ * meaning, output layout and return codes; the work runs on the HIP device.  McgHost keeps the
+ * reference's public fields (version, dim, mode, preset); the engine state behind it is opaque.
+ * ================================================================================================ */
+typedef struct McgHost {
+    int version;        /* 0 = 26.1, 1 = 26.2, 2 = 26.3 */
+    int dim;            /* 0 overworld, 1 nether, 2 end */
+    int mode;           /* 0 = double noise (26.1/26.2), 1 = float noise (26.3) */
+    char preset[16];    /* "overworld" | "amplified" | "large_biomes" | "-" */
+    void *impl;         /* McbEngine* */
+} McgHost;
+
+/* replaces mcg_host_load (engine/cuda/mcg_host.h:34 / mcg_host.c:8-27): 0 = ok, -1 = failure */
+int mcg_host_load(McgHost *H, int version, int dim, const char *preset);
+/* replaces mcg_host_free (engine/cuda/mcg_host.h:35) */
+void mcg_host_free(McgHost *H);
+/* replace mcg_version_from_name / mcg_version_name / mcg_biome_count / mcg_biome_name / mcg_biome_id
+ * (engine/cuda/mcg_host.h:37-42) */
+int mcg_version_from_name(const char *s);
+const char *mcg_version_name(int v);
+int mcg_biome_count(void);
+const char *mcg_biome_name(int id);
+int mcg_biome_id(const char *name);
+/* replaces mcg_ref_points (engine/cuda/mcg_host.h:47, mcg_host.c:36-83): host buffers, synchronous.
+ * pts = np (qx,qy,qz) quart triples shared by all seeds; out_biome[s*np+p];
+ * out_tg optional: 6 i32 per sample (t,h,c,e,d,w; Nether t,h,0,0,0,0; End zeros). */
+void mcg_ref_points(const McgHost *H, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out_biome,
+                    int32_t *out_tg);
+
+/* ================================================================================================
+ * Native engine API (batched, device buffers).  Counterparts: gb_open / gb_points_gpu of
+ * crack/src/gpu_biomes/gpu_biomes.h:22,30, and the per-call mcgpu_biome_grid of
+ * libmcgen/gpu/mcgpu_biome.cu:203-249 that the batched plans replace.
+ * ================================================================================================ */
+typedef struct McbEngine McbEngine;
+typedef struct McbPlan McbPlan;
+
+const char *mcb_last_error(void);
+int mcb_device_count(void);
+
+/* One (version, dimension, preset) world on the current HIP device.  max_slots = seed-state slots
+ * kept resident on the device (15,472 bytes each; 0 = default 8192). */
+int mcb_open(McbEngine **out, int version, int dim, const char *preset, const char *data_dir, int max_slots);
+void mcb_close(McbEngine *e);
+int mcb_engine_info(const McbEngine *e, int *version, int *dim, int *mode, int *tree_nodes);
+
+/* Seeds x points, host buffers, synchronous (mcg_ref_points contract). */
+int mcb_points(McbEngine *e, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out_biome,
+               int32_t *out_tg);
+/* Same with device buffers, asynchronous on `stream` (a hipStream_t, NULL = default stream). */
+int mcb_points_dev(McbEngine *e, const int64_t *d_seeds, int ns, const int32_t *d_pts, int np, uint8_t *d_out_biome,
+                   int32_t *d_out_tg, void *stream);
+
+/* A region request.  zoom = 0: quart grid, out[out_off + j*nx*nz + iz*nx + ix] = biome at quart
+ * (x0+ix, y[j], z0+iz).  zoom = 1: block grid at block resolution through BiomeManager with the
+ * SHA-256 hashed seed, biome at block (x0+ix, y[j], z0+iz) (engine/cuda/mc_gpu.cuh:96-118);
+ * at most 4 layers for zoom, 8 otherwise. */
+typedef struct McbRegion {
+    int64_t seed;
+    int32_t x0, z0, nx, nz;
+    int32_t ny;
+    int32_t y[8];
+    int32_t zoom;
+    int32_t world;      /* context world index (mcb_context_world); ignored by single-engine plans */
+    int64_t out_off;
+} McbRegion;
+
+/* Bytes of output a request list needs (max over out_off + ny*nx*nz). */
+int64_t mcb_regions_bytes(const McbRegion *reqs, int n);
+/* Plain per-call path: one request, host output, synchronous (allocates, launches, copies back). */
+int mcb_region(McbEngine *e, const McbRegion *req, uint8_t *out_host);
+
+/* Batched plan over many requests of this engine's world: seed slots, tile list and launch shape are
+ * fixed at creation; each run initialises the seeds' noise state and evaluates every tile in one
+ * persistent grid launch (plus the zoom pass), asynchronously on `stream`, into device memory d_out.
+ * tile_w: tile width in columns (tile = tile_w x 256/tile_w; 0 = autotuned default).
+ * blocks: persistent workgroups (0 = one per tile). */
+int mcb_plan_create(McbEngine *e, const McbRegion *reqs, int n, int tile_w, int blocks, McbPlan **out);
+int mcb_plan_run(McbPlan *p, uint8_t *d_out, void *stream);
+/* One stage of a run: 0 = seed state, 1 = quart-grid tiles, 2 = quart layers of zoomed requests,
+ * 3 = zoom pass.  mcb_plan_run = stages 0..3 in order on one stream. */
+int mcb_plan_run_stage(McbPlan *p, int stage, uint8_t *d_out, void *stream);
+/* Capture the run into a hipGraph once and replay it (same d_out each time). */
+int mcb_plan_run_graph(McbPlan *p, uint8_t *d_out, void *stream);
+int mcb_plan_info(const McbPlan *p, int64_t *tiles, int64_t *columns, int64_t *samples, int64_t *seeds);
+void mcb_plan_free(McbPlan *p);
+/* Project-written autotuner: per world group of the plan, time the quart-grid launch for each tile
+ * shape (tile_w in 8/16/32/64) and persistent-workgroup count (one per tile, or 2/4/8 per CU) and keep the
+ * fastest.  Uses d_out as scratch output.  Writes the chosen tile_w of the first group to *tile_w. */
+int mcb_plan_autotune(McbPlan *p, uint8_t *d_out, void *stream, int *tile_w, double *best_ms);
+
+/* ================================================================================================
+ * Contexts: several worlds (version x dimension x preset) on one device, for uneven mixed batches
+ * and the serving loop.  A region's `world` field is an index returned by mcb_context_world.
+ * ================================================================================================ */
+typedef struct McbContext McbContext;
+int mcb_context_create(McbContext **out, const char *data_dir);
+void mcb_context_free(McbContext *c);
+/* Index of the world (opened on first use), or a negative MCB_E_* code. */
+int mcb_context_world(McbContext *c, int version, int dim, const char *preset);
+/* Plan over regions of any worlds of the context: one group (seed init + persistent grid launch +
+ * zoom pass) per world present, run back to back on one stream; mcb_plan_run/_graph/_autotune apply. */
+int mcb_context_plan(McbContext *c, const McbRegion *reqs, int n, int tile_w, int blocks, McbPlan **out);
+
+/* Engine of a context world (owned by the context). */
+int mcb_context_world_engine(McbContext *c, int world, McbEngine **e);
+
+/* ================================================================================================
+ * Serving loop (continuous batching).  Requests of any context world are admitted in arrival order
+ * into one of `lanes` in-flight batches while their output and zoom-staging bytes fit the lane's
+ * device arena of `batch_bytes`; seed state is kept in a per-world device LRU of `slot_capacity`
+ * slots (15,472 bytes each) so repeated seeds skip their initialisation; batches on different lanes
+ * overlap on the device.  Results are copied into the caller's host buffer (ny*nx*nz bytes, layout of
+ * McbRegion, out_off ignored).  Latency = submit to result in host memory.  No counterpart in the
+ * reference (its closest is the terrain slot producer, libmcgen/src/gpu_bridge.c:605-760).
+ * ================================================================================================ */
+typedef struct McbServer McbServer;
+int mcb_server_create(McbServer **out, McbContext *ctx, int64_t batch_bytes, int lanes, int slot_capacity);
+int mcb_server_submit(McbServer *s, const McbRegion *req, uint8_t *host_out, int64_t *ticket);
+int mcb_server_wait(McbServer *s, int64_t ticket, double *latency_ms);
+int mcb_server_drain(McbServer *s);
+int mcb_server_stats(McbServer *s, double *p50_ms, double *p99_ms, double *p999_ms, int64_t *completed,
+                     double *slot_hit_rate, int64_t *batches);
+int mcb_server_reset_stats(McbServer *s);
+void mcb_server_destroy(McbServer *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

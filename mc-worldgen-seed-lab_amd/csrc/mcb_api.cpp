@@ -1,0 +1,215 @@
+// mcb_api.cpp - engine objects, the points path and the reference CPU-engine drop-in
+// (include/mcbiome.h).  Host code only: data loading goes through build_world (mcb_world.cpp), all
+// sampling through the HIP kernels (mcb_kernels.hip).  There is no CPU compute path: a missing device
+// is an error.
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mcb_internal.h"
+
+using namespace mcb;
+
+namespace mcb {
+static thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+std::string default_data_dir() {
+    if (const char *d = getenv("MCB_DATA")) return d;
+    if (const char *r = getenv("MCGEN_ROOT")) return std::string(r) + "/data";
+    Dl_info info;
+    if (dladdr((void *)&default_data_dir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        size_t s = p.rfind('/');
+        std::string dir = s == std::string::npos ? "." : p.substr(0, s);
+        return dir + "/../data";      // <pkg>/lib/libmcbiome_hip.so -> <pkg>/data
+    }
+    return "data";
+}
+}  // namespace mcb
+
+namespace {
+int device_ready() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(MCB_E_DEVICE, "no HIP device");
+    return 0;
+}
+
+int ensure_ident(McbEngine *e, int n) {
+    if ((size_t)n <= e->ident_cap) return 0;
+    if (int rc = dev_grow(&e->d_ident, &e->ident_cap, (size_t)n)) return rc;
+    std::vector<int32_t> id(n);
+    for (int i = 0; i < n; i++) id[i] = i;
+    MCB_HIPCHK(hipMemcpy(e->d_ident, id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// seeds are processed in chunks so the points grid stays within the y-dimension limit
+const int kSeedChunk = 32768;
+
+int points_locked(McbEngine *e, const int64_t *d_seeds, int ns, const int32_t *d_pts, int np, uint8_t *d_out,
+                  int32_t *d_tg, hipStream_t st) {
+    const int chunk = std::min(ns, kSeedChunk);
+    if (int rc = dev_grow(&e->d_slots, &e->slots_cap, (size_t)chunk)) return rc;
+    if (int rc = ensure_ident(e, chunk)) return rc;
+    for (int s0 = 0; s0 < ns; s0 += chunk) {
+        const int n = std::min(chunk, ns - s0);
+        const size_t o = (size_t)s0 * np;
+        MCB_HIPCHK(launch_seed_init(e->w, d_seeds + s0, e->d_ident, n, e->d_slots, st));
+        MCB_HIPCHK(launch_points(e->w, e->d_slots, e->d_ident, n, d_pts, np, d_out + o, d_tg ? d_tg + o * 6 : nullptr, st));
+    }
+    return MCB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char *mcb_last_error(void) { return g_err.c_str(); }
+
+int mcb_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int mcg_version_from_name(const char *s) {
+    if (!s) return -1;
+    if (!strncmp(s, "26.1", 4)) return MCB_V26_1;
+    if (!strncmp(s, "26.2", 4)) return MCB_V26_2;
+    if (!strncmp(s, "26.3", 4)) return MCB_V26_3;
+    return -1;
+}
+const char *mcg_version_name(int v) { return v == 0 ? "26.1" : v == 1 ? "26.2" : v == 2 ? "26.3" : "?"; }
+int mcg_biome_count(void) { return biome_count(); }
+const char *mcg_biome_name(int id) { return biome_name(id); }
+int mcg_biome_id(const char *name) { return name ? biome_id(name) : -1; }
+
+int mcb_open(McbEngine **out, int version, int dim, const char *preset, const char *data_dir, int max_slots) {
+    (void)max_slots;
+    if (!out) return fail(MCB_E_ARG, "null out");
+    *out = nullptr;
+    if (int rc = device_ready()) return rc;
+    McbEngine *e = new McbEngine();
+    std::string err;
+    std::vector<McbNode> tree;
+    std::string root = data_dir ? data_dir : default_data_dir();
+    if (build_world(root, version, dim, preset ? preset : "normal", &e->w.host, tree, err) != 0) {
+        delete e;
+        return fail(MCB_E_DATA, "%s", err.c_str());
+    }
+    e->w.version = version;
+    e->w.dim = dim;
+    e->w.preset = preset ? preset : "normal";
+    if (hipGetDevice(&e->device) != hipSuccess || hipMalloc((void **)&e->w.d_world, sizeof(McbWorld)) != hipSuccess ||
+        hipMemcpy(e->w.d_world, &e->w.host, sizeof(McbWorld), hipMemcpyHostToDevice) != hipSuccess) {
+        mcb_close(e);
+        return fail(MCB_E_DEVICE, "world upload failed");
+    }
+    if (!tree.empty()) {
+        if (hipMalloc((void **)&e->w.d_tree, tree.size() * sizeof(McbNode)) != hipSuccess ||
+            hipMemcpy(e->w.d_tree, tree.data(), tree.size() * sizeof(McbNode), hipMemcpyHostToDevice) != hipSuccess) {
+            mcb_close(e);
+            return fail(MCB_E_DEVICE, "tree upload failed");
+        }
+    }
+    *out = e;
+    return MCB_OK;
+}
+
+void mcb_close(McbEngine *e) {
+    if (!e) return;
+    (void)hipFree(e->w.d_world);
+    (void)hipFree(e->w.d_tree);
+    (void)hipFree(e->d_slots);
+    (void)hipFree(e->d_ident);
+    (void)hipFree(e->d_seeds);
+    (void)hipFree(e->d_pts);
+    (void)hipFree(e->d_out);
+    (void)hipFree(e->d_tg);
+    delete e;
+}
+
+int mcb_engine_info(const McbEngine *e, int *version, int *dim, int *mode, int *tree_nodes) {
+    if (!e) return fail(MCB_E_ARG, "null engine");
+    if (version) *version = e->w.version;
+    if (dim) *dim = e->w.dim;
+    if (mode) *mode = e->w.host.fmt_new;
+    if (tree_nodes) *tree_nodes = e->w.host.tree_n;
+    return MCB_OK;
+}
+
+int mcb_points_dev(McbEngine *e, const int64_t *d_seeds, int ns, const int32_t *d_pts, int np, uint8_t *d_out,
+                   int32_t *d_tg, void *stream) {
+    if (!e || ns < 0 || np < 0 || (ns && np && (!d_seeds || !d_pts || !d_out))) return fail(MCB_E_ARG, "bad arguments");
+    if (ns == 0 || np == 0) return MCB_OK;
+    std::lock_guard<std::mutex> lk(e->mu);
+    return points_locked(e, d_seeds, ns, d_pts, np, d_out, d_tg, (hipStream_t)stream);
+}
+
+int mcb_points(McbEngine *e, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out, int32_t *tg) {
+    if (!e || ns < 0 || np < 0 || (ns && np && (!seeds || !pts || !out))) return fail(MCB_E_ARG, "bad arguments");
+    if (ns == 0 || np == 0) return MCB_OK;
+    const size_t n = (size_t)ns * np;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (int rc = dev_grow(&e->d_seeds, &e->seeds_cap, (size_t)ns)) return rc;
+    if (int rc = dev_grow(&e->d_pts, &e->pts_cap, (size_t)np * 3)) return rc;
+    if (int rc = dev_grow(&e->d_out, &e->out_cap, n)) return rc;
+    if (tg)
+        if (int rc = dev_grow(&e->d_tg, &e->tg_cap, n * 6)) return rc;
+    MCB_HIPCHK(hipMemcpy(e->d_seeds, seeds, ns * sizeof(int64_t), hipMemcpyHostToDevice));
+    MCB_HIPCHK(hipMemcpy(e->d_pts, pts, (size_t)np * 3 * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (int rc = points_locked(e, e->d_seeds, ns, e->d_pts, np, e->d_out, tg ? e->d_tg : nullptr, nullptr)) return rc;
+    MCB_HIPCHK(hipMemcpy(out, e->d_out, n, hipMemcpyDeviceToHost));
+    if (tg) MCB_HIPCHK(hipMemcpy(tg, e->d_tg, n * 6 * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return MCB_OK;
+}
+
+int mcg_host_load(McgHost *H, int version, int dim, const char *preset) {
+    if (!H) return -1;
+    memset(H, 0, sizeof *H);
+    H->version = version;
+    H->dim = dim;
+    McbEngine *e = nullptr;
+    const char *p = (!preset || !strcmp(preset, "normal")) ? "overworld" : preset;
+    if (mcb_open(&e, version, dim, dim == MCB_OVERWORLD ? p : "overworld", nullptr, 0) != MCB_OK) return -1;
+    H->impl = e;
+    H->mode = e->w.host.fmt_new;
+    snprintf(H->preset, sizeof H->preset, "%s", dim == MCB_OVERWORLD ? p : "-");
+    return 0;
+}
+
+void mcg_host_free(McgHost *H) {
+    if (!H) return;
+    mcb_close((McbEngine *)H->impl);
+    H->impl = nullptr;
+}
+
+void mcg_ref_points(const McgHost *H, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out_biome,
+                    int32_t *out_tg) {
+    if (!H || !H->impl) {
+        fprintf(stderr, "mcg_ref_points: engine not loaded\n");
+        abort();
+    }
+    if (mcb_points((McbEngine *)H->impl, seeds, ns, pts, np, out_biome, out_tg) != MCB_OK) {
+        // the reference returns void; a device failure must not look like a result
+        fprintf(stderr, "mcg_ref_points: %s\n", g_err.c_str());
+        abort();
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// mcb_internal.h - host-side internals shared by the C-ABI layer, the plans and the server.
+#ifndef MCB_INTERNAL_H
+#define MCB_INTERNAL_H
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mcbiome.h"
+#include "mcb_common.h"
+
+namespace mcb {
+
+int biome_count();
+const char *biome_name(int id);
+int biome_id(const char *name);
+
+// Fills the world constants and the pre-order R-tree from the data directory.  0 on success.
+int build_world(const std::string &root, int version, int dim, const std::string &preset, McbWorld *w,
+                std::vector<McbNode> &tree, std::string &err);
+
+// BiomeManager.obfuscateSeed (SHA-256 of the seed's little-endian bytes, first 8 digest bytes LE)
+int64_t hashed_seed(int64_t seed);
+
+// Device copy of one world.
+struct DevWorld {
+    McbWorld host;
+    McbWorld *d_world = nullptr;
+    McbNode *d_tree = nullptr;
+    int version = 0, dim = 0;
+    std::string preset;
+};
+
+// error reporting: sets the thread-local message returned by mcb_last_error, returns `code`
+int fail(int code, const char *fmt, ...);
+std::string default_data_dir();
+
+#define MCB_HIPCHK(x)                                                                                    \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) return ::mcb::fail(MCB_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_));     \
+    } while (0)
+
+template <class T>
+int dev_grow(T **p, size_t *cap, size_t n) {
+    if (n <= *cap) return 0;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return fail(MCB_E_NOMEM, "hipMalloc %zu bytes", n * sizeof(T));
+    *cap = n;
+    return 0;
+}
+
+// ---- kernel launchers (mcb_kernels.hip); all asynchronous on `stream` ----
+hipError_t launch_seed_init(const DevWorld &w, const int64_t *d_seeds, const int32_t *d_slot_of, int n,
+                            McbSlot *d_slots, hipStream_t stream);
+hipError_t launch_grid(const DevWorld &w, const McbSlot *d_slots, const McbRequest *d_reqs, const McbTile *d_tiles,
+                       int ntiles, uint8_t *d_out, int tile_w, int tile_h, int grid_blocks, hipStream_t stream);
+hipError_t launch_points(const DevWorld &w, const McbSlot *d_slots, const int32_t *d_slot_of, int ns,
+                         const int32_t *d_pts, int np, uint8_t *d_out, int32_t *d_tg, hipStream_t stream);
+hipError_t launch_zoom(const McbRequest *d_zreqs, const McbRequest *d_qreqs, const McbTile *d_tiles, int ntiles,
+                       const uint8_t *d_quart, uint8_t *d_out, hipStream_t stream);
+
+// ---- request lowering shared by plans and the server ----
+// Appends the kernel descriptors of one region: a direct quart grid (g) or, for zoom, the quart layers
+// (q) feeding a zoom request (z).  quart_off accumulates the zoom staging bytes.
+struct Lowered {
+    std::vector<McbRequest> g, q, z;
+    std::vector<McbTile> gt, qt, zt;
+    int64_t quart_bytes = 0, columns = 0, samples = 0;
+};
+int check_region(const McbRegion &r);
+void lower_region(const McbRegion &r, int slot, int tile_w, Lowered &L);
+
+}  // namespace mcb
+
+// Engine object behind the C ABI's opaque McbEngine.
+struct McbEngine {
+    mcb::DevWorld w;
+    int device = 0;
+    std::mutex mu;
+    // per-call scratch (grow-only), guarded by mu
+    McbSlot *d_slots = nullptr;
+    size_t slots_cap = 0;
+    int32_t *d_ident = nullptr;
+    size_t ident_cap = 0;
+    int64_t *d_seeds = nullptr;
+    size_t seeds_cap = 0;
+    int32_t *d_pts = nullptr;
+    size_t pts_cap = 0;
+    uint8_t *d_out = nullptr;
+    size_t out_cap = 0;
+    int32_t *d_tg = nullptr;
+    size_t tg_cap = 0;
+};
+
+#endif

@@ -1,0 +1,834 @@
+// mcb_kernels.hip - HIP kernels of the MI355X biome engine (gfx950, wave64).
+//
+// Precision follows the reference CPU engine exactly (engine/mc_noise.h, engine/mc_climate.h):
+// 26.1/26.2 noise in double, 26.3 noise with double coordinates and float arithmetic.  The build
+// passes -ffp-contract=off and no fast-math flags, so every multiply and add below is a separate
+// IEEE operation as in the reference; divisions and square roots are the correctly rounded ones.
+//
+// Kernels
+//   k_seed_ow / k_seed_ne / k_seed_end : per-seed noise state (Xoroshiro or legacy LCG octave init,
+//                                       Fisher-Yates permutations built in a per-lane LDS tile)
+//   k_grid<MODE,DIM>                   : quart-grid tiles of many requests; the seed's 46 permutation
+//                                       tables are staged in LDS, one column per lane, the climate of a
+//                                       column is computed once and shared by all its y layers
+//   k_points<MODE,DIM>                 : seeds x arbitrary quart points (the mcg_ref_points contract)
+//   k_zoom                             : BiomeManager block zoom over quart layers written by k_grid
+#include "mcb_internal.h"
+
+namespace mcb {
+namespace {
+
+// ================================================================================================
+// integer generators (engine/mc_rng.h:11-131)
+// ================================================================================================
+struct Xo { uint64_t a, b; };
+
+__device__ __forceinline__ uint64_t rotl64(uint64_t v, int k) { return (v << k) | (v >> (64 - k)); }
+__device__ __forceinline__ uint64_t mix13(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ Xo xo_make(uint64_t a, uint64_t b) {
+    Xo r{a, b};
+    if ((a | b) == 0) { r.a = 0x9E3779B97F4A7C15ULL; r.b = 0x6A09E667F3BCC909ULL; }
+    return r;
+}
+__device__ __forceinline__ Xo xo_from_seed(int64_t seed) {
+    uint64_t lo = (uint64_t)seed ^ 0x6A09E667F3BCC909ULL;
+    return xo_make(mix13(lo), mix13(lo + 0x9E3779B97F4A7C15ULL));
+}
+__device__ __forceinline__ uint64_t xo_next(Xo &r) {
+    uint64_t s0 = r.a, s1 = r.b;
+    uint64_t out = rotl64(s0 + s1, 17) + s0;
+    s1 ^= s0;
+    r.a = rotl64(s0, 49) ^ s1 ^ (s1 << 21);
+    r.b = rotl64(s1, 28);
+    return out;
+}
+__device__ __forceinline__ int xo_bound(Xo &r, uint32_t bound) {
+    uint64_t m = (uint64_t)(uint32_t)xo_next(r) * bound;
+    if ((uint32_t)m < bound) {
+        uint32_t thr = (0u - bound) % bound;
+        while ((uint32_t)m < thr) m = (uint64_t)(uint32_t)xo_next(r) * bound;
+    }
+    return (int)(m >> 32);
+}
+__device__ __forceinline__ double xo_double(Xo &r) { return (double)(xo_next(r) >> 11) * 1.1102230246251565e-16; }
+__device__ __forceinline__ Xo xo_fork(Xo &r) { Xo p; p.a = xo_next(r); p.b = xo_next(r); return p; }
+
+struct Lcg { uint64_t s; };
+#define LCG_MASK ((1ULL << 48) - 1)
+__device__ __forceinline__ int32_t lcg_bits(Lcg &r, int bits) {
+    r.s = (r.s * 0x5DEECE66DULL + 0xBULL) & LCG_MASK;
+    return (int32_t)(int64_t)(r.s >> (48 - bits));
+}
+__device__ __forceinline__ int lcg_bound(Lcg &r, int32_t bound) {
+    if ((bound & -bound) == bound) return (int32_t)(((int64_t)bound * (int64_t)lcg_bits(r, 31)) >> 31);
+    for (;;) {
+        int32_t bits = lcg_bits(r, 31), val = bits % bound;
+        if ((int32_t)((uint32_t)bits - (uint32_t)val + (uint32_t)(bound - 1)) >= 0) return val;
+    }
+}
+__device__ __forceinline__ double lcg_double(Lcg &r) {
+    int64_t hi = lcg_bits(r, 26), lo = lcg_bits(r, 27);
+    return (double)((hi << 27) + lo) * 1.1102230246251565e-16;
+}
+
+// ================================================================================================
+// per-lane permutation tile in LDS: byte i of lane L sits in word (i>>2)*64 + L, so every lane's
+// access hits its own bank column whatever i is (conflict-free Fisher-Yates).
+// ================================================================================================
+__device__ __forceinline__ uint8_t *tile_byte(uint32_t *tile, int lane, int i) {
+    return reinterpret_cast<uint8_t *>(tile + (i >> 2) * 64 + lane) + (i & 3);
+}
+
+template <class Draw>
+__device__ void fisher_yates(uint32_t *tile, int lane, Draw draw) {
+    for (int w = 0; w < 64; w++) tile[w * 64 + lane] = (uint32_t)(4 * w) | (uint32_t)(4 * w + 1) << 8 | (uint32_t)(4 * w + 2) << 16 | (uint32_t)(4 * w + 3) << 24;
+    for (int i = 0; i < 256; i++) {
+        int j = i + draw(256 - i);
+        uint8_t *pi = tile_byte(tile, lane, i), *pj = tile_byte(tile, lane, j);
+        uint8_t a = *pi, b = *pj;
+        *pi = b;
+        *pj = a;
+    }
+}
+
+__device__ __forceinline__ double fade_d(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+__device__ __forceinline__ float fade_f(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+
+// Write one octave (permutation + derived constants) of a seed slot.
+__device__ void store_octave(McbSlot *slot, int o, const uint32_t *tile, int lane, double xo, double yo, double zo) {
+    uint4 *dst = reinterpret_cast<uint4 *>(slot->perm[o]);
+    for (int q = 0; q < 16; q++)
+        dst[q] = make_uint4(tile[(4 * q) * 64 + lane], tile[(4 * q + 1) * 64 + lane], tile[(4 * q + 2) * 64 + lane],
+                            tile[(4 * q + 3) * 64 + lane]);
+    McbOct r;
+    r.xo = xo; r.yo = yo; r.zo = zo;
+    double yf = floor(yo), zf = floor(zo);
+    r.yi = (int)yf; r.zi = (int)zf;
+    r.fy_d = yo - yf; r.fz_d = zo - zf;
+    r.wy_d = fade_d(r.fy_d); r.wz_d = fade_d(r.fz_d);
+    r.fy_f = (float)(yo - yf); r.fz_f = (float)(zo - zf);
+    r.wy_f = fade_f(r.fy_f); r.wz_f = fade_f(r.fz_f);
+    slot->oct[o] = r;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Overworld seed state: one lane per (seed, octave).  RandomState: root = Xoroshiro(seed).forkPositional();
+// noise = root.fromHashOf("minecraft:<noise>"); each stack = noise.forkPositional(); octave =
+// stack.fromHashOf("octave_<n>")  (engine/mc_climate.h:97-106, engine/mc_noise.h:257-265).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_seed_ow(const McbWorld *__restrict__ W, const int64_t *__restrict__ seeds,
+                                               const int32_t *__restrict__ slot_of, int n, McbSlot *__restrict__ slots) {
+    __shared__ uint32_t tile[64 * 64];
+    const int lane = threadIdx.x;
+    const int n_oct = W->n_oct;
+    const int gid = blockIdx.x * 64 + lane;
+    if (gid >= n * n_oct) return;                        // no barriers below: early exit is safe
+    const int si = gid / n_oct, o = gid - si * n_oct;
+    int nn = 0;
+    while (nn + 1 < W->n_noise && o >= W->oct_base[nn + 1]) nn++;
+    const McbNoise &N = W->noise[nn];
+    const int rel = o - W->oct_base[nn], st = rel / N.nlev, k = rel - st * N.nlev;
+    const int64_t seed = seeds[si];
+    Xo root = xo_from_seed(seed);
+    Xo pos = xo_fork(root);
+    Xo r = xo_make(W->nlo[nn] ^ pos.a, W->nhi[nn] ^ pos.b);
+    Xo s0 = xo_fork(r);
+    Xo s1 = xo_fork(r);
+    Xo sp = st ? s1 : s0;
+    Xo rk = xo_make(N.hlo[k] ^ sp.a, N.hhi[k] ^ sp.b);
+    double xo = xo_double(rk) * 256.0, yo = xo_double(rk) * 256.0, zo = xo_double(rk) * 256.0;
+    fisher_yates(tile, lane, [&](int b) { return xo_bound(rk, (uint32_t)b); });
+    McbSlot *slot = slots + slot_of[si];
+    store_octave(slot, o, tile, lane, xo, yo, zo);
+    if (o == 0) slot->seed = seed;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Nether seed state: LegacyFbm from java.util.Random(seed + noise): per stack the zero-index octave is
+// drawn first (kept only if present), then indices zero-1 .. 0, absent ones skip 262 draws
+// (engine/mc_noise.h:267-282; skip by LCG jump).  One lane per (seed, noise).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_seed_ne(const McbWorld *__restrict__ W, const int64_t *__restrict__ seeds,
+                                               const int32_t *__restrict__ slot_of, int n, McbSlot *__restrict__ slots) {
+    __shared__ uint32_t tile[64 * 64];
+    const int lane = threadIdx.x;
+    const int gid = blockIdx.x * 64 + lane;
+    if (gid >= n * 2) return;
+    const int si = gid >> 1, nn = gid & 1;
+    const McbNoise &N = W->noise[nn];
+    const int64_t seed = seeds[si];
+    McbSlot *slot = slots + slot_of[si];
+    Lcg r{((uint64_t)(seed + nn) ^ 0x5DEECE66DULL) & LCG_MASK};
+    auto draw = [&](int b) { return lcg_bound(r, b); };
+    for (int st = 0; st < 2; st++) {
+        const int zero = -N.first;
+        int kzero = -1;
+        for (int k = 0; k < N.nlev; k++) if (N.lvl[k] == zero) kzero = k;
+        {
+            double xo = lcg_double(r) * 256.0, yo = lcg_double(r) * 256.0, zo = lcg_double(r) * 256.0;
+            fisher_yates(tile, lane, draw);
+            if (kzero >= 0 && zero < N.octs) store_octave(slot, W->oct_base[nn] + st * N.nlev + kzero, tile, lane, xo, yo, zo);
+        }
+        for (int i = zero - 1; i >= 0; i--) {
+            int kk = -1;
+            for (int k = 0; k < N.nlev; k++) if (N.lvl[k] == i) kk = k;
+            if (i < N.octs && kk >= 0) {
+                double xo = lcg_double(r) * 256.0, yo = lcg_double(r) * 256.0, zo = lcg_double(r) * 256.0;
+                fisher_yates(tile, lane, draw);
+                store_octave(slot, W->oct_base[nn] + st * N.nlev + kk, tile, lane, xo, yo, zo);
+            } else {
+                r.s = (r.s * W->jm262 + W->ja262) & LCG_MASK;
+            }
+        }
+    }
+    if (nn == 0) slot->seed = seed;
+}
+
+// End: java.util.Random(seed), consumeCount(17292), SimplexNoise permutation (engine/mc_end.h:18-27)
+__global__ __launch_bounds__(64) void k_seed_end(const McbWorld *__restrict__ W, const int64_t *__restrict__ seeds,
+                                                const int32_t *__restrict__ slot_of, int n, McbSlot *__restrict__ slots) {
+    __shared__ uint32_t tile[64 * 64];
+    const int lane = threadIdx.x;
+    const int gid = blockIdx.x * 64 + lane;
+    if (gid >= n) return;
+    const int64_t seed = seeds[gid];
+    Lcg r{((uint64_t)seed ^ 0x5DEECE66DULL) & LCG_MASK};
+    r.s = (r.s * W->jm_end + W->ja_end) & LCG_MASK;
+    double xo = lcg_double(r) * 256.0, yo = lcg_double(r) * 256.0, zo = lcg_double(r) * 256.0;
+    fisher_yates(tile, lane, [&](int b) { return lcg_bound(r, b); });
+    McbSlot *slot = slots + slot_of[gid];
+    store_octave(slot, 0, tile, lane, xo, yo, zo);
+    slot->seed = seed;
+}
+
+// ================================================================================================
+// octave evaluation (engine/mc_noise.h:54-110) on a permutation table in LDS
+// ================================================================================================
+#define PERM(i) ((int)P[(i) & 255])
+
+// gradient table of ImprovedNoise (16 x {gx,gy,gz}), each component a 2-bit two's complement field
+constexpr int kGrad[16][3] = {{1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
+                              {0, 1, 1},  {0, -1, 1}, {0, 1, -1}, {0, -1, -1}, {1, 1, 0},  {0, -1, 1}, {-1, 1, 0}, {0, -1, -1}};
+constexpr uint32_t grad_bits(int c) {
+    uint32_t v = 0;
+    for (int h = 0; h < 16; h++) v |= (uint32_t)(kGrad[h][c] & 3) << (2 * h);
+    return v;
+}
+constexpr uint32_t kGX = grad_bits(0), kGY = grad_bits(1), kGZ = grad_bits(2);
+__device__ __forceinline__ int gcomp(uint32_t bits, int h) { return __builtin_amdgcn_sbfe((int)bits, 2 * h, 2); }
+
+__device__ __forceinline__ float gdot_f(int h, float x, float y, float z) {
+    h &= 15;
+    return (float)gcomp(kGX, h) * x + (float)gcomp(kGY, h) * y + (float)gcomp(kGZ, h) * z;
+}
+__device__ __forceinline__ double gdot_d(int h, double x, double y, double z) {
+    h &= 15;
+    return (double)gcomp(kGX, h) * x + (double)gcomp(kGY, h) * y + (double)gcomp(kGZ, h) * z;
+}
+__device__ __forceinline__ float lerp_f(float t, float a, float b) { return a + t * (b - a); }
+__device__ __forceinline__ double lerp_d(double t, double a, double b) { return a + t * (b - a); }
+
+// PerlinNoise.wrap: identity inside +-2^24, else x - floor(x/2^25 + 0.5)*2^25 (mc_noise.h:77,86-89)
+__device__ __forceinline__ double wrapc(double x) {
+    if (__builtin_expect(!(fabs(x) < 16777216.0), 0)) x = x - floor(x * (1.0 / 33554432.0) + 0.5) * 33554432.0;
+    return x;
+}
+
+// Octave whose y coordinate is the octave offset (climate noises, ShiftA): x, z vary.
+template <int MODE> struct R;
+template <> struct R<1> { typedef float T; };
+template <> struct R<0> { typedef double T; };
+
+template <int MODE>
+__device__ __forceinline__ typename R<MODE>::T oct_yc(const uint8_t *P, const McbOct &o, double X, double Z) {
+    typedef typename R<MODE>::T T;
+    double x = wrapc(X) + o.xo, z = wrapc(Z) + o.zo;
+    double xf = floor(x), zf = floor(z);
+    int xi = (int)xf, zi = (int)zf, yi = o.yi;
+    T fx = (T)(x - xf), fz = (T)(z - zf);
+    T fy = MODE ? (T)o.fy_f : (T)o.fy_d, wy = MODE ? (T)o.wy_f : (T)o.wy_d;
+    int a = PERM(xi), b = PERM(xi + 1);
+    int aa = PERM(a + yi), ab = PERM(a + yi + 1), ba = PERM(b + yi), bb = PERM(b + yi + 1);
+    T c000, c100, c010, c110, c001, c101, c011, c111;
+    if (MODE) {
+        c000 = gdot_f(PERM(aa + zi), fx, fy, fz);
+        c100 = gdot_f(PERM(ba + zi), fx - 1.0f, fy, fz);
+        c010 = gdot_f(PERM(ab + zi), fx, fy - 1.0f, fz);
+        c110 = gdot_f(PERM(bb + zi), fx - 1.0f, fy - 1.0f, fz);
+        c001 = gdot_f(PERM(aa + zi + 1), fx, fy, fz - 1.0f);
+        c101 = gdot_f(PERM(ba + zi + 1), fx - 1.0f, fy, fz - 1.0f);
+        c011 = gdot_f(PERM(ab + zi + 1), fx, fy - 1.0f, fz - 1.0f);
+        c111 = gdot_f(PERM(bb + zi + 1), fx - 1.0f, fy - 1.0f, fz - 1.0f);
+        float u = fade_f(fx), w = fade_f(fz);
+        return lerp_f(w, lerp_f(wy, lerp_f(u, c000, c100), lerp_f(u, c010, c110)),
+                      lerp_f(wy, lerp_f(u, c001, c101), lerp_f(u, c011, c111)));
+    } else {
+        c000 = gdot_d(PERM(aa + zi), fx, fy, fz);
+        c100 = gdot_d(PERM(ba + zi), fx - 1.0, fy, fz);
+        c010 = gdot_d(PERM(ab + zi), fx, fy - 1.0, fz);
+        c110 = gdot_d(PERM(bb + zi), fx - 1.0, fy - 1.0, fz);
+        c001 = gdot_d(PERM(aa + zi + 1), fx, fy, fz - 1.0);
+        c101 = gdot_d(PERM(ba + zi + 1), fx - 1.0, fy, fz - 1.0);
+        c011 = gdot_d(PERM(ab + zi + 1), fx, fy - 1.0, fz - 1.0);
+        c111 = gdot_d(PERM(bb + zi + 1), fx - 1.0, fy - 1.0, fz - 1.0);
+        double u = fade_d(fx), w = fade_d(fz);
+        return lerp_d(w, lerp_d(wy, lerp_d(u, c000, c100), lerp_d(u, c010, c110)),
+                      lerp_d(wy, lerp_d(u, c001, c101), lerp_d(u, c011, c111)));
+    }
+}
+
+// Octave whose z coordinate is the octave offset (ShiftB): x, y vary.
+template <int MODE>
+__device__ __forceinline__ typename R<MODE>::T oct_zc(const uint8_t *P, const McbOct &o, double X, double Y) {
+    typedef typename R<MODE>::T T;
+    double x = wrapc(X) + o.xo, y = wrapc(Y) + o.yo;
+    double xf = floor(x), yf = floor(y);
+    int xi = (int)xf, yi = (int)yf, zi = o.zi;
+    T fx = (T)(x - xf), fy = (T)(y - yf);
+    T fz = MODE ? (T)o.fz_f : (T)o.fz_d, wz = MODE ? (T)o.wz_f : (T)o.wz_d;
+    int a = PERM(xi), b = PERM(xi + 1);
+    int aa = PERM(a + yi), ab = PERM(a + yi + 1), ba = PERM(b + yi), bb = PERM(b + yi + 1);
+    if (MODE) {
+        float c000 = gdot_f(PERM(aa + zi), fx, fy, fz);
+        float c100 = gdot_f(PERM(ba + zi), fx - 1.0f, fy, fz);
+        float c010 = gdot_f(PERM(ab + zi), fx, fy - 1.0f, fz);
+        float c110 = gdot_f(PERM(bb + zi), fx - 1.0f, fy - 1.0f, fz);
+        float c001 = gdot_f(PERM(aa + zi + 1), fx, fy, fz - 1.0f);
+        float c101 = gdot_f(PERM(ba + zi + 1), fx - 1.0f, fy, fz - 1.0f);
+        float c011 = gdot_f(PERM(ab + zi + 1), fx, fy - 1.0f, fz - 1.0f);
+        float c111 = gdot_f(PERM(bb + zi + 1), fx - 1.0f, fy - 1.0f, fz - 1.0f);
+        float u = fade_f(fx), v = fade_f(fy);
+        return lerp_f(wz, lerp_f(v, lerp_f(u, c000, c100), lerp_f(u, c010, c110)),
+                      lerp_f(v, lerp_f(u, c001, c101), lerp_f(u, c011, c111)));
+    } else {
+        double c000 = gdot_d(PERM(aa + zi), fx, fy, fz);
+        double c100 = gdot_d(PERM(ba + zi), fx - 1.0, fy, fz);
+        double c010 = gdot_d(PERM(ab + zi), fx, fy - 1.0, fz);
+        double c110 = gdot_d(PERM(bb + zi), fx - 1.0, fy - 1.0, fz);
+        double c001 = gdot_d(PERM(aa + zi + 1), fx, fy, fz - 1.0);
+        double c101 = gdot_d(PERM(ba + zi + 1), fx - 1.0, fy, fz - 1.0);
+        double c011 = gdot_d(PERM(ab + zi + 1), fx, fy - 1.0, fz - 1.0);
+        double c111 = gdot_d(PERM(bb + zi + 1), fx - 1.0, fy - 1.0, fz - 1.0);
+        double u = fade_d(fx), v = fade_d(fy);
+        return lerp_d(wz, lerp_d(v, lerp_d(u, c000, c100), lerp_d(u, c010, c110)),
+                      lerp_d(v, lerp_d(u, c001, c101), lerp_d(u, c011, c111)));
+    }
+}
+
+#define INPUT_FACTOR 1.0181268882175227
+
+// NormalNoise with y fixed at the octave offsets.  26.3 interleaves first/second layers and
+// accumulates in float (NoiseStack.get, mc_noise.h:303-312; legacy-nether order :313-321); 26.1/26.2
+// sums each stack in double, then (v1+v2)*valueFactor (mc_noise.h:286-301).
+template <int MODE, bool ZC>
+__device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm, const McbOct *oct, const McbNoise &N, int base,
+                                                       double X, double Y) {
+    const int n = N.nlev;
+    auto ev = [&](int o, double a, double b) {
+        return ZC ? oct_zc<MODE>(perm + o * 256, oct[o], a, b) : oct_yc<MODE>(perm + o * 256, oct[o], a, b);
+    };
+    if (MODE) {
+        float v = 0.0f;
+        if (!N.legacy) {
+            for (int k = 0; k < n; k++) {
+                double f = N.freq[k], f2 = f * INPUT_FACTOR;
+                v += N.amp_f[k] * ev(base + k, X * f, Y * f);
+                v += N.amp_f[k] * ev(base + n + k, X * f2, Y * f2);
+            }
+        } else {
+            for (int k = 0; k < n; k++) {
+                double f = N.freq[k];
+                v += N.amp_f[k] * ev(base + k, X * f, Y * f);
+            }
+            for (int k = 0; k < n; k++) {
+                double f2 = N.freq[k] * INPUT_FACTOR;
+                v += N.amp_f[k] * ev(base + n + k, X * f2, Y * f2);
+            }
+        }
+        return (typename R<MODE>::T)v;
+    } else {
+        double X2 = X * INPUT_FACTOR, Y2 = Y * INPUT_FACTOR;
+        double v1 = 0.0, v2 = 0.0;
+        for (int k = 0; k < n; k++) {
+            double f = N.freq[k];
+            v1 += N.amp_d[k] * (double)ev(base + k, wrapc(X * f), wrapc(Y * f)) * N.vf_d[k];
+        }
+        for (int k = 0; k < n; k++) {
+            double f = N.freq[k];
+            v2 += N.amp_d[k] * (double)ev(base + n + k, wrapc(X2 * f), wrapc(Y2 * f)) * N.vf_d[k];
+        }
+        return (typename R<MODE>::T)((v1 + v2) * N.value_factor);
+    }
+}
+
+// ================================================================================================
+// offset spline (engine/mc_climate.h:40-66), float
+// ================================================================================================
+// Iterative evaluation with a bounded frame stack (the offset splines nest 5 deep): phase 0 = enter
+// node, 1 = first child returned, 2 = second child returned.
+#define SPL_DEPTH 8
+__device__ float spline_eval(const McbSpline &S, int root, const float *in4) {
+    int nd[SPL_DEPTH], ph[SPL_DEPTH], iv[SPL_DEPTH];
+    float y1[SPL_DEPTH];
+    int sp = 0;
+    nd[0] = root;
+    ph[0] = 0;
+    float ret = 0.0f;
+    for (;;) {
+        const int id = nd[sp];
+        const int first = S.first[id], last = S.count[id] - 1;
+        const float x = in4[S.coord[id]];
+        if (ph[sp] == 0) {
+            if (S.kind[id] == 0) {
+                ret = S.value[id];
+                if (sp == 0) return ret;
+                sp--;
+                continue;
+            }
+            int lo = 0, hi = last + 1;                 // first i with x < loc[i], minus one
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (x < S.loc[first + mid]) hi = mid; else lo = mid + 1;
+            }
+            const int i = lo - 1;
+            iv[sp] = i;
+            ph[sp] = 1;
+            if (sp + 1 >= SPL_DEPTH) return __builtin_nanf("");
+            nd[sp + 1] = S.child[first + (i < 0 ? 0 : i)];
+            ph[sp + 1] = 0;
+            sp++;
+            continue;
+        }
+        const int i = iv[sp];
+        if (ph[sp] == 1) {
+            if (i < 0 || i == last) {                 // linear extension beyond the end points
+                const int k = i < 0 ? 0 : last;
+                const float d = S.der[first + k];
+                ret = d == 0.0f ? ret : ret + d * (x - S.loc[first + k]);
+                if (sp == 0) return ret;
+                sp--;
+                continue;
+            }
+            y1[sp] = ret;
+            ph[sp] = 2;
+            nd[sp + 1] = S.child[first + i + 1];
+            ph[sp + 1] = 0;
+            sp++;
+            continue;
+        }
+        const float ya = y1[sp], yb = ret;
+        const float x1 = S.loc[first + i], x2 = S.loc[first + i + 1];
+        const float t = (x - x1) / (x2 - x1);
+        const float a = S.der[first + i] * (x2 - x1) - (yb - ya);
+        const float b = -S.der[first + i + 1] * (x2 - x1) + (yb - ya);
+        ret = lerp_f(t, ya, yb) + t * (1.0f - t) * lerp_f(t, a, b);
+        if (sp == 0) return ret;
+        sp--;
+    }
+}
+
+// ================================================================================================
+// R-tree search (engine/mc_biomes.h:31-62), exact: the result is the leaf minimising
+// (fitness, pre-order index) -- the first minimum in the reference's child order.  The array is in
+// pre-order with escape links, so the walk needs no stack; a warm-start candidate (a nearby sample's
+// leaf) only tightens the initial bound and never changes the result.
+// ================================================================================================
+__device__ __forceinline__ double node_dist(const McbNode &nd, const int *t) {
+    double d = (double)nd.d6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        int above = t[k] - (int)nd.hi[k], below = (int)nd.lo[k] - t[k];
+        int x = max(max(above, below), 0);
+        double xd = (double)x;
+        d = fma(xd, xd, d);                     // integer-valued, exact (< 2^53)
+    }
+    return d;
+}
+__device__ __forceinline__ int tree_search(const McbNode *__restrict__ N, int n, const int *t, int &cand) {
+    int bi = n;
+    double bd = __builtin_inf();
+    if (cand > 0) { bi = cand; bd = node_dist(N[cand], t); }
+    int cur = 1;
+    while (cur < n) {
+        const McbNode nd = N[cur];
+        double cd = node_dist(nd, t);
+        bool better = cd < bd || (cd == bd && cur < bi);
+        bool leaf = nd.leaf != 0;
+        if (better && leaf) { bd = cd; bi = cur; }
+        cur = (better && !leaf) ? cur + 1 : (int)nd.escape;
+    }
+    cand = bi;
+    return N[bi].leaf & 0xFF;
+}
+
+__device__ __forceinline__ int quantize(float v) { return (int)(v * 10000.0f); }   // mc_climate.h:93
+
+// ================================================================================================
+// climate of one Overworld column (engine/mc_climate.h:113-164); depth is finished per layer
+// ================================================================================================
+struct Column {
+    int t, h, c, e, w;      // quantized temperature, humidity, continentalness, erosion, weirdness
+    float off_f;            // 26.3 offset (float)
+    double off_d;           // 26.1/26.2 offset (double)
+};
+
+template <int MODE>
+__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const uint8_t *perm, const McbOct *oct, int qx, int qz,
+                                          Column &c) {
+    const McbNoise *N = W->noise;
+    const double X = (double)qx, Z = (double)qz;     // blockX*0.25 with blockX = 4*quartX
+    if (MODE) {
+        float sx = normal2<1, false>(perm, oct, N[0], W->oct_base[0], X, Z) * 4.0f;
+        float sz = normal2<1, true>(perm, oct, N[0], W->oct_base[0], Z, X) * 4.0f;
+        double x = X + (double)sx, z = Z + (double)sz;
+        float T = normal2<1, false>(perm, oct, N[1], W->oct_base[1], x, z);
+        float H = normal2<1, false>(perm, oct, N[2], W->oct_base[2], x, z);
+        float C = normal2<1, false>(perm, oct, N[3], W->oct_base[3], x, z);
+        float E = normal2<1, false>(perm, oct, N[4], W->oct_base[4], x, z);
+        float Wr = normal2<1, false>(perm, oct, N[5], W->oct_base[5], x, z);
+        float a = fabsf(Wr) + W->rf_f[0];
+        float rf = (fabsf(a) + W->rf_f[1]) * W->rf_f[2];
+        float in4[4] = {C, E, Wr, rf};
+        c.off_f = W->offc_f + spline_eval(W->spline, 0, in4);
+        c.t = quantize(T); c.h = quantize(H); c.c = quantize(C); c.e = quantize(E); c.w = quantize(Wr);
+    } else {
+        double sx = normal2<0, false>(perm, oct, N[0], W->oct_base[0], X, Z) * 4.0;
+        double sz = normal2<0, true>(perm, oct, N[0], W->oct_base[0], Z, X) * 4.0;
+        double x = X + sx, z = Z + sz;
+        double T = normal2<0, false>(perm, oct, N[1], W->oct_base[1], x, z);
+        double H = normal2<0, false>(perm, oct, N[2], W->oct_base[2], x, z);
+        double C = normal2<0, false>(perm, oct, N[3], W->oct_base[3], x, z);
+        double E = normal2<0, false>(perm, oct, N[4], W->oct_base[4], x, z);
+        double Wr = normal2<0, false>(perm, oct, N[5], W->oct_base[5], x, z);
+        double rf = (fabs(fabs(Wr) + W->rf_d[0]) + W->rf_d[1]) * W->rf_d[2];
+        float in4[4] = {(float)C, (float)E, (float)Wr, (float)rf};
+        float sp = spline_eval(W->spline, 0, in4);
+        c.off_d = 0.0 + (W->offc_d + (double)sp);
+        c.t = quantize((float)T); c.h = quantize((float)H); c.c = quantize((float)C); c.e = quantize((float)E);
+        c.w = quantize((float)Wr);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ int ow_depth(const McbWorld *__restrict__ W, const Column &c, int qy) {
+    const int by = qy * 4;
+    if (MODE) {
+        int lo = min(W->gy0, W->gy1), hi = max(W->gy0, W->gy1);
+        int yy = by < lo ? lo : (by > hi ? hi : by);
+        float g = W->gv0_f + (float)(yy - W->gy0) * W->grad_factor_f;
+        return quantize(g + c.off_f);
+    } else {
+        double fac = ((double)by - (double)W->gy0) / ((double)W->gy1 - (double)W->gy0);
+        double g = fac < 0.0 ? W->gv0_d : (fac > 1.0 ? W->gv1_d : W->gv0_d + fac * (W->gv1_d - W->gv0_d));
+        return quantize((float)(g + c.off_d));
+    }
+}
+
+// Nether climate: two legacy NormalNoises at (bx*0.25, by*0.0, bz*0.25) (engine/cuda/mcg_host.c:63-74)
+template <int MODE>
+__device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, const uint8_t *perm, const McbOct *oct, int qx, int qz,
+                                          int *t) {
+    const double X = (double)qx, Z = (double)qz;
+    float T = (float)normal2<MODE, false>(perm, oct, W->noise[0], W->oct_base[0], X, Z);
+    float H = (float)normal2<MODE, false>(perm, oct, W->noise[1], W->oct_base[1], X, Z);
+    t[0] = quantize(T); t[1] = quantize(H);
+    t[2] = t[3] = t[4] = t[5] = 0;
+}
+
+// ================================================================================================
+// End islands (engine/mc_end.h:38-119)
+// ================================================================================================
+__device__ double simplex2(const uint8_t *P, double xin, double yin) {
+    const double SQ3 = 1.7320508075688772;
+    const double F2 = 0.5 * (SQ3 - 1.0), G2 = (3.0 - SQ3) / 6.0;
+    double s = (xin + yin) * F2;
+    int i = (int)floor(xin + s), j = (int)floor(yin + s);
+    double t = (double)(i + j) * G2;
+    double x0 = xin - ((double)i - t), y0 = yin - ((double)j - t);
+    int i1 = x0 > y0 ? 1 : 0, j1 = 1 - i1;
+    double x1 = x0 - (double)i1 + G2, y1 = y0 - (double)j1 + G2;
+    double x2 = x0 - 1.0 + 2.0 * G2, y2 = y0 - 1.0 + 2.0 * G2;
+    int ii = i & 255, jj = j & 255;
+    int g0 = PERM(ii + PERM(jj)) % 12, g1 = PERM(ii + i1 + PERM(jj + j1)) % 12, g2 = PERM(ii + 1 + PERM(jj + 1)) % 12;
+    double n0 = 0.0, n1 = 0.0, n2 = 0.0;
+    double t0 = 0.5 - x0 * x0 - y0 * y0 - 0.0 * 0.0;
+    if (!(t0 < 0.0)) { t0 *= t0; n0 = t0 * t0 * ((double)gcomp(kGX, g0) * x0 + (double)gcomp(kGY, g0) * y0 + (double)gcomp(kGZ, g0) * 0.0); }
+    double t1 = 0.5 - x1 * x1 - y1 * y1 - 0.0 * 0.0;
+    if (!(t1 < 0.0)) { t1 *= t1; n1 = t1 * t1 * ((double)gcomp(kGX, g1) * x1 + (double)gcomp(kGY, g1) * y1 + (double)gcomp(kGZ, g1) * 0.0); }
+    double t2 = 0.5 - x2 * x2 - y2 * y2 - 0.0 * 0.0;
+    if (!(t2 < 0.0)) { t2 *= t2; n2 = t2 * t2 * ((double)gcomp(kGX, g2) * x2 + (double)gcomp(kGY, g2) * y2 + (double)gcomp(kGZ, g2) * 0.0); }
+    return 70.0 * (n0 + n1 + n2);
+}
+__device__ __forceinline__ float clampf_ref(float v, float lo, float hi) { float m = v < hi ? v : hi; return v < lo ? lo : m; }
+
+template <int MODE>
+__device__ float end_height(const uint8_t *P, int sx, int sz) {
+    const int cx = sx / 2, cz = sz / 2, ox = sx % 2, oz = sz % 2;
+    float best;
+    if (!MODE) {
+        uint32_t r2 = (uint32_t)sx * (uint32_t)sx + (uint32_t)sz * (uint32_t)sz;
+        float len = (float)sqrt((double)(float)(int32_t)r2);
+        best = clampf_ref(100.0f - len * 8.0f, -100.0f, 80.0f);
+    } else best = -100.0f;
+    for (int dx = -12; dx <= 12; dx++)
+        for (int dz = -12; dz <= 12; dz++) {
+            int64_t tx = (int64_t)cx + dx, tz = (int64_t)cz + dz;
+            if (tx * tx + tz * tz <= 4096) continue;
+            double nv = simplex2(P, (double)tx, (double)tz);
+            bool island = MODE ? ((float)nv < -0.9f) : (nv < (double)-0.9f);
+            if (!island) continue;
+            float size = fmodf(fabsf((float)tx) * 3439.0f + fabsf((float)tz) * 147.0f, 13.0f) + 9.0f;
+            float xd = (float)(ox - dx * 2), zd = (float)(oz - dz * 2);
+            float h = 100.0f - (float)sqrt((double)(xd * xd + zd * zd)) * size;
+            h = clampf_ref(h, -100.0f, 80.0f);
+            best = best > h ? best : h;
+        }
+    return best;
+}
+template <int MODE>
+__device__ int end_biome(const McbWorld *__restrict__ W, const uint8_t *P, int qx, int qz) {
+    const int cx = qx >> 2, cz = qz >> 2;
+    if ((int64_t)cx * cx + (int64_t)cz * cz <= 4096) return W->end_ids[0];
+    const int bx = (cx * 2 + 1) * 8, bz = (cz * 2 + 1) * 8;
+    float hgt = end_height<MODE>(P, bx / 8, bz / 8);
+    double h;
+    if (!MODE) h = ((double)hgt - 8.0) / 128.0;
+    else {
+        float dx = (float)(0 - bx), dz = (float)(0 - bz);
+        float len = (float)sqrt((double)(dx * dx + 0.0f * 0.0f + dz * dz));
+        float c = (clampf_ref(100.0f - len, -100.0f, 80.0f) - 8.0f) * 0.0078125f;
+        float e = (hgt - 8.0f) / 128.0f;
+        h = (double)(c > e ? c : e);
+    }
+    if (h > 0.25) return W->end_ids[1];
+    if (h >= -0.0625) return W->end_ids[2];
+    return h < -0.21875 ? W->end_ids[3] : W->end_ids[4];
+}
+
+// ================================================================================================
+// LDS staging of a seed slot's permutation tables
+// ================================================================================================
+__device__ __forceinline__ void stage_perm(uint8_t *lds, const McbSlot *slot, int n_oct) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(slot->perm[0]);
+    uint4 *dst = reinterpret_cast<uint4 *>(lds);
+    const int n16 = n_oct * 16;
+    for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+}
+
+// ================================================================================================
+// grid kernel: tiles of quart-grid requests of one world
+// ================================================================================================
+template <int MODE, int DIM>
+__global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                              const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
+                                              const McbTile *__restrict__ tiles, int ntiles, int tile_w, uint8_t *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_SLOT_PERM_BYTES];
+    const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
+    const int tree_n = W->tree_n;
+    int staged = -1;
+    // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables
+    const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t_end = min(ntiles, ((int)blockIdx.x + 1) * per);
+    for (int ti = (int)blockIdx.x * per; ti < t_end; ti++) {
+        const McbTile tl = tiles[ti];
+        const McbRequest &R = reqs[tl.req];
+        const int slot = R.slot;
+        if (slot != staged) {
+            __syncthreads();
+            stage_perm(perm, slots + slot, n_oct);
+            __syncthreads();
+            staged = slot;
+        }
+        const McbOct *oct = slots[slot].oct;
+        const int tile_h = 256 / tile_w;
+        const int ix = tl.tx * tile_w + (int)threadIdx.x % tile_w;
+        const int iz = tl.tz * tile_h + (int)threadIdx.x / tile_w;
+        if (ix >= R.nx || iz >= R.nz) continue;
+        const int qx = R.x0 + ix, qz = R.z0 + iz;
+        const int64_t plane = (int64_t)R.nx * R.nz;
+        uint8_t *o = out + R.out_off + (int64_t)iz * R.nx + ix;
+        if (DIM == MCB_OVERWORLD) {
+            Column c;
+            ow_column<MODE>(W, perm, oct, qx, qz, c);
+            int t[6] = {c.t, c.h, c.c, c.e, 0, c.w};
+            int cand = -1;
+            for (int j = 0; j < R.ny; j++) {
+                t[4] = ow_depth<MODE>(W, c, R.y[j]);
+                o[j * plane] = (uint8_t)tree_search(tree, tree_n, t, cand);
+            }
+        } else if (DIM == MCB_NETHER) {
+            int t[6];
+            ne_column<MODE>(W, perm, oct, qx, qz, t);
+            int cand = -1;
+            uint8_t b = (uint8_t)tree_search(tree, tree_n, t, cand);
+            for (int j = 0; j < R.ny; j++) o[j * plane] = b;
+        } else {
+            uint8_t b = (uint8_t)end_biome<MODE>(W, perm, qx, qz);
+            for (int j = 0; j < R.ny; j++) o[j * plane] = b;
+        }
+    }
+}
+
+// ================================================================================================
+// points kernel: seeds x quart points (mcg_ref_points contract, engine/cuda/mcg_host.c:36-83)
+// ================================================================================================
+template <int MODE, int DIM>
+__global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                                const McbSlot *__restrict__ slots, const int32_t *__restrict__ slot_of,
+                                                const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
+                                                int32_t *__restrict__ tg) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_SLOT_PERM_BYTES];
+    const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
+    const int s = blockIdx.y;
+    const int slot = slot_of[s];
+    stage_perm(perm, slots + slot, n_oct);
+    __syncthreads();
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= np) return;
+    const McbOct *oct = slots[slot].oct;
+    const int qx = pts[3 * p], qy = pts[3 * p + 1], qz = pts[3 * p + 2];
+    const size_t o = (size_t)s * np + p;
+    int t[6] = {0, 0, 0, 0, 0, 0};
+    int b;
+    if (DIM == MCB_OVERWORLD) {
+        Column c;
+        ow_column<MODE>(W, perm, oct, qx, qz, c);
+        t[0] = c.t; t[1] = c.h; t[2] = c.c; t[3] = c.e; t[5] = c.w;
+        t[4] = ow_depth<MODE>(W, c, qy);
+        int cand = -1;
+        b = tree_search(tree, W->tree_n, t, cand);
+    } else if (DIM == MCB_NETHER) {
+        ne_column<MODE>(W, perm, oct, qx, qz, t);
+        int cand = -1;
+        b = tree_search(tree, W->tree_n, t, cand);
+    } else {
+        b = end_biome<MODE>(W, perm, qx, qz);
+    }
+    out[o] = (uint8_t)b;
+    if (tg)
+        for (int k = 0; k < 6; k++) tg[o * 6 + k] = t[k];
+}
+
+// ================================================================================================
+// BiomeManager zoom (engine/cuda/mc_gpu.cuh:96-118, libmcgen/src/region.c:62-85): block -> one of the
+// 8 surrounding quart cells by the smallest fiddled distance; reads the quart layers of its request.
+// ================================================================================================
+__device__ __forceinline__ int64_t zstep(int64_t r, int64_t c) {
+    uint64_t v = (uint64_t)r;
+    return (int64_t)(v * (v * 6364136223846793005ULL + 1442695040888963407ULL) + (uint64_t)c);
+}
+__device__ __forceinline__ double zfiddle(int64_t r) { return ((double)((r >> 24) & 1023) / 1024.0 - 0.5) * 0.9; }
+
+__global__ __launch_bounds__(256) void k_zoom(const McbRequest *__restrict__ zreqs, const McbRequest *__restrict__ qreqs,
+                                              const McbTile *__restrict__ tiles, int ntiles, const uint8_t *__restrict__ quart,
+                                              uint8_t *__restrict__ out) {
+    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+        const McbTile tl = tiles[ti];
+        const McbRequest &Z = zreqs[tl.req];
+        const McbRequest &Q = qreqs[tl.req];
+        const int ix = tl.tx * 16 + (int)threadIdx.x % 16, iz = tl.tz * 16 + (int)threadIdx.x / 16;
+        if (ix >= Z.nx || iz >= Z.nz) continue;
+        const int64_t hs = Z.hashed_seed;
+        const int x = Z.x0 + ix, z = Z.z0 + iz;
+        const int ax = x - 2, az = z - 2;
+        const int px = ax >> 2, pz = az >> 2;
+        const double fx = (double)(ax & 3) / 4.0, fz = (double)(az & 3) / 4.0;
+        const int64_t qplane = (int64_t)Q.nx * Q.nz;
+        for (int j = 0; j < Z.ny; j++) {
+            const int ay = Z.y[j] - 2, py = ay >> 2;
+            const double fy = (double)(ay & 3) / 4.0;
+            int best = 0;
+            double bd = __builtin_inf();
+            for (int c = 0; c < 8; c++) {
+                const int ux = (c >> 2) & 1, uy = (c >> 1) & 1, uz = c & 1;
+                const int64_t cx = px + ux, cy = py + uy, cz = pz + uz;
+                double dx = ux ? fx - 1.0 : fx, dy = uy ? fy - 1.0 : fy, dz = uz ? fz - 1.0 : fz;
+                int64_t r = hs;
+                r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
+                r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
+                double ox = zfiddle(r);
+                r = zstep(r, hs);
+                double oy = zfiddle(r);
+                r = zstep(r, hs);
+                double oz = zfiddle(r);
+                double d = (dz + oz) * (dz + oz) + (dy + oy) * (dy + oy) + (dx + ox) * (dx + ox);
+                if (bd > d) { best = c; bd = d; }
+            }
+            const int qxi = px + ((best >> 2) & 1) - Q.x0, qzi = pz + (best & 1) - Q.z0;
+            const int layer = 2 * j + ((best >> 1) & 1);
+            out[Z.out_off + j * (int64_t)Z.nx * Z.nz + (int64_t)iz * Z.nx + ix] =
+                quart[Q.out_off + layer * qplane + (int64_t)qzi * Q.nx + qxi];
+        }
+    }
+}
+
+template <int MODE, int DIM>
+hipError_t grid_launch(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
+                       uint8_t *out, int tile_w, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_grid<MODE, DIM>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, slots, reqs, tiles, ntiles,
+                       tile_w, out);
+    return hipGetLastError();
+}
+template <int MODE, int DIM>
+hipError_t points_launch(const DevWorld &w, const McbSlot *slots, const int32_t *slot_of, int ns, const int32_t *pts, int np,
+                         uint8_t *out, int32_t *tg, hipStream_t st) {
+    hipLaunchKernelGGL((k_points<MODE, DIM>), dim3((np + 255) / 256, ns), dim3(256), 0, st, w.d_world, w.d_tree, slots,
+                       slot_of, pts, np, out, tg);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_seed_init(const DevWorld &w, const int64_t *seeds, const int32_t *slot_of, int n, McbSlot *slots,
+                            hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    int lanes = w.dim == MCB_OVERWORLD ? n * w.host.n_oct : (w.dim == MCB_NETHER ? n * 2 : n);
+    dim3 grid((lanes + 63) / 64);
+    if (w.dim == MCB_OVERWORLD) hipLaunchKernelGGL(k_seed_ow, grid, dim3(64), 0, st, w.d_world, seeds, slot_of, n, slots);
+    else if (w.dim == MCB_NETHER) hipLaunchKernelGGL(k_seed_ne, grid, dim3(64), 0, st, w.d_world, seeds, slot_of, n, slots);
+    else hipLaunchKernelGGL(k_seed_end, grid, dim3(64), 0, st, w.d_world, seeds, slot_of, n, slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_grid(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
+                       uint8_t *out, int tile_w, int tile_h, int blocks, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    if (tile_w * tile_h != 256 || tile_w <= 0) return hipErrorInvalidValue;
+    if (blocks <= 0) blocks = ntiles;
+    const bool f = w.host.fmt_new != 0;
+    switch (w.dim) {
+        case MCB_OVERWORLD: return f ? grid_launch<1, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st)
+                                     : grid_launch<0, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
+        case MCB_NETHER: return f ? grid_launch<1, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st)
+                                  : grid_launch<0, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
+        default: return f ? grid_launch<1, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st)
+                          : grid_launch<0, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
+    }
+}
+
+hipError_t launch_points(const DevWorld &w, const McbSlot *slots, const int32_t *slot_of, int ns, const int32_t *pts, int np,
+                         uint8_t *out, int32_t *tg, hipStream_t st) {
+    if (ns <= 0 || np <= 0) return hipSuccess;
+    const bool f = w.host.fmt_new != 0;
+    switch (w.dim) {
+        case MCB_OVERWORLD: return f ? points_launch<1, MCB_OVERWORLD>(w, slots, slot_of, ns, pts, np, out, tg, st)
+                                     : points_launch<0, MCB_OVERWORLD>(w, slots, slot_of, ns, pts, np, out, tg, st);
+        case MCB_NETHER: return f ? points_launch<1, MCB_NETHER>(w, slots, slot_of, ns, pts, np, out, tg, st)
+                                  : points_launch<0, MCB_NETHER>(w, slots, slot_of, ns, pts, np, out, tg, st);
+        default: return f ? points_launch<1, MCB_END>(w, slots, slot_of, ns, pts, np, out, tg, st)
+                          : points_launch<0, MCB_END>(w, slots, slot_of, ns, pts, np, out, tg, st);
+    }
+}
+
+hipError_t launch_zoom(const McbRequest *zreqs, const McbRequest *qreqs, const McbTile *tiles, int ntiles, const uint8_t *quart,
+                       uint8_t *out, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zoom, dim3(ntiles), dim3(256), 0, st, zreqs, qreqs, tiles, ntiles, quart, out);
+    return hipGetLastError();
+}
+
+}  // namespace mcb

@@ -1,0 +1,459 @@
+// mcb_serve.cpp - serving loop: continuous batching of region requests on one device.
+//
+// Requests (McbRegion of any world of a context) are queued by mcb_server_submit.  A dispatcher thread
+// forms a batch whenever one of `lanes` in-flight batch slots is free: it admits queued requests in
+// arrival order while their output and zoom-staging bytes fit the lane's device arena (memory-based
+// admission), looks their seeds up in a per-world device LRU of seed-state slots (only misses are
+// initialised), lowers the batch into kernel descriptors and launches it on the lane's stream
+// (seed state for misses, persistent quart-grid launch, zoom pass, copy-out).  Batches on different
+// lanes overlap on the device; a completion thread copies results to the callers' buffers and
+// records per-request latency (submit -> result in host memory).
+//
+// The nearest precedent in the reference is the pinned-slot terrain producer of
+// libmcgen/src/gpu_bridge.c:605-760; the reference has no request server for biomes.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "mcb_internal.h"
+
+using namespace mcb;
+
+struct McbContext;    // defined in mcb_plan.cpp
+extern "C" int mcb_context_world_engine(McbContext *c, int world, McbEngine **e);
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+    McbRegion r;
+    uint8_t *host_out;
+    int64_t ticket;
+    Clock::time_point t_submit;
+    int64_t out_bytes, stage_bytes;
+};
+
+// Device LRU of seed-state slots for one world.
+struct SlotPool {
+    McbEngine *e = nullptr;
+    McbSlot *d_slots = nullptr;
+    int cap = 0;
+    std::unordered_map<int64_t, int> slot_of;
+    std::vector<int64_t> seed_of;
+    std::vector<int> pins;
+    std::vector<int> init_lane;                           // lane whose batch initialised the slot
+    std::vector<int64_t> init_batch;
+    std::list<int> lru;                                   // front = most recent
+    std::vector<std::list<int>::iterator> pos;
+    int64_t hits = 0, misses = 0;
+
+    int init(McbEngine *eng, int capacity) {
+        e = eng;
+        cap = capacity;
+        if (hipMalloc((void **)&d_slots, (size_t)cap * sizeof(McbSlot)) != hipSuccess) return fail(MCB_E_NOMEM, "slot pool");
+        seed_of.assign(cap, 0);
+        pins.assign(cap, 0);
+        init_lane.assign(cap, -1);
+        init_batch.assign(cap, -1);
+        pos.resize(cap);
+        for (int i = 0; i < cap; i++) { lru.push_back(i); pos[i] = std::prev(lru.end()); seed_of[i] = INT64_MIN; }
+        return 0;
+    }
+    // slot for seed; *fresh = 1 when it must be initialised.  -1 when every slot is pinned.
+    int acquire(int64_t seed, int *fresh) {
+        auto it = slot_of.find(seed);
+        if (it != slot_of.end() && seed_of[it->second] == seed) {
+            int s = it->second;
+            lru.splice(lru.begin(), lru, pos[s]);
+            pins[s]++;
+            hits++;
+            *fresh = 0;
+            return s;
+        }
+        for (auto r = lru.rbegin(); r != lru.rend(); ++r) {
+            int s = *r;
+            if (pins[s]) continue;
+            if (seed_of[s] != INT64_MIN) slot_of.erase(seed_of[s]);
+            seed_of[s] = seed;
+            slot_of[seed] = s;
+            lru.splice(lru.begin(), lru, pos[s]);
+            pins[s]++;
+            misses++;
+            *fresh = 1;
+            return s;
+        }
+        return -1;
+    }
+    void release(int s) { if (pins[s] > 0) pins[s]--; }
+};
+
+struct Lane {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    hipEvent_t inited = nullptr;                          // after this lane's seed-state kernels
+    int64_t batch_id = 0;
+    uint8_t *d_out = nullptr, *d_quart = nullptr, *h_out = nullptr;
+    char *d_desc = nullptr, *h_desc = nullptr;
+    int64_t desc_cap = 0;
+    std::vector<Req> reqs;
+    std::vector<std::pair<int, int>> pinned;              // (world, slot)
+    bool busy = false;
+};
+}  // namespace
+
+struct McbServer {
+    McbContext *ctx = nullptr;
+    int64_t batch_bytes = 0;
+    int tile_w = 16;
+    std::vector<Lane> lanes;
+    std::unordered_map<int, SlotPool> pools;
+    int slot_capacity = 0;
+    std::mutex mu;
+    std::condition_variable cv_submit, cv_done, cv_lane;
+    std::deque<Req> pending;
+    std::deque<int> inflight;                              // lane indices in launch order
+    std::unordered_map<int64_t, std::pair<int, double>> results;   // ticket -> (status, latency ms)
+    std::vector<double> latencies;
+    int64_t next_ticket = 1;
+    bool stop = false;
+    std::thread dispatcher, completer;
+    std::string fatal;
+    int64_t batches = 0, admitted = 0;
+};
+
+namespace {
+
+int64_t region_out_bytes(const McbRegion &r) { return (int64_t)r.ny * r.nx * r.nz; }
+int64_t region_stage_bytes(const McbRegion &r) {
+    if (!r.zoom) return 0;
+    const int64_t qx = (((r.x0 + r.nx - 1 - 2) >> 2) + 1) - ((r.x0 - 2) >> 2) + 1;
+    const int64_t qz = (((r.z0 + r.nz - 1 - 2) >> 2) + 1) - ((r.z0 - 2) >> 2) + 1;
+    return qx * qz * 2 * r.ny;
+}
+
+// launch one batch on lane L; called with the server lock NOT held
+int launch_batch(McbServer *S, Lane &L) {
+    // group by world
+    std::vector<int> worlds;
+    for (auto &q : L.reqs)
+        if (std::find(worlds.begin(), worlds.end(), q.r.world) == worlds.end()) worlds.push_back(q.r.world);
+    int64_t out_off = 0, quart_off = 0;
+    std::vector<int64_t> req_out(L.reqs.size());
+    // descriptor staging layout per world: seeds, slot ids, greqs, gtiles, qreqs, qtiles, zreqs, ztiles
+    struct Seg { int64_t off, n; };
+    struct WorldBatch {
+        int world;
+        SlotPool *pool;
+        std::vector<int64_t> init_seeds;
+        std::vector<int32_t> init_slots;
+        Lowered L;
+        Seg seeds, slots, g, gt, q, qt, z, zt;
+    };
+    std::vector<WorldBatch> wb(worlds.size());
+    std::vector<int> wait_lanes;
+    const int my_lane = (int)(&L - &S->lanes[0]);
+    for (size_t wi = 0; wi < worlds.size(); wi++) {
+        WorldBatch &B = wb[wi];
+        B.world = worlds[wi];
+        std::unique_lock<std::mutex> lk(S->mu);
+        B.pool = &S->pools[B.world];
+        if (!B.pool->e) {
+            McbEngine *e = nullptr;
+            if (int rc = mcb_context_world_engine(S->ctx, B.world, &e)) return rc;
+            if (int rc = B.pool->init(e, S->slot_capacity)) return rc;
+        }
+        for (size_t i = 0; i < L.reqs.size(); i++) {
+            Req &q = L.reqs[i];
+            if (q.r.world != B.world) continue;
+            int fresh = 0;
+            int slot = B.pool->acquire(q.r.seed, &fresh);
+            if (slot < 0) return fail(MCB_E_LIMIT, "seed slot pool exhausted");
+            L.pinned.push_back({B.world, slot});
+            if (fresh) {
+                B.init_seeds.push_back(q.r.seed);
+                B.init_slots.push_back(slot);
+                B.pool->init_lane[slot] = my_lane;
+                B.pool->init_batch[slot] = L.batch_id;
+            } else {
+                // state written by a batch still in flight on another lane: order after its init kernels
+                const int il = B.pool->init_lane[slot];
+                if (il >= 0 && il != my_lane && S->lanes[il].busy && S->lanes[il].batch_id == B.pool->init_batch[slot] &&
+                    std::find(wait_lanes.begin(), wait_lanes.end(), il) == wait_lanes.end())
+                    wait_lanes.push_back(il);
+            }
+            McbRegion r = q.r;
+            r.out_off = out_off;
+            req_out[i] = out_off;
+            out_off += q.out_bytes;
+            const size_t g0 = B.L.g.size(), q0 = B.L.q.size(), z0 = B.L.z.size();
+            Lowered one;
+            one.quart_bytes = quart_off;          // zoom staging offsets run across the whole lane
+            lower_region(r, slot, S->tile_w, one);
+            quart_off = one.quart_bytes;
+            for (auto t : one.gt) { t.req += (int)g0; B.L.gt.push_back(t); }
+            for (auto t : one.qt) { t.req += (int)q0; B.L.qt.push_back(t); }
+            for (auto t : one.zt) { t.req += (int)z0; B.L.zt.push_back(t); }
+            B.L.g.insert(B.L.g.end(), one.g.begin(), one.g.end());
+            B.L.q.insert(B.L.q.end(), one.q.begin(), one.q.end());
+            B.L.z.insert(B.L.z.end(), one.z.begin(), one.z.end());
+        }
+    }
+    for (int il : wait_lanes) MCB_HIPCHK(hipStreamWaitEvent(L.st, S->lanes[il].inited, 0));
+    // pack descriptors into the lane's pinned staging, one H2D copy
+    int64_t off = 0;
+    auto put = [&](Seg &s, const void *src, int64_t bytes, int64_t n) {
+        off = (off + 15) & ~(int64_t)15;
+        s.off = off;
+        s.n = n;
+        if (off + bytes <= L.desc_cap && bytes) memcpy(L.h_desc + off, src, (size_t)bytes);
+        off += bytes;
+    };
+    for (auto &B : wb) {
+        put(B.seeds, B.init_seeds.data(), (int64_t)B.init_seeds.size() * 8, (int64_t)B.init_seeds.size());
+        put(B.slots, B.init_slots.data(), (int64_t)B.init_slots.size() * 4, (int64_t)B.init_slots.size());
+        put(B.g, B.L.g.data(), (int64_t)B.L.g.size() * sizeof(McbRequest), (int64_t)B.L.g.size());
+        put(B.gt, B.L.gt.data(), (int64_t)B.L.gt.size() * sizeof(McbTile), (int64_t)B.L.gt.size());
+        put(B.q, B.L.q.data(), (int64_t)B.L.q.size() * sizeof(McbRequest), (int64_t)B.L.q.size());
+        put(B.qt, B.L.qt.data(), (int64_t)B.L.qt.size() * sizeof(McbTile), (int64_t)B.L.qt.size());
+        put(B.z, B.L.z.data(), (int64_t)B.L.z.size() * sizeof(McbRequest), (int64_t)B.L.z.size());
+        put(B.zt, B.L.zt.data(), (int64_t)B.L.zt.size() * sizeof(McbTile), (int64_t)B.L.zt.size());
+    }
+    if (off > L.desc_cap) return fail(MCB_E_LIMIT, "batch descriptors exceed %lld bytes", (long long)L.desc_cap);
+    MCB_HIPCHK(hipMemcpyAsync(L.d_desc, L.h_desc, (size_t)off, hipMemcpyHostToDevice, L.st));
+    auto dp = [&](const Seg &s) { return (void *)(L.d_desc + s.off); };
+    for (auto &B : wb)
+        MCB_HIPCHK(launch_seed_init(B.pool->e->w, (const int64_t *)dp(B.seeds), (const int32_t *)dp(B.slots), (int)B.seeds.n,
+                                    B.pool->d_slots, L.st));
+    MCB_HIPCHK(hipEventRecord(L.inited, L.st));
+    for (auto &B : wb) {
+        const DevWorld &w = B.pool->e->w;
+        MCB_HIPCHK(launch_grid(w, B.pool->d_slots, (const McbRequest *)dp(B.g), (const McbTile *)dp(B.gt), (int)B.gt.n,
+                               L.d_out, S->tile_w, 256 / S->tile_w, (int)B.gt.n, L.st));
+        MCB_HIPCHK(launch_grid(w, B.pool->d_slots, (const McbRequest *)dp(B.q), (const McbTile *)dp(B.qt), (int)B.qt.n,
+                               L.d_quart, S->tile_w, 256 / S->tile_w, (int)B.qt.n, L.st));
+        MCB_HIPCHK(launch_zoom((const McbRequest *)dp(B.z), (const McbRequest *)dp(B.q), (const McbTile *)dp(B.zt),
+                               (int)B.zt.n, L.d_quart, L.d_out, L.st));
+    }
+    if (out_off) MCB_HIPCHK(hipMemcpyAsync(L.h_out, L.d_out, (size_t)out_off, hipMemcpyDeviceToHost, L.st));
+    MCB_HIPCHK(hipEventRecord(L.done, L.st));
+    for (size_t i = 0; i < L.reqs.size(); i++) L.reqs[i].r.out_off = req_out[i];
+    return 0;
+}
+
+void dispatcher_main(McbServer *S) {
+    for (;;) {
+        int li = -1;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_submit.wait(lk, [&] {
+                if (S->stop) return true;
+                if (S->pending.empty()) return false;
+                for (auto &L : S->lanes) if (!L.busy) return true;
+                return false;
+            });
+            if (S->stop && S->pending.empty()) return;
+            if (S->stop) return;
+            for (size_t i = 0; i < S->lanes.size(); i++) if (!S->lanes[i].busy) { li = (int)i; break; }
+            Lane &L = S->lanes[li];
+            L.reqs.clear();
+            L.pinned.clear();
+            int64_t out = 0, stage = 0;
+            // memory-based admission: arrival order, while the lane arena holds the outputs and staging
+            while (!S->pending.empty()) {
+                Req &q = S->pending.front();
+                if (!L.reqs.empty() && (out + q.out_bytes > S->batch_bytes || stage + q.stage_bytes > S->batch_bytes)) break;
+                out += q.out_bytes;
+                stage += q.stage_bytes;
+                L.reqs.push_back(q);
+                S->pending.pop_front();
+            }
+            L.busy = true;
+            L.batch_id = ++S->batches;
+            S->admitted += (int64_t)L.reqs.size();
+        }
+        Lane &L = S->lanes[li];
+        int rc = launch_batch(S, L);
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (rc) {
+            (void)hipStreamSynchronize(L.st);
+            S->fatal = mcb_last_error();
+            for (auto &q : L.reqs) S->results[q.ticket] = {rc, 0.0};
+            for (auto &p : L.pinned) S->pools[p.first].release(p.second);
+            L.busy = false;
+            S->cv_done.notify_all();
+            continue;
+        }
+        S->inflight.push_back(li);
+        S->cv_lane.notify_all();
+    }
+}
+
+void completer_main(McbServer *S) {
+    for (;;) {
+        int li;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_lane.wait(lk, [&] { return S->stop || !S->inflight.empty(); });
+            if (S->inflight.empty()) return;
+            li = S->inflight.front();
+        }
+        Lane &L = S->lanes[li];
+        hipError_t e = hipEventSynchronize(L.done);
+        const auto now = Clock::now();
+        if (e == hipSuccess)
+            for (auto &q : L.reqs) memcpy(q.host_out, L.h_out + q.r.out_off, (size_t)q.out_bytes);
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->inflight.pop_front();
+        for (auto &q : L.reqs) {
+            double ms = std::chrono::duration<double, std::milli>(now - q.t_submit).count();
+            S->results[q.ticket] = {e == hipSuccess ? 0 : (int)MCB_E_DEVICE, ms};
+            if (e == hipSuccess) S->latencies.push_back(ms);
+        }
+        for (auto &p : L.pinned) S->pools[p.first].release(p.second);
+        L.busy = false;
+        S->cv_done.notify_all();
+        S->cv_submit.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcb_server_create(McbServer **out, McbContext *ctx, int64_t batch_bytes, int lanes, int slot_capacity) {
+    if (!out || !ctx) return fail(MCB_E_ARG, "bad arguments");
+    *out = nullptr;
+    if (batch_bytes <= 0) batch_bytes = 64ll << 20;
+    if (lanes <= 0) lanes = 3;
+    if (slot_capacity <= 0) slot_capacity = 16384;
+    std::unique_ptr<McbServer> S(new McbServer());
+    S->ctx = ctx;
+    S->batch_bytes = batch_bytes;
+    S->slot_capacity = slot_capacity;
+    S->lanes.resize(lanes);
+    for (auto &L : S->lanes) {
+        // descriptors: generous bound for requests and 16x16 tiles of a full arena
+        L.desc_cap = (batch_bytes / 256 + 4096) * (int64_t)sizeof(McbTile) * 2 + (64ll << 20);
+        MCB_HIPCHK(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking));
+        MCB_HIPCHK(hipEventCreateWithFlags(&L.done, hipEventDisableTiming));
+        MCB_HIPCHK(hipEventCreateWithFlags(&L.inited, hipEventDisableTiming));
+        MCB_HIPCHK(hipMalloc((void **)&L.d_out, (size_t)batch_bytes));
+        MCB_HIPCHK(hipMalloc((void **)&L.d_quart, (size_t)batch_bytes));
+        MCB_HIPCHK(hipMalloc((void **)&L.d_desc, (size_t)L.desc_cap));
+        MCB_HIPCHK(hipHostMalloc((void **)&L.h_out, (size_t)batch_bytes, hipHostMallocDefault));
+        MCB_HIPCHK(hipHostMalloc((void **)&L.h_desc, (size_t)L.desc_cap, hipHostMallocDefault));
+    }
+    McbServer *s = S.release();
+    s->dispatcher = std::thread(dispatcher_main, s);
+    s->completer = std::thread(completer_main, s);
+    *out = s;
+    return MCB_OK;
+}
+
+int mcb_server_submit(McbServer *S, const McbRegion *r, uint8_t *host_out, int64_t *ticket) {
+    if (!S || !r || !host_out) return fail(MCB_E_ARG, "bad arguments");
+    if (int rc = check_region(*r)) return rc;
+    Req q;
+    q.r = *r;
+    q.host_out = host_out;
+    q.out_bytes = region_out_bytes(*r);
+    q.stage_bytes = region_stage_bytes(*r);
+    if (q.out_bytes > S->batch_bytes || q.stage_bytes > S->batch_bytes) return fail(MCB_E_LIMIT, "request larger than a batch arena");
+    q.t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        q.ticket = S->next_ticket++;
+        S->pending.push_back(q);
+    }
+    if (ticket) *ticket = q.ticket;
+    S->cv_submit.notify_one();
+    return MCB_OK;
+}
+
+int mcb_server_wait(McbServer *S, int64_t ticket, double *latency_ms) {
+    if (!S) return fail(MCB_E_ARG, "null server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_done.wait(lk, [&] { return S->results.count(ticket) > 0; });
+    auto r = S->results[ticket];
+    S->results.erase(ticket);
+    if (latency_ms) *latency_ms = r.second;
+    if (r.first) return fail(r.first, "request failed: %s", S->fatal.c_str());
+    return MCB_OK;
+}
+
+int mcb_server_drain(McbServer *S) {
+    if (!S) return fail(MCB_E_ARG, "null server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_done.wait(lk, [&] {
+        if (!S->pending.empty() || !S->inflight.empty()) return false;
+        for (auto &L : S->lanes) if (L.busy) return false;
+        return true;
+    });
+    return MCB_OK;
+}
+
+static double pct(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    size_t k = (size_t)std::min<double>((double)v.size() - 1, std::max(0.0, p / 100.0 * (double)v.size() - 1.0 + 0.999999));
+    return v[k];
+}
+
+int mcb_server_stats(McbServer *S, double *p50, double *p99, double *p999, int64_t *completed, double *hit_rate,
+                     int64_t *batches) {
+    if (!S) return fail(MCB_E_ARG, "null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (p50) *p50 = pct(S->latencies, 50.0);
+    if (p99) *p99 = pct(S->latencies, 99.0);
+    if (p999) *p999 = pct(S->latencies, 99.9);
+    if (completed) *completed = (int64_t)S->latencies.size();
+    int64_t h = 0, m = 0;
+    for (auto &kv : S->pools) { h += kv.second.hits; m += kv.second.misses; }
+    if (hit_rate) *hit_rate = (h + m) ? (double)h / (double)(h + m) : 0.0;
+    if (batches) *batches = S->batches;
+    return MCB_OK;
+}
+
+int mcb_server_reset_stats(McbServer *S) {
+    if (!S) return fail(MCB_E_ARG, "null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->latencies.clear();
+    for (auto &kv : S->pools) { kv.second.hits = 0; kv.second.misses = 0; }
+    S->batches = 0;
+    return MCB_OK;
+}
+
+void mcb_server_destroy(McbServer *S) {
+    if (!S) return;
+    mcb_server_drain(S);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_submit.notify_all();
+    S->cv_lane.notify_all();
+    S->dispatcher.join();
+    S->completer.join();
+    for (auto &L : S->lanes) {
+        (void)hipStreamSynchronize(L.st);
+        (void)hipStreamDestroy(L.st);
+        (void)hipEventDestroy(L.done);
+        (void)hipEventDestroy(L.inited);
+        (void)hipFree(L.d_out);
+        (void)hipFree(L.d_quart);
+        (void)hipFree(L.d_desc);
+        (void)hipHostFree(L.h_out);
+        (void)hipHostFree(L.h_desc);
+    }
+    for (auto &kv : S->pools) (void)hipFree(kv.second.d_slots);
+    delete S;
+}
+
+}  // extern "C"

@@ -1,0 +1,299 @@
+"""Python binding of libmcbiome_hip.so (the MI355X biome engine), over its C ABI (include/mcbiome.h).
+
+Host-side mirror of the reference's CPU-engine interface (engine/cuda/mcg_host.h): ``Host`` loads a
+(version, dimension, preset) world like ``mcg_host_load`` and ``Host.ref_points`` has the contract of
+``mcg_ref_points``.  ``Engine`` exposes the native batched API (device buffers, plans, hipGraph).
+
+There is no CPU fallback: importing this module fails loudly when the HIP library is not built, and
+every compute call raises ``EngineError`` when the device is unusable.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmcbiome_hip.so")
+DATA_DIR = os.path.join(HERE, "data")
+
+VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2}
+DIMS = {"overworld": 0, "the_nether": 1, "nether": 1, "the_end": 2, "end": 2}
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Region(C.Structure):
+    _fields_ = [("seed", C.c_int64), ("x0", C.c_int32), ("z0", C.c_int32), ("nx", C.c_int32), ("nz", C.c_int32),
+                ("ny", C.c_int32), ("y", C.c_int32 * 8), ("zoom", C.c_int32), ("world", C.c_int32),
+                ("out_off", C.c_int64)]
+
+
+class McgHost(C.Structure):
+    _fields_ = [("version", C.c_int), ("dim", C.c_int), ("mode", C.c_int), ("preset", C.c_char * 16),
+                ("impl", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise EngineError("HIP engine not built: %s missing (run __graft_entry__.build())" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        P, i64p, i32p, u8p = C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_uint8)
+        L.mcb_last_error.restype = C.c_char_p
+        L.mcb_open.argtypes = [C.POINTER(P), C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int]
+        L.mcb_close.argtypes = [P]
+        L.mcb_engine_info.argtypes = [P, i32p, i32p, i32p, i32p]
+        L.mcb_points.argtypes = [P, i64p, C.c_int, i32p, C.c_int, u8p, i32p]
+        L.mcb_points_dev.argtypes = [P, P, C.c_int, P, C.c_int, P, P, P]
+        L.mcb_regions_bytes.restype = C.c_int64
+        L.mcb_regions_bytes.argtypes = [C.POINTER(Region), C.c_int]
+        L.mcb_region.argtypes = [P, C.POINTER(Region), u8p]
+        L.mcb_plan_create.argtypes = [P, C.POINTER(Region), C.c_int, C.c_int, C.c_int, C.POINTER(P)]
+        L.mcb_plan_run.argtypes = [P, P, P]
+        L.mcb_plan_run_graph.argtypes = [P, P, P]
+        L.mcb_plan_run_stage.argtypes = [P, C.c_int, P, P]
+        L.mcb_plan_info.argtypes = [P, i64p, i64p, i64p, i64p]
+        L.mcb_plan_free.argtypes = [P]
+        L.mcb_plan_autotune.argtypes = [P, P, P, i32p, C.POINTER(C.c_double)]
+        L.mcb_context_create.argtypes = [C.POINTER(P), C.c_char_p]
+        L.mcb_context_free.argtypes = [P]
+        L.mcb_context_world.argtypes = [P, C.c_int, C.c_int, C.c_char_p]
+        L.mcb_context_plan.argtypes = [P, C.POINTER(Region), C.c_int, C.c_int, C.c_int, C.POINTER(P)]
+        L.mcb_server_create.argtypes = [C.POINTER(P), P, C.c_int64, C.c_int, C.c_int]
+        L.mcb_server_submit.argtypes = [P, C.POINTER(Region), P, i64p]
+        L.mcb_server_wait.argtypes = [P, C.c_int64, C.POINTER(C.c_double)]
+        L.mcb_server_drain.argtypes = [P]
+        L.mcb_server_stats.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), i64p,
+                                       C.POINTER(C.c_double), i64p]
+        L.mcb_server_reset_stats.argtypes = [P]
+        L.mcb_server_destroy.argtypes = [P]
+        L.mcg_host_load.argtypes = [C.POINTER(McgHost), C.c_int, C.c_int, C.c_char_p]
+        L.mcg_host_free.argtypes = [C.POINTER(McgHost)]
+        L.mcg_ref_points.argtypes = [C.POINTER(McgHost), i64p, C.c_int, i32p, C.c_int, u8p, i32p]
+        L.mcg_biome_name.restype = C.c_char_p
+        L.mcg_biome_id.argtypes = [C.c_char_p]
+        L.mcg_version_name.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise EngineError(lib().mcb_last_error().decode())
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def biome_names():
+    L = lib()
+    return [L.mcg_biome_name(i).decode() for i in range(L.mcg_biome_count())]
+
+
+def device_count():
+    return lib().mcb_device_count()
+
+
+def make_region(seed, x0, z0, nx, nz, ys, zoom=False, out_off=0):
+    r = Region()
+    r.seed, r.x0, r.z0, r.nx, r.nz = int(seed), int(x0), int(z0), int(nx), int(nz)
+    ys = list(ys)
+    if len(ys) > 8:
+        raise EngineError("at most 8 layers per region")
+    r.ny = len(ys)
+    for j, y in enumerate(ys):
+        r.y[j] = int(y)
+    r.zoom = 1 if zoom else 0
+    r.out_off = int(out_off)
+    return r
+
+
+class Host:
+    """mcg_host_load / mcg_ref_points drop-in (engine/cuda/mcg_host.h:34-47), on the HIP device."""
+
+    def __init__(self, version="26.3", dim="overworld", preset="normal"):
+        self.h = McgHost()
+        if lib().mcg_host_load(C.byref(self.h), VERSIONS[version], DIMS[dim], preset.encode()) != 0:
+            raise EngineError("mcg_host_load: " + lib().mcb_last_error().decode())
+
+    @property
+    def mode(self):
+        return self.h.mode
+
+    def ref_points(self, seeds, pts, want_tg=False):
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+        pts = np.ascontiguousarray(pts, dtype=np.int32).reshape(-1, 3)
+        ns, np_ = seeds.size, pts.shape[0]
+        out = np.zeros(ns * np_, np.uint8)
+        tg = np.zeros(ns * np_ * 6, np.int32) if want_tg else None
+        lib().mcg_ref_points(C.byref(self.h), _p(seeds, C.c_int64), ns, _p(pts, C.c_int32), np_, _p(out, C.c_uint8),
+                             _p(tg, C.c_int32) if want_tg else None)
+        out = out.reshape(ns, np_)
+        return (out, tg.reshape(ns, np_, 6)) if want_tg else out
+
+    def close(self):
+        if self.h.impl:
+            lib().mcg_host_free(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    def __init__(self, engine, regions, tile_w=0, blocks=0, context=None):
+        self.engine = engine
+        self.context = context
+        arr = (Region * len(regions))(*regions)
+        self._arr = arr
+        self.h = C.c_void_p()
+        if context is not None:
+            _check(lib().mcb_context_plan(context.h, arr, len(regions), tile_w, blocks, C.byref(self.h)))
+        else:
+            _check(lib().mcb_plan_create(engine.h, arr, len(regions), tile_w, blocks, C.byref(self.h)))
+        self.out_bytes = lib().mcb_regions_bytes(arr, len(regions))
+        t, c, s, n = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().mcb_plan_info(self.h, C.byref(t), C.byref(c), C.byref(s), C.byref(n)))
+        self.tiles, self.columns, self.samples, self.seeds = t.value, c.value, s.value, n.value
+
+    def run(self, d_out_ptr, stream_ptr=0, graph=False):
+        fn = lib().mcb_plan_run_graph if graph else lib().mcb_plan_run
+        _check(fn(self.h, C.c_void_p(d_out_ptr), C.c_void_p(stream_ptr or None)))
+
+    def autotune(self, d_out_ptr, stream_ptr=0):
+        tw, ms = C.c_int32(), C.c_double()
+        _check(lib().mcb_plan_autotune(self.h, C.c_void_p(d_out_ptr), C.c_void_p(stream_ptr or None), C.byref(tw),
+                                       C.byref(ms)))
+        return tw.value, ms.value
+
+    def run_stage(self, stage, d_out_ptr, stream_ptr=0):
+        _check(lib().mcb_plan_run_stage(self.h, stage, C.c_void_p(d_out_ptr), C.c_void_p(stream_ptr or None)))
+
+    def free(self):
+        if self.h:
+            lib().mcb_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Engine:
+    """Native engine: one (version, dimension, preset) world on the current HIP device."""
+
+    def __init__(self, version="26.3", dim="overworld", preset="normal", data_dir=None):
+        self.version, self.dim, self.preset = version, dim, preset
+        self.h = C.c_void_p()
+        _check(lib().mcb_open(C.byref(self.h), VERSIONS[version], DIMS[dim], preset.encode(),
+                              data_dir.encode() if data_dir else None, 0))
+        v, d, m, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().mcb_engine_info(self.h, C.byref(v), C.byref(d), C.byref(m), C.byref(n)))
+        self.mode, self.tree_nodes = m.value, n.value
+
+    def points(self, seeds, pts, want_tg=False):
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+        pts = np.ascontiguousarray(pts, dtype=np.int32).reshape(-1, 3)
+        ns, np_ = seeds.size, pts.shape[0]
+        out = np.zeros(ns * np_, np.uint8)
+        tg = np.zeros(ns * np_ * 6, np.int32) if want_tg else None
+        _check(lib().mcb_points(self.h, _p(seeds, C.c_int64), ns, _p(pts, C.c_int32), np_, _p(out, C.c_uint8),
+                                _p(tg, C.c_int32) if want_tg else None))
+        out = out.reshape(ns, np_)
+        return (out, tg.reshape(ns, np_, 6)) if want_tg else out
+
+    def points_dev(self, d_seeds, ns, d_pts, np_, d_out, d_tg=0, stream=0):
+        _check(lib().mcb_points_dev(self.h, C.c_void_p(d_seeds), ns, C.c_void_p(d_pts), np_, C.c_void_p(d_out),
+                                    C.c_void_p(d_tg or None), C.c_void_p(stream or None)))
+
+    def region(self, seed, x0, z0, nx, nz, ys, zoom=False):
+        r = make_region(seed, x0, z0, nx, nz, ys, zoom)
+        out = np.zeros(len(list(ys)) * nx * nz, np.uint8)
+        _check(lib().mcb_region(self.h, C.byref(r), _p(out, C.c_uint8)))
+        return out.reshape(len(list(ys)), nz, nx)
+
+    def plan(self, regions, tile_w=0, blocks=0):
+        return Plan(self, regions, tile_w, blocks)
+
+    def close(self):
+        if self.h:
+            lib().mcb_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """Several worlds on one device (mixed batches, serving)."""
+
+    def __init__(self, data_dir=None):
+        self.h = C.c_void_p()
+        _check(lib().mcb_context_create(C.byref(self.h), data_dir.encode() if data_dir else None))
+
+    def world(self, version, dim, preset="normal"):
+        w = lib().mcb_context_world(self.h, VERSIONS[version], DIMS[dim], preset.encode())
+        if w < 0:
+            raise EngineError(lib().mcb_last_error().decode())
+        return w
+
+    def plan(self, regions, tile_w=0, blocks=0):
+        return Plan(None, regions, tile_w, blocks, context=self)
+
+    def close(self):
+        if self.h:
+            lib().mcb_context_free(self.h)
+            self.h = None
+
+
+class Server:
+    """Continuous-batching request server over a Context (mcb_server_*)."""
+
+    def __init__(self, context, batch_bytes=64 << 20, lanes=3, slot_capacity=16384):
+        self.context = context
+        self.h = C.c_void_p()
+        _check(lib().mcb_server_create(C.byref(self.h), context.h, batch_bytes, lanes, slot_capacity))
+
+    def submit(self, region, host_out):
+        """host_out: a writable contiguous uint8 numpy array of ny*nx*nz bytes (kept alive by the caller)."""
+        t = C.c_int64()
+        _check(lib().mcb_server_submit(self.h, C.byref(region), host_out.ctypes.data_as(C.c_void_p), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        ms = C.c_double()
+        _check(lib().mcb_server_wait(self.h, ticket, C.byref(ms)))
+        return ms.value
+
+    def drain(self):
+        _check(lib().mcb_server_drain(self.h))
+
+    def stats(self):
+        p50, p99, p999, hit = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        n, b = C.c_int64(), C.c_int64()
+        _check(lib().mcb_server_stats(self.h, C.byref(p50), C.byref(p99), C.byref(p999), C.byref(n), C.byref(hit),
+                                      C.byref(b)))
+        return {"p50_ms": p50.value, "p99_ms": p99.value, "p999_ms": p999.value, "completed": n.value,
+                "slot_hit_rate": hit.value, "batches": b.value}
+
+    def reset_stats(self):
+        _check(lib().mcb_server_reset_stats(self.h))
+
+    def close(self):
+        if self.h:
+            lib().mcb_server_destroy(self.h)
+            self.h = None
