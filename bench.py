@@ -145,10 +145,14 @@ def main():
     regions = [mcbiome.make_region(s, qx0, qz0, SIDE, SIDE, QYS, False, i * SIDE * SIDE * len(QYS))
                for i, (s, qx0, qz0) in enumerate(work)]
     engines, plans, outs = {}, {}, {}
+    tuned = {}
     for v in VERSIONS:
         engines[v] = mcbiome.Engine(v, "overworld")
         plans[v] = engines[v].plan(regions, tile_w=args.tile_w)
         outs[v] = torch.empty(plans[v].out_bytes, dtype=torch.uint8, device="cuda")
+        if not args.tile_w:      # project autotuner: tile shape, columns per lane, persistent workgroups
+            tw, ms = plans[v].autotune(outs[v].data_ptr(), sp)
+            tuned[v] = {"tile_w": tw & 0xFFFF, "cols_per_lane": tw >> 16, "grid_ms": round(ms, 3)}
     samples_per_step = sum(plans[v].samples for v in VERSIONS)
 
     def step():
@@ -214,7 +218,7 @@ def main():
         "data": "synthetic (splitmix64-derived seeds and origins)",
         "config": {"workload": "batched Overworld quart grids: 4096 seeds x 64x64 x qy{-16,0,16,48}, 26.3 and 26.1",
                    "samples_per_step": samples_per_step, "columns_per_step": sum(plans[v].columns for v in VERSIONS),
-                   "tile_w": args.tile_w or 16, "parallelism": "single GPU"},
+                   "autotuned": tuned, "parallelism": "single GPU"},
         "per_call_samples_per_s": round(per_call, 1),
         "batched_over_per_call_26_3": round(kern["26.3"]["samples_per_s"] / per_call, 2),
         "per_version_kernel_samples_per_s": {v: round(kern[v]["samples_per_s"], 1) for v in VERSIONS},

@@ -100,7 +100,9 @@ int mcb_region(McbEngine *e, const McbRegion *req, uint8_t *out_host);
 /* Batched plan over many requests of this engine's world: seed slots, tile list and launch shape are
  * fixed at creation; each run initialises the seeds' noise state and evaluates every tile in one
  * persistent grid launch (plus the zoom pass), asynchronously on `stream`, into device memory d_out.
- * tile_w: tile width in columns (tile = tile_w x 256/tile_w; 0 = autotuned default).
+ * tile_w: tile width in columns, optionally | cpl << 16 with cpl = columns evaluated per lane (1, 2, 4;
+ *         each lane walks cpl columns along z, warm-starting the biome search from the previous one);
+ *         tile = tile_w x (256/tile_w)*cpl columns; 0 = default 16, cpl 1.
  * blocks: persistent workgroups (0 = one per tile). */
 int mcb_plan_create(McbEngine *e, const McbRegion *reqs, int n, int tile_w, int blocks, McbPlan **out);
 int mcb_plan_run(McbPlan *p, uint8_t *d_out, void *stream);

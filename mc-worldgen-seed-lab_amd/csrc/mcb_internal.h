@@ -56,8 +56,9 @@ int dev_grow(T **p, size_t *cap, size_t n) {
 // ---- kernel launchers (mcb_kernels.hip); all asynchronous on `stream` ----
 hipError_t launch_seed_init(const DevWorld &w, const int64_t *d_seeds, const int32_t *d_slot_of, int n,
                             McbSlot *d_slots, hipStream_t stream);
+// tile = tile_w x (256/tile_w)*cpl columns, cpl (1/2/4) columns per lane; max_ny = most layers of any request
 hipError_t launch_grid(const DevWorld &w, const McbSlot *d_slots, const McbRequest *d_reqs, const McbTile *d_tiles,
-                       int ntiles, uint8_t *d_out, int tile_w, int tile_h, int grid_blocks, hipStream_t stream);
+                       int ntiles, uint8_t *d_out, int tile_w, int cpl, int max_ny, int grid_blocks, hipStream_t stream);
 hipError_t launch_points(const DevWorld &w, const McbSlot *d_slots, const int32_t *d_slot_of, int ns,
                          const int32_t *d_pts, int np, uint8_t *d_out, int32_t *d_tg, hipStream_t stream);
 hipError_t launch_zoom(const McbRequest *d_zreqs, const McbRequest *d_qreqs, const McbTile *d_tiles, int ntiles,
@@ -70,9 +71,10 @@ struct Lowered {
     std::vector<McbRequest> g, q, z;
     std::vector<McbTile> gt, qt, zt;
     int64_t quart_bytes = 0, columns = 0, samples = 0;
+    int max_ny_g = 1, max_ny_q = 1;
 };
 int check_region(const McbRegion &r);
-void lower_region(const McbRegion &r, int slot, int tile_w, Lowered &L);
+void lower_region(const McbRegion &r, int slot, int tile_w, int cpl, Lowered &L);
 
 }  // namespace mcb
 

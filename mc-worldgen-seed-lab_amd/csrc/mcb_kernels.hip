@@ -8,7 +8,7 @@
 // Kernels
 //   k_seed_ow / k_seed_ne / k_seed_end : per-seed noise state (Xoroshiro or legacy LCG octave init,
 //                                       Fisher-Yates permutations built in a per-lane LDS tile)
-//   k_grid<MODE,DIM>                   : quart-grid tiles of many requests; the seed's 46 permutation
+//   k_grid<MODE,DIM,MAXL,CPL>          : quart-grid tiles of many requests; the seed's 46 permutation
 //                                       tables are staged in LDS, one column per lane, the climate of a
 //                                       column is computed once and shared by all its y layers
 //   k_points<MODE,DIM>                 : seeds x arbitrary quart points (the mcg_ref_points contract)
@@ -437,15 +437,16 @@ __device__ float spline_eval(const McbSpline &S, int root, const float *in4) {
 // pre-order with escape links, so the walk needs no stack; a warm-start candidate (a nearby sample's
 // leaf) only tightens the initial bound and never changes the result.
 // ================================================================================================
+// squared distance of one parameter interval to the target: (t - clamp(t, lo, hi))^2, integer-valued
+// doubles (exact below 2^53)
+__device__ __forceinline__ double dim_sq(int t, int lo, int hi, double acc) {
+    const double x = (double)(t - min(max(t, lo), hi));
+    return fma(x, x, acc);
+}
 __device__ __forceinline__ double node_dist(const McbNode &nd, const int *t) {
     double d = (double)nd.d6;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-        int above = t[k] - (int)nd.hi[k], below = (int)nd.lo[k] - t[k];
-        int x = max(max(above, below), 0);
-        double xd = (double)x;
-        d = fma(xd, xd, d);                     // integer-valued, exact (< 2^53)
-    }
+    for (int k = 0; k < 6; k++) d = dim_sq(t[k], nd.lo[k], nd.hi[k], d);
     return d;
 }
 __device__ __forceinline__ int tree_search(const McbNode *__restrict__ N, int n, const int *t, int &cand) {
@@ -463,6 +464,59 @@ __device__ __forceinline__ int tree_search(const McbNode *__restrict__ N, int n,
     }
     cand = bi;
     return N[bi].leaf & 0xFF;
+}
+
+// All y layers of one column in one walk.  The layers share temperature..weirdness and the offset
+// dimension, so each node's 5-dimension part is computed once and only the depth term per layer; a
+// subtree is entered when any layer may improve in it.  Extra visits never change a layer's result
+// (its own (distance, index) rule decides every update), so each layer gets exactly its
+// single-search leaf.  cand[j] carries the previous column's leaf of layer j as warm start.
+template <int MAXL>
+__device__ __forceinline__ void tree_search_layers(const McbNode *__restrict__ N, int n, const int *t5, const int *dep,
+                                                   int ny, int *cand, int *biome) {
+    double bd[MAXL];
+    int bi[MAXL];
+#pragma unroll
+    for (int j = 0; j < MAXL; j++) {
+        bi[j] = n;
+        bd[j] = __builtin_inf();
+        if (j < ny && cand[j] > 0) {
+            const McbNode c = N[cand[j]];
+            double d = (double)c.d6;
+            d = dim_sq(t5[0], c.lo[0], c.hi[0], d);
+            d = dim_sq(t5[1], c.lo[1], c.hi[1], d);
+            d = dim_sq(t5[2], c.lo[2], c.hi[2], d);
+            d = dim_sq(t5[3], c.lo[3], c.hi[3], d);
+            d = dim_sq(t5[4], c.lo[5], c.hi[5], d);
+            bd[j] = dim_sq(dep[j], c.lo[4], c.hi[4], d);
+            bi[j] = cand[j];
+        }
+    }
+    int cur = 1;
+    while (cur < n) {
+        const McbNode nd = N[cur];
+        double d5 = (double)nd.d6;
+        d5 = dim_sq(t5[0], nd.lo[0], nd.hi[0], d5);
+        d5 = dim_sq(t5[1], nd.lo[1], nd.hi[1], d5);
+        d5 = dim_sq(t5[2], nd.lo[2], nd.hi[2], d5);
+        d5 = dim_sq(t5[3], nd.lo[3], nd.hi[3], d5);
+        d5 = dim_sq(t5[4], nd.lo[5], nd.hi[5], d5);
+        const bool leaf = nd.leaf != 0;
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < MAXL; j++) {
+            if (j < ny) {
+                const double cd = dim_sq(dep[j], nd.lo[4], nd.hi[4], d5);
+                const bool better = cd < bd[j] || (cd == bd[j] && cur < bi[j]);
+                if (better && leaf) { bd[j] = cd; bi[j] = cur; }
+                any |= better;
+            }
+        }
+        cur = (any && !leaf) ? cur + 1 : (int)nd.escape;
+    }
+#pragma unroll
+    for (int j = 0; j < MAXL; j++)
+        if (j < ny) { cand[j] = bi[j]; biome[j] = N[bi[j]].leaf & 0xFF; }
 }
 
 __device__ __forceinline__ int quantize(float v) { return (int)(v * 10000.0f); }   // mc_climate.h:93
@@ -622,13 +676,16 @@ __device__ __forceinline__ void stage_perm(uint8_t *lds, const McbSlot *slot, in
 // ================================================================================================
 // grid kernel: tiles of quart-grid requests of one world
 // ================================================================================================
-template <int MODE, int DIM>
+// One workgroup = one tile of tile_w x (256/tile_w * CPL) columns; lane (lx, lz) evaluates the CPL
+// columns (lx, lz*CPL + c) in order, each column's search warm-started from the previous one.
+template <int MODE, int DIM, int MAXL, int CPL>
 __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
                                               const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
                                               const McbTile *__restrict__ tiles, int ntiles, int tile_w, uint8_t *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_SLOT_PERM_BYTES];
     const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
     const int tree_n = W->tree_n;
+    const int tile_h = 256 / tile_w;
     int staged = -1;
     // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables
     const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
@@ -644,31 +701,40 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
             staged = slot;
         }
         const McbOct *oct = slots[slot].oct;
-        const int tile_h = 256 / tile_w;
         const int ix = tl.tx * tile_w + (int)threadIdx.x % tile_w;
-        const int iz = tl.tz * tile_h + (int)threadIdx.x / tile_w;
-        if (ix >= R.nx || iz >= R.nz) continue;
-        const int qx = R.x0 + ix, qz = R.z0 + iz;
+        const int iz0 = (tl.tz * tile_h + (int)threadIdx.x / tile_w) * CPL;
+        if (ix >= R.nx || iz0 >= R.nz) continue;
+        const int qx = R.x0 + ix;
         const int64_t plane = (int64_t)R.nx * R.nz;
-        uint8_t *o = out + R.out_off + (int64_t)iz * R.nx + ix;
-        if (DIM == MCB_OVERWORLD) {
-            Column c;
-            ow_column<MODE>(W, perm, oct, qx, qz, c);
-            int t[6] = {c.t, c.h, c.c, c.e, 0, c.w};
-            int cand = -1;
-            for (int j = 0; j < R.ny; j++) {
-                t[4] = ow_depth<MODE>(W, c, R.y[j]);
-                o[j * plane] = (uint8_t)tree_search(tree, tree_n, t, cand);
+        const int ny = R.ny;
+        int cand[MAXL];
+#pragma unroll
+        for (int j = 0; j < MAXL; j++) cand[j] = -1;
+        for (int c = 0; c < CPL; c++) {
+            const int iz = iz0 + c;
+            if (iz >= R.nz) break;
+            const int qz = R.z0 + iz;
+            uint8_t *o = out + R.out_off + (int64_t)iz * R.nx + ix;
+            if (DIM == MCB_OVERWORLD) {
+                Column col;
+                ow_column<MODE>(W, perm, oct, qx, qz, col);
+                const int t5[5] = {col.t, col.h, col.c, col.e, col.w};
+                int dep[MAXL], b[MAXL];
+#pragma unroll
+                for (int j = 0; j < MAXL; j++) dep[j] = j < ny ? ow_depth<MODE>(W, col, R.y[j]) : 0;
+                tree_search_layers<MAXL>(tree, tree_n, t5, dep, ny, cand, b);
+#pragma unroll
+                for (int j = 0; j < MAXL; j++)
+                    if (j < ny) o[j * plane] = (uint8_t)b[j];
+            } else if (DIM == MCB_NETHER) {
+                int t[6];
+                ne_column<MODE>(W, perm, oct, qx, qz, t);
+                uint8_t b = (uint8_t)tree_search(tree, tree_n, t, cand[0]);
+                for (int j = 0; j < ny; j++) o[j * plane] = b;
+            } else {
+                uint8_t b = (uint8_t)end_biome<MODE>(W, perm, qx, qz);
+                for (int j = 0; j < ny; j++) o[j * plane] = b;
             }
-        } else if (DIM == MCB_NETHER) {
-            int t[6];
-            ne_column<MODE>(W, perm, oct, qx, qz, t);
-            int cand = -1;
-            uint8_t b = (uint8_t)tree_search(tree, tree_n, t, cand);
-            for (int j = 0; j < R.ny; j++) o[j * plane] = b;
-        } else {
-            uint8_t b = (uint8_t)end_biome<MODE>(W, perm, qx, qz);
-            for (int j = 0; j < R.ny; j++) o[j * plane] = b;
         }
     }
 }
@@ -766,12 +832,32 @@ __global__ __launch_bounds__(256) void k_zoom(const McbRequest *__restrict__ zre
     }
 }
 
+template <int MODE, int DIM, int MAXL, int CPL>
+hipError_t grid_launch4(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
+                        uint8_t *out, int tile_w, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL, CPL>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, slots, reqs, tiles,
+                       ntiles, tile_w, out);
+    return hipGetLastError();
+}
+template <int MODE, int DIM, int MAXL>
+hipError_t grid_launch3(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
+                        uint8_t *out, int tile_w, int cpl, int blocks, hipStream_t st) {
+    switch (cpl) {
+        case 1: return grid_launch4<MODE, DIM, MAXL, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
+        case 2: return grid_launch4<MODE, DIM, MAXL, 2>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
+        default: return grid_launch4<MODE, DIM, MAXL, 4>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
+    }
+}
 template <int MODE, int DIM>
 hipError_t grid_launch(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
-                       uint8_t *out, int tile_w, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_grid<MODE, DIM>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, slots, reqs, tiles, ntiles,
-                       tile_w, out);
-    return hipGetLastError();
+                       uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, hipStream_t st) {
+    if constexpr (DIM != MCB_OVERWORLD) return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    else {
+    if (max_ny <= 1) return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    if (max_ny <= 2) return grid_launch3<MODE, DIM, 2>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    if (max_ny <= 4) return grid_launch3<MODE, DIM, 4>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    return grid_launch3<MODE, DIM, 8>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    }
 }
 template <int MODE, int DIM>
 hipError_t points_launch(const DevWorld &w, const McbSlot *slots, const int32_t *slot_of, int ns, const int32_t *pts, int np,
@@ -795,18 +881,22 @@ hipError_t launch_seed_init(const DevWorld &w, const int64_t *seeds, const int32
 }
 
 hipError_t launch_grid(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
-                       uint8_t *out, int tile_w, int tile_h, int blocks, hipStream_t st) {
+                       uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    if (tile_w * tile_h != 256 || tile_w <= 0) return hipErrorInvalidValue;
+    if (tile_w <= 0 || tile_w > 256 || 256 % tile_w || (cpl != 1 && cpl != 2 && cpl != 4) || max_ny < 1 || max_ny > 8)
+        return hipErrorInvalidValue;
     if (blocks <= 0) blocks = ntiles;
     const bool f = w.host.fmt_new != 0;
     switch (w.dim) {
-        case MCB_OVERWORLD: return f ? grid_launch<1, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st)
-                                     : grid_launch<0, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
-        case MCB_NETHER: return f ? grid_launch<1, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st)
-                                  : grid_launch<0, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
-        default: return f ? grid_launch<1, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st)
-                          : grid_launch<0, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
+        case MCB_OVERWORLD:
+            return f ? grid_launch<1, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st)
+                     : grid_launch<0, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st);
+        case MCB_NETHER:
+            return f ? grid_launch<1, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st)
+                     : grid_launch<0, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st);
+        default:
+            return f ? grid_launch<1, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st)
+                     : grid_launch<0, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st);
     }
 }
 

@@ -32,8 +32,8 @@ static void add_tiles(std::vector<McbTile> &tiles, int req, int nx, int nz, int 
         for (int tx = 0; tx < (nx + tw - 1) / tw; tx++) tiles.push_back(McbTile{req, (int16_t)tx, (int16_t)tz});
 }
 
-void lower_region(const McbRegion &r, int slot, int tile_w, Lowered &L) {
-    const int tile_h = 256 / tile_w;
+void lower_region(const McbRegion &r, int slot, int tile_w, int cpl, Lowered &L) {
+    const int tile_h = 256 / tile_w * cpl;
     McbRequest R;
     memset(&R, 0, sizeof R);
     R.slot = slot;
@@ -44,6 +44,7 @@ void lower_region(const McbRegion &r, int slot, int tile_w, Lowered &L) {
         R.out_off = r.out_off;
         add_tiles(L.gt, (int)L.g.size(), r.nx, r.nz, tile_w, tile_h);
         L.g.push_back(R);
+        L.max_ny_g = std::max(L.max_ny_g, (int)r.ny);
         L.columns += (int64_t)r.nx * r.nz;
         L.samples += (int64_t)r.nx * r.nz * r.ny;
         return;
@@ -67,6 +68,7 @@ void lower_region(const McbRegion &r, int slot, int tile_w, Lowered &L) {
     Z.out_off = r.out_off;
     Z.hashed_seed = hashed_seed(r.seed);
     add_tiles(L.qt, (int)L.q.size(), Q.nx, Q.nz, tile_w, tile_h);
+    L.max_ny_q = std::max(L.max_ny_q, (int)Q.ny);
     add_tiles(L.zt, (int)L.z.size(), Z.nx, Z.nz, 16, 16);
     L.q.push_back(Q);
     L.z.push_back(Z);
@@ -142,7 +144,8 @@ struct Group {
     int ngtiles = 0, nqtiles = 0, nztiles = 0;
     uint8_t *d_quart = nullptr;
     int64_t quart_cap = 0;
-    int tile_w = 16, blocks_arg = 0, blocks_g = 0, blocks_q = 0;
+    int tile_w = 16, cpl = 1, blocks_arg = 0, blocks_g = 0, blocks_q = 0;
+    int max_ny_g = 1, max_ny_q = 1;
     int64_t columns = 0, samples = 0;
 
     void release() {
@@ -155,9 +158,9 @@ struct Group {
     }
 
     // (re)build the descriptors for a tile shape
-    int lower(int tw, int blocks) {
+    int lower(int tw, int cp, int blocks) {
         Lowered L;
-        for (size_t i = 0; i < regs.size(); i++) lower_region(regs[i], slot_of_reg[i], tw, L);
+        for (size_t i = 0; i < regs.size(); i++) lower_region(regs[i], slot_of_reg[i], tw, cp, L);
         if ((int64_t)L.gt.size() > INT32_MAX || (int64_t)L.qt.size() > INT32_MAX) return fail(MCB_E_LIMIT, "too many tiles");
         int rc;
         if ((rc = upload(&d_greqs, L.g)) || (rc = upload(&d_gtiles, L.gt)) || (rc = upload(&d_qreqs, L.q)) ||
@@ -174,6 +177,9 @@ struct Group {
         nqtiles = (int)L.qt.size();
         nztiles = (int)L.zt.size();
         tile_w = tw;
+        cpl = cp;
+        max_ny_g = L.max_ny_g;
+        max_ny_q = L.max_ny_q;
         blocks_arg = blocks;
         blocks_g = blocks > 0 ? std::min(blocks, std::max(ngtiles, 1)) : ngtiles;
         blocks_q = blocks > 0 ? std::min(blocks, std::max(nqtiles, 1)) : nqtiles;
@@ -184,11 +190,10 @@ struct Group {
 
     int stage(int s, uint8_t *d_out, hipStream_t st) const {
         const DevWorld &w = e->w;
-        const int th = 256 / tile_w;
         switch (s) {
             case 0: MCB_HIPCHK(launch_seed_init(w, d_seeds, d_slot_of, nseeds, d_slots, st)); break;
-            case 1: MCB_HIPCHK(launch_grid(w, d_slots, d_greqs, d_gtiles, ngtiles, d_out, tile_w, th, blocks_g, st)); break;
-            case 2: MCB_HIPCHK(launch_grid(w, d_slots, d_qreqs, d_qtiles, nqtiles, d_quart, tile_w, th, blocks_q, st)); break;
+            case 1: MCB_HIPCHK(launch_grid(w, d_slots, d_greqs, d_gtiles, ngtiles, d_out, tile_w, cpl, max_ny_g, blocks_g, st)); break;
+            case 2: MCB_HIPCHK(launch_grid(w, d_slots, d_qreqs, d_qtiles, nqtiles, d_quart, tile_w, cpl, max_ny_q, blocks_q, st)); break;
             default: MCB_HIPCHK(launch_zoom(d_zreqs, d_qreqs, d_ztiles, nztiles, d_quart, d_out, st)); break;
         }
         return 0;
@@ -220,6 +225,11 @@ struct McbContext {
 
 static int build_plan(const std::vector<McbEngine *> &engine_of_world, const McbRegion *reqs, int n, int tile_w, int blocks,
                       McbPlan **out) {
+    // tile_w encodes the columns per lane in its upper bits: tile_w = width | cpl << 16 (cpl 1, 2 or 4)
+    int cpl = (tile_w >> 16) & 0xFF;
+    tile_w &= 0xFFFF;
+    if (cpl == 0) cpl = 1;
+    if (cpl != 1 && cpl != 2 && cpl != 4) return fail(MCB_E_ARG, "columns per lane must be 1, 2 or 4");
     if (!reqs || n <= 0 || !out) return fail(MCB_E_ARG, "bad arguments");
     *out = nullptr;
     if (tile_w == 0) tile_w = 16;
@@ -260,7 +270,7 @@ static int build_plan(const std::vector<McbEngine *> &engine_of_world, const Mcb
         std::vector<int32_t> ident(G.nseeds);
         for (int i = 0; i < G.nseeds; i++) ident[i] = i;
         int rc;
-        if ((rc = upload(&G.d_seeds, seeds[gi])) || (rc = upload(&G.d_slot_of, ident)) || (rc = G.lower(tile_w, blocks))) {
+        if ((rc = upload(&G.d_seeds, seeds[gi])) || (rc = upload(&G.d_slot_of, ident)) || (rc = G.lower(tile_w, cpl, blocks))) {
             for (auto &g : p->groups) g.release();
             return rc;
         }
@@ -362,32 +372,35 @@ int mcb_plan_autotune(McbPlan *p, uint8_t *d_out, void *stream, int *tile_w_out,
     for (auto &G : p->groups) {
         if (int rc = G.stage(0, d_out, st)) return rc;
         const int shapes[4] = {16, 32, 64, 8};
-        const int per_cu[4] = {0, 2, 4, 8};      // 0 = one workgroup per tile
+        const int cpls[3] = {1, 2, 4};
+        const int per_cu[3] = {0, 2, 4};      // 0 = one workgroup per tile
         double best = 1e30;
-        int best_tw = G.tile_w, best_blocks = G.blocks_arg;
+        int best_tw = G.tile_w, best_cpl = G.cpl, best_blocks = G.blocks_arg;
         for (int tw : shapes) {
-            for (int k : per_cu) {
-                const int blocks = k ? k * cus : 0;
-                if (int rc = G.lower(tw, blocks)) return rc;
-                float ms = 0.f, acc = 0.f;
-                for (int rep = 0; rep < 3; rep++) {    // first rep warms caches, keep the best of the rest
-                    MCB_HIPCHK(hipEventRecord(e0, st));
-                    if (int rc = G.stage(1, d_out, st)) return rc;
-                    if (int rc = G.stage(2, d_out, st)) return rc;
-                    MCB_HIPCHK(hipEventRecord(e1, st));
-                    MCB_HIPCHK(hipEventSynchronize(e1));
-                    MCB_HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                    if (rep == 1 || (rep > 1 && ms < acc)) acc = ms;
+            for (int cp : cpls) {
+                for (int k : per_cu) {
+                    const int blocks = k ? k * cus : 0;
+                    if (int rc = G.lower(tw, cp, blocks)) return rc;
+                    float ms = 0.f, acc = 0.f;
+                    for (int rep = 0; rep < 3; rep++) {    // first rep warms caches, keep the best of the rest
+                        MCB_HIPCHK(hipEventRecord(e0, st));
+                        if (int rc = G.stage(1, d_out, st)) return rc;
+                        if (int rc = G.stage(2, d_out, st)) return rc;
+                        MCB_HIPCHK(hipEventRecord(e1, st));
+                        MCB_HIPCHK(hipEventSynchronize(e1));
+                        MCB_HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                        if (rep == 1 || (rep > 1 && ms < acc)) acc = ms;
+                    }
+                    if (acc < best) { best = acc; best_tw = tw; best_cpl = cp; best_blocks = blocks; }
                 }
-                if (acc < best) { best = acc; best_tw = tw; best_blocks = blocks; }
             }
         }
-        if (int rc = G.lower(best_tw, best_blocks)) return rc;
+        if (int rc = G.lower(best_tw, best_cpl, best_blocks)) return rc;
         total_best += best;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (tile_w_out) *tile_w_out = p->groups.empty() ? 0 : p->groups[0].tile_w;
+    if (tile_w_out) *tile_w_out = p->groups.empty() ? 0 : (p->groups[0].tile_w | p->groups[0].cpl << 16);
     if (best_ms_out) *best_ms_out = total_best;
     return MCB_OK;
 }

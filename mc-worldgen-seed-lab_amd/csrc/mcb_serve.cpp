@@ -111,7 +111,7 @@ struct Lane {
 struct McbServer {
     McbContext *ctx = nullptr;
     int64_t batch_bytes = 0;
-    int tile_w = 16;
+    int tile_w = 16, cpl = 2;
     std::vector<Lane> lanes;
     std::unordered_map<int, SlotPool> pools;
     int slot_capacity = 0;
@@ -195,11 +195,13 @@ int launch_batch(McbServer *S, Lane &L) {
             const size_t g0 = B.L.g.size(), q0 = B.L.q.size(), z0 = B.L.z.size();
             Lowered one;
             one.quart_bytes = quart_off;          // zoom staging offsets run across the whole lane
-            lower_region(r, slot, S->tile_w, one);
+            lower_region(r, slot, S->tile_w, S->cpl, one);
             quart_off = one.quart_bytes;
             for (auto t : one.gt) { t.req += (int)g0; B.L.gt.push_back(t); }
             for (auto t : one.qt) { t.req += (int)q0; B.L.qt.push_back(t); }
             for (auto t : one.zt) { t.req += (int)z0; B.L.zt.push_back(t); }
+            B.L.max_ny_g = std::max(B.L.max_ny_g, one.max_ny_g);
+            B.L.max_ny_q = std::max(B.L.max_ny_q, one.max_ny_q);
             B.L.g.insert(B.L.g.end(), one.g.begin(), one.g.end());
             B.L.q.insert(B.L.q.end(), one.q.begin(), one.q.end());
             B.L.z.insert(B.L.z.end(), one.z.begin(), one.z.end());
@@ -235,9 +237,9 @@ int launch_batch(McbServer *S, Lane &L) {
     for (auto &B : wb) {
         const DevWorld &w = B.pool->e->w;
         MCB_HIPCHK(launch_grid(w, B.pool->d_slots, (const McbRequest *)dp(B.g), (const McbTile *)dp(B.gt), (int)B.gt.n,
-                               L.d_out, S->tile_w, 256 / S->tile_w, (int)B.gt.n, L.st));
+                               L.d_out, S->tile_w, S->cpl, B.L.max_ny_g, (int)B.gt.n, L.st));
         MCB_HIPCHK(launch_grid(w, B.pool->d_slots, (const McbRequest *)dp(B.q), (const McbTile *)dp(B.qt), (int)B.qt.n,
-                               L.d_quart, S->tile_w, 256 / S->tile_w, (int)B.qt.n, L.st));
+                               L.d_quart, S->tile_w, S->cpl, B.L.max_ny_q, (int)B.qt.n, L.st));
         MCB_HIPCHK(launch_zoom((const McbRequest *)dp(B.z), (const McbRequest *)dp(B.q), (const McbTile *)dp(B.zt),
                                (int)B.zt.n, L.d_quart, L.d_out, L.st));
     }

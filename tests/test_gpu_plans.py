@@ -29,7 +29,7 @@ def _mixed_regions(rng, n, zoom_every=0):
 
 
 @pytest.mark.parametrize("version", ["26.1", "26.3"])
-@pytest.mark.parametrize("tile_w,blocks", [(16, 0), (32, 7), (64, 0)])
+@pytest.mark.parametrize("tile_w,blocks", [(16, 0), (32, 7), (64, 0), (16 | 2 << 16, 0), (32 | 4 << 16, 5), (8 | 4 << 16, 0)])
 def test_plan_equals_alone_and_oracle(version, tile_w, blocks):
     rng = np.random.default_rng(41 + tile_w + blocks)
     eng = mcbiome.Engine(version, "overworld")
