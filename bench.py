@@ -1,14 +1,16 @@
 #!/usr/bin/env python3
-"""Benchmark: bit-exact biome samples/s, batched vs plain per-call, on one MI355X.
+"""Benchmark: bit-exact biome samples/s, batched vs plain per-call, on one MI355X; p50/p99 tile latency.
 
-Workload (one step): 4096 seeds from splitmix64(1); each seed a 64x64 quart grid at quart-y
-{-16, 0, 16, 48} with its origin uniform in +-2^22 blocks; Overworld 26.3 (float noise path) and
-26.1 (double noise path) on identical inputs.  A step is one batched plan run per version: the seeds'
-noise state is rebuilt and every sample evaluated (nothing cached between steps).
+Default workload (one step): 4096 seeds from splitmix64(1); each seed a 64x64 quart grid at quart-y
+{-16, 0, 16, 48} with its origin uniform in +-2^22 blocks; Overworld 26.3 (float noise path) and 26.1
+(double noise path) on identical inputs.  A step is one batched plan run per version: the seeds' noise
+state is rebuilt and every sample evaluated (nothing is cached between steps).  The same run also
+measures the plain per-call path (one synchronous request per seed), the serving loop's tile latency
+(Poisson arrivals of 512x512-block tiles at 4 y levels), the dominant kernel with HIP events on its own
+stream, and the reference CPU engine on this host (cpu_baseline).  Prints ONE JSON line.
 
-Prints ONE JSON line.  Besides the contract keys it reports the plain per-call rate (one synchronous
-request per seed, like a per-call GPU library), the dominant kernel's roofline figures measured with
-HIP events on its own stream, and the reference CPU engine timed on this host (cpu_baseline).
+Other workloads (--workload): plain (one seed, 256x256 quarts at qy 16), mixed (10,000 uneven requests of
+every version and dimension, 10% zoomed), serve (Poisson arrivals at 200/1000/5000 req/s).
 """
 import argparse
 import json
@@ -28,12 +30,12 @@ import numpy as np  # noqa: E402
 N_SEEDS, SIDE, QYS, RANGE = 4096, 64, (-16, 0, 16, 48), 1 << 22
 VERSIONS = ("26.3", "26.1")
 
-# Algorithmic work per unique column and per sample, counting IEEE add/sub/mul/div the kernel must
-# execute (conversions, compares, floors and integer/tree work excluded), derived in DESIGN.md.
-# Per 2-D octave evaluation: 8 gradient dots (3 mul + 2 add) = 40, 2 corner offsets (r-1), 2 fades (7
-# each), 7 lerps (3 each), amplitude mul + accumulate = 79 in the noise precision; coordinates
-# (x*f, z*f, +offset x2, -floor x2) = 6 double.  52 evaluations per column; per level pair one f*1.018;
-# column glue (shift *4, ridges folded, spline segment, offset) ~40 float.
+# Algorithmic floating-point work per unique column and per sample: IEEE add/sub/mul/div the kernel
+# must execute (conversions, compares, floors and the integer R-tree work excluded).  Per 2-D octave
+# evaluation: 8 gradient dots (3 mul + 2 add) = 40, 2 corner offsets (r-1), 2 fades (7 each), 7 lerps
+# (3 each), amplitude mul + accumulate = 79 in the noise precision; coordinates (x*f, z*f, +offset x2,
+# -floor x2) = 6 double.  52 evaluations per column; per level pair one f*1.018 (26 per column); column
+# glue (shift *4 x2, offsets x2 in double, ridges folded 3, spline segment ~30, offset add) ~40.
 FLOPS = {
     "26.3": {"f32": 52 * 79 + 40, "f64": 52 * 6 + 26 + 2},
     "26.1": {"f32": 0, "f64": 52 * 79 + 52 * 6 + 26 + 2 + 40},
@@ -59,23 +61,88 @@ def signed64(v):
     return v - (1 << 64) if v >= (1 << 63) else v
 
 
-def workload():
+def uniform(d, r):
+    return d % (2 * r + 1) - r
+
+
+def workload_batched():
     """(seed, qx0, qz0) per seed: draws 3i, 3i+1, 3i+2 of splitmix64(1); origin = (draw mod (2R+1)) - R blocks."""
     d = splitmix64_stream(1, 3 * N_SEEDS)
+    return [(signed64(d[3 * i]), uniform(d[3 * i + 1], RANGE) >> 2, uniform(d[3 * i + 2], RANGE) >> 2) for i in range(N_SEEDS)]
+
+
+def workload_mixed(n=10000):
+    """Uneven mixed batch from splitmix64(2), 6 draws per request: seed; dimension (u < 0.7 Overworld,
+    < 0.9 Nether, else End); version = draw mod 3; side = round(16 * 64^u) quarts; origin +-2^22 blocks
+    (2 draws); 10% (every tenth request) zoomed: side x side blocks at block y 64, else quart y 16."""
+    d = splitmix64_stream(2, 6 * n)
     out = []
-    for i in range(N_SEEDS):
-        seed = signed64(d[3 * i])
-        bx0 = d[3 * i + 1] % (2 * RANGE + 1) - RANGE
-        bz0 = d[3 * i + 2] % (2 * RANGE + 1) - RANGE
-        out.append((seed, bx0 >> 2, bz0 >> 2))
+    for i in range(n):
+        seed = signed64(d[6 * i])
+        u = d[6 * i + 1] / 2.0**64
+        dim = "overworld" if u < 0.7 else ("the_nether" if u < 0.9 else "the_end")
+        version = ("26.1", "26.2", "26.3")[d[6 * i + 2] % 3]
+        side = int(round(16 * 64 ** (d[6 * i + 3] / 2.0**64)))
+        bx, bz = uniform(d[6 * i + 4], RANGE), uniform(d[6 * i + 5], RANGE)
+        zoom = i % 10 == 0
+        out.append((seed, dim, version, side, bx, bz, zoom))
     return out
+
+
+def serve_requests(n, rate, seed_base=3):
+    """Serving workload: Poisson arrivals at `rate` req/s; seed rank ~ Zipf(s=1.1) over the first 10^5
+    outputs of splitmix64(3); 512x512-block tile, origin aligned to 512 within +-2^22; y in {-32,64,128,256}."""
+    seeds = [signed64(v) for v in splitmix64_stream(seed_base, 100000)]
+    w = 1.0 / np.arange(1, 100001) ** 1.1
+    cdf = np.cumsum(w) / w.sum()
+    rng = np.random.default_rng(int(rate))
+    ranks = np.minimum(np.searchsorted(cdf, rng.random(n)), 99999)
+    gaps = rng.exponential(1.0 / rate, n)
+    t = np.cumsum(gaps) - gaps[0]
+    tiles = rng.integers(-(RANGE // 512), RANGE // 512, (n, 2)) * 512
+    return [(float(t[i]), seeds[int(ranks[i])], int(tiles[i, 0]), int(tiles[i, 1])) for i in range(n)]
+
+
+def run_serving(mcbiome, rate, n, lanes=3, batch_mb=64):
+    ctx = mcbiome.Context()
+    w = ctx.world("26.3", "overworld")
+    srv = mcbiome.Server(ctx, batch_bytes=batch_mb << 20, lanes=lanes, slot_capacity=16384)
+    reqs = serve_requests(n, rate)
+    nbuf = 512
+    bufs = [np.empty(512 * 512 * 4, np.uint8) for _ in range(nbuf)]
+    warm = mcbiome.make_region(reqs[0][1], reqs[0][2], reqs[0][3], 512, 512, (-32, 64, 128, 256), True)
+    warm.world = w
+    srv.wait(srv.submit(warm, bufs[0]))
+    srv.reset_stats()
+    tickets = []
+    t0 = time.perf_counter()
+    for i, (ta, seed, x0, z0) in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if ta > now:
+            time.sleep(ta - now)
+        r = mcbiome.make_region(seed, x0, z0, 512, 512, (-32, 64, 128, 256), True)
+        r.world = w
+        tickets.append(srv.submit(r, bufs[i % nbuf]))
+        if len(tickets) > nbuf // 2:
+            srv.wait(tickets.pop(0))
+    for t in tickets:
+        srv.wait(t)
+    dt = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    ctx.close()
+    st.update({"rate_req_s": rate, "requests": n, "achieved_req_s": round(n / dt, 1),
+               "samples_per_s": round(n * 512 * 512 * 4 / dt, 1)})
+    for k in ("p50_ms", "p99_ms", "p999_ms", "slot_hit_rate"):
+        st[k] = round(st[k], 3)
+    return st
 
 
 def cpu_baseline(work, budget_s=12.0):
     """The reference CPU engine (engine/cuda/mcg_host.c built from its sources into oracle/_ref) on a
-    bounded sample of the same workload: seeds in order, each its own 64x64x4 quart points, one seed
-    per mcg_ref_points call, calls spread over host threads (parallel over seeds, as the reference's
-    OpenMP loop); falls back to the oracle restatement where oracle/_ref is absent."""
+    bounded sample of the same workload: seeds in order, each its own 64x64x4 quart points, one seed per
+    mcg_ref_points call, calls spread over host threads (parallel over seeds, as the reference's OpenMP
+    loop); the oracle restatement stands in where oracle/_ref is absent."""
     import ctypes as C
     import oracle
     kind = "reference" if oracle.RefEngine.available() else "port"
@@ -115,37 +182,18 @@ def cpu_baseline(work, budget_s=12.0):
         t.join()
     dt = time.perf_counter() - t0
     samples = state["done"] * SIDE * SIDE * len(QYS)
-    return {"value": samples / dt, "unit": "samples/s", "cores": ncores, "kind": kind,
+    return {"value": round(samples / dt, 1), "unit": "samples/s", "cores": ncores, "kind": kind,
             "sample": "%d of the %d seeds (each 64x64 quarts x 4 y, its own origin), Overworld 26.3, %.1f s wall, "
                       "one seed per mcg_ref_points call over %d host threads" % (state["done"], N_SEEDS, dt, ncores)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--tile-w", type=int, default=0)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--percall-seeds", type=int, default=256)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
-              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    import mcbiome
-    assert torch.cuda.is_available(), "bench needs a HIP device"
-    torch.cuda.set_device(0)
+def bench_batched(args, torch, mcbiome):
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
-
-    work = workload()
+    work = workload_batched()
     regions = [mcbiome.make_region(s, qx0, qz0, SIDE, SIDE, QYS, False, i * SIDE * SIDE * len(QYS))
                for i, (s, qx0, qz0) in enumerate(work)]
-    engines, plans, outs = {}, {}, {}
-    tuned = {}
+    engines, plans, outs, tuned = {}, {}, {}, {}
     for v in VERSIONS:
         engines[v] = mcbiome.Engine(v, "overworld")
         plans[v] = engines[v].plan(regions, tile_w=args.tile_w)
@@ -157,7 +205,7 @@ def main():
 
     def step():
         for v in VERSIONS:
-            plans[v].run(outs[v].data_ptr(), sp)
+            plans[v].run(outs[v].data_ptr(), sp, graph=args.graph)
 
     for _ in range(args.warmup):
         step()
@@ -186,15 +234,16 @@ def main():
         f64 = FLOPS[v]["f64"] * cols + PER_SAMPLE[v].get("f64", 0) * samp
         kern[v] = {"ms": ms, "f32_tflops": f32 / ms / 1e9, "f64_tflops": f64 / ms / 1e9,
                    "samples_per_s": samp / (ms / 1e3), "out_gbps": samp / (ms / 1e3) / 1e9}
-    # headline kernel: 26.3 grid, FP32 path
     k = kern["26.3"]
     roofline = {"bound": "valu", "achieved": round(k["f32_tflops"], 3), "peak": PEAK["f32"], "unit": "TFLOP/s",
                 "frac": round(k["f32_tflops"] / PEAK["f32"], 4), "traffic": None,
-                "kernel": "k_grid<26.3 float,overworld>", "kernel_ms": round(k["ms"], 3),
+                "kernel": "k_grid<26.3 float, overworld>", "kernel_ms": round(k["ms"], 3),
                 "f64_side_tflops": round(k["f64_tflops"], 3),
-                "issue_peak_note": "FMA-free, unpacked: 1 op/lane/clk = 39.3 T op/s is the attainable ceiling (25% of 157.3)",
+                "issue_ceiling_note": "FMA-free unpacked FP32/FP64 issue = 1 op/lane/clk = 39.3 T op/s (25% of 157.3); "
+                                      "the R-tree walk is integer/compare work not counted in these flops",
                 "k_grid_26_1_ms": round(kern["26.1"]["ms"], 3),
                 "k_grid_26_1_f64_tflops": round(kern["26.1"]["f64_tflops"], 3),
+                "k_grid_26_1_f64_frac": round(kern["26.1"]["f64_tflops"] / PEAK["f64"], 4),
                 "output_write_GBps": round(k["out_gbps"], 3)}
 
     # plain per-call path: one synchronous request per seed (allocate, launch, copy back, free)
@@ -206,11 +255,9 @@ def main():
         e.region(s, qx0, qz0, SIDE, SIDE, QYS)
     per_call = npc * SIDE * SIDE * len(QYS) / (time.perf_counter() - t1)
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(work)
-
-    line = {
+    lat = run_serving(mcbiome, args.serve_rate, args.serve_requests) if args.serve_requests > 0 else None
+    cpu = None if args.no_cpu else cpu_baseline(work)
+    return {
         "metric": "bit-exact biome samples/s batched vs per-call, 1 MI355X; p50/p99 tile latency",
         "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(dt / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
@@ -218,15 +265,126 @@ def main():
         "data": "synthetic (splitmix64-derived seeds and origins)",
         "config": {"workload": "batched Overworld quart grids: 4096 seeds x 64x64 x qy{-16,0,16,48}, 26.3 and 26.1",
                    "samples_per_step": samples_per_step, "columns_per_step": sum(plans[v].columns for v in VERSIONS),
-                   "autotuned": tuned, "parallelism": "single GPU"},
+                   "autotuned": tuned, "graph": bool(args.graph), "parallelism": "single GPU"},
         "per_call_samples_per_s": round(per_call, 1),
-        "batched_over_per_call_26_3": round(kern["26.3"]["samples_per_s"] / per_call, 2),
+        "batched_over_per_call": round(value / per_call, 2),
         "per_version_kernel_samples_per_s": {v: round(kern[v]["samples_per_s"], 1) for v in VERSIONS},
-        "tile_latency_ms": {"p50": None, "p99": None, "note": "serving loop not in this bench"},
+        "tile_latency_ms": lat,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line))
+
+
+def bench_plain(args, torch, mcbiome):
+    """One seed (first output of splitmix64(0)), 256x256 quarts at qy 16, origin from the next two draws."""
+    d = splitmix64_stream(0, 3)
+    seed, bx0, bz0 = signed64(d[0]), uniform(d[1], 1 << 20), uniform(d[2], 1 << 20)
+    e = mcbiome.Engine("26.3", "overworld")
+    e.region(seed, bx0 >> 2, bz0 >> 2, 256, 256, [16])
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        e.region(seed, bx0 >> 2, bz0 >> 2, 256, 256, [16])
+    dt = (time.perf_counter() - t0) / args.steps
+    import oracle
+    kind = "reference" if oracle.RefEngine.available() else "port"
+    ref = oracle.RefEngine("26.3", "overworld") if kind == "reference" else oracle.Oracle("26.3", "overworld")
+    xs, zs = np.meshgrid(np.arange(256), np.arange(256))
+    pts = np.stack([(bx0 >> 2) + xs.ravel(), np.full(65536, 16), (bz0 >> 2) + zs.ravel()], 1).astype(np.int32)
+    t1 = time.perf_counter()
+    ref.points([seed], pts)
+    ct = time.perf_counter() - t1
+    return {"metric": "plain per-call biome samples/s (one seed, 256x256 quarts)", "value": round(65536 / dt, 1),
+            "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(dt * 1e3, 3),
+            "higher_is_better": True, "scaling": "strong", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "plain: seed splitmix64(0)[0], 256x256 quarts at qy 16, Overworld 26.3"},
+            "cpu_baseline": {"value": round(65536 / ct, 1), "unit": "samples/s", "cores": 1, "kind": kind,
+                             "sample": "the whole grid; the reference parallelises over seeds, so one seed runs on one thread"}}
+
+
+def bench_mixed(args, torch, mcbiome):
+    reqs = workload_mixed(args.mixed_requests)
+    ctx = mcbiome.Context()
+    regions, off = [], 0
+    for seed, dim, version, side, bx, bz, zoom in reqs:
+        if zoom:
+            r = mcbiome.make_region(seed, bx, bz, side, side, [64], True, off)
+        else:
+            r = mcbiome.make_region(seed, bx >> 2, bz >> 2, side, side, [16], False, off)
+        r.world = ctx.world(version, dim)
+        regions.append(r)
+        off += side * side
+    samples = off
+    plan = ctx.plan(regions)
+    out = torch.empty(off, dtype=torch.uint8, device="cuda")
+    sp = torch.cuda.current_stream().cuda_stream
+    tw, tms = plan.autotune(out.data_ptr(), sp)
+    plan.run(out.data_ptr(), sp, graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.run(out.data_ptr(), sp, graph=True)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    # plain per-call path on a sample of the requests (one engine per world, synchronous region calls)
+    engines = {}
+    npc = min(args.percall_seeds, len(reqs))
+    t1 = time.perf_counter()
+    pc_samples = 0
+    for seed, dim, version, side, bx, bz, zoom in reqs[:npc]:
+        key = (version, dim)
+        if key not in engines:
+            engines[key] = mcbiome.Engine(version, dim)
+        if zoom:
+            engines[key].region(seed, bx, bz, side, side, [64], True)
+        else:
+            engines[key].region(seed, bx >> 2, bz >> 2, side, side, [16])
+        pc_samples += side * side
+    pc = pc_samples / (time.perf_counter() - t1)
+    return {"metric": "uneven mixed batch: bit-exact biome samples/s, batched vs per-call", "value": round(samples / dt, 1),
+            "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": round(dt * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": "10,000 requests, Overworld/Nether/End 70/20/10, 26.1/26.2/26.3, side 16..1024 "
+                                   "log-uniform, 10% zoomed; context plan, autotuned, hipGraph",
+                       "samples": samples, "autotuned_first_group_tile": tw, "autotune_ms": round(tms, 3)},
+            "per_call_samples_per_s": round(pc, 1), "batched_over_per_call": round(samples / dt / pc, 2),
+            "per_call_sample_requests": npc}
+
+
+def bench_serve(args, torch, mcbiome):
+    rates = [200, 1000, 5000]
+    res = [run_serving(mcbiome, r, max(200, min(int(r * args.serve_seconds), 20000))) for r in rates]
+    return {"metric": "serving: 512x512-block tiles x 4 y via BiomeManager, p50/p99/p999 latency",
+            "value": res[-1]["achieved_req_s"], "unit": "req/s", "n_gpus": 1, "steps": len(rates), "warmup": 1,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic Poisson arrivals, Zipf(1.1) seeds over 10^5", "config": {"workload": "serve"},
+            "rates": res}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve"])
+    ap.add_argument("--tile-w", type=int, default=0, help="fixed tile (width | cols_per_lane<<16); 0 = autotune")
+    ap.add_argument("--graph", action="store_true", help="replay each plan as a hipGraph")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--percall-seeds", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=1000.0)
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--mixed-requests", type=int, default=10000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import mcbiome
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    torch.cuda.set_device(0)
+    fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve}[args.workload]
+    print(json.dumps(fn(args, torch, mcbiome)))
 
 
 if __name__ == "__main__":

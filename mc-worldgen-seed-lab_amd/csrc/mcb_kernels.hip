@@ -8,9 +8,10 @@
 // Kernels
 //   k_seed_ow / k_seed_ne / k_seed_end : per-seed noise state (Xoroshiro or legacy LCG octave init,
 //                                       Fisher-Yates permutations built in a per-lane LDS tile)
-//   k_grid<MODE,DIM,MAXL,CPL>          : quart-grid tiles of many requests; the seed's 46 permutation
+//   k_grid<MODE,DIM,MAXL>              : quart-grid tiles of many requests; the seed's 46 permutation
 //                                       tables are staged in LDS, one column per lane, the climate of a
 //                                       column is computed once and shared by all its y layers
+//   k_grid_end<MODE>                   : End quart-grid tiles over an LDS list of the tile's island cells
 //   k_points<MODE,DIM>                 : seeds x arbitrary quart points (the mcg_ref_points contract)
 //   k_zoom                             : BiomeManager block zoom over quart layers written by k_grid
 #include "mcb_internal.h"
@@ -676,14 +677,15 @@ __device__ __forceinline__ void stage_perm(uint8_t *lds, const McbSlot *slot, in
 // ================================================================================================
 // grid kernel: tiles of quart-grid requests of one world
 // ================================================================================================
-// One workgroup = one tile of tile_w x (256/tile_w * CPL) columns; lane (lx, lz) evaluates the CPL
-// columns (lx, lz*CPL + c) in order, each column's search warm-started from the previous one.
-template <int MODE, int DIM, int MAXL, int CPL>
+// One workgroup = one tile of tile_w x (256/tile_w * cpl) columns; lane (lx, lz) evaluates the cpl
+// columns (lx, lz*cpl + c) in order, each column's search warm-started from the previous one.
+template <int MODE, int DIM, int MAXL>
 __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
                                               const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
-                                              const McbTile *__restrict__ tiles, int ntiles, int tile_w, uint8_t *__restrict__ out) {
+                                              const McbTile *__restrict__ tiles, int ntiles, int tile_w, int cpl,
+                                              uint8_t *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_SLOT_PERM_BYTES];
-    const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
+    const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT;
     const int tree_n = W->tree_n;
     const int tile_h = 256 / tile_w;
     int staged = -1;
@@ -702,7 +704,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
         }
         const McbOct *oct = slots[slot].oct;
         const int ix = tl.tx * tile_w + (int)threadIdx.x % tile_w;
-        const int iz0 = (tl.tz * tile_h + (int)threadIdx.x / tile_w) * CPL;
+        const int iz0 = (tl.tz * tile_h + (int)threadIdx.x / tile_w) * cpl;
         if (ix >= R.nx || iz0 >= R.nz) continue;
         const int qx = R.x0 + ix;
         const int64_t plane = (int64_t)R.nx * R.nz;
@@ -710,9 +712,8 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
         int cand[MAXL];
 #pragma unroll
         for (int j = 0; j < MAXL; j++) cand[j] = -1;
-        for (int c = 0; c < CPL; c++) {
-            const int iz = iz0 + c;
-            if (iz >= R.nz) break;
+        const int iz_end = min(iz0 + cpl, R.nz);
+        for (int iz = iz0; iz < iz_end; iz++) {
             const int qz = R.z0 + iz;
             uint8_t *o = out + R.out_off + (int64_t)iz * R.nx + ix;
             if (DIM == MCB_OVERWORLD) {
@@ -726,15 +727,117 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
 #pragma unroll
                 for (int j = 0; j < MAXL; j++)
                     if (j < ny) o[j * plane] = (uint8_t)b[j];
-            } else if (DIM == MCB_NETHER) {
+            } else {
                 int t[6];
                 ne_column<MODE>(W, perm, oct, qx, qz, t);
                 uint8_t b = (uint8_t)tree_search(tree, tree_n, t, cand[0]);
                 for (int j = 0; j < ny; j++) o[j * plane] = b;
-            } else {
-                uint8_t b = (uint8_t)end_biome<MODE>(W, perm, qx, qz);
-                for (int j = 0; j < ny; j++) o[j * plane] = b;
             }
+        }
+    }
+}
+
+// End tiles.  The biome of a quart depends only on its chunk, through the island field of the 25x25
+// cells around the chunk (engine/mc_end.h:66-119).  A tile first finds the islands of the union of its
+// chunks' cell windows (one simplex evaluation per cell, spread over the workgroup, islands appended to
+// an LDS list), then every lane takes the maximum over the listed islands inside its chunk's window.  The
+// maximum does not depend on the order of the cells, so the result is the reference's.
+#define END_CAP 1024
+template <int MODE>
+__device__ float end_height_list(const int *ix, const int *iz, const float *isz, int n, int sx, int sz) {
+    const int cx = sx / 2, cz = sz / 2, ox = sx % 2, oz = sz % 2;
+    float best;
+    if (!MODE) {
+        uint32_t r2 = (uint32_t)sx * (uint32_t)sx + (uint32_t)sz * (uint32_t)sz;
+        float len = (float)sqrt((double)(float)(int32_t)r2);
+        best = clampf_ref(100.0f - len * 8.0f, -100.0f, 80.0f);
+    } else best = -100.0f;
+    for (int i = 0; i < n; i++) {
+        const int dx = ix[i] - cx, dz = iz[i] - cz;
+        if (dx < -12 || dx > 12 || dz < -12 || dz > 12) continue;
+        float xd = (float)(ox - dx * 2), zd = (float)(oz - dz * 2);
+        float h = 100.0f - (float)sqrt((double)(xd * xd + zd * zd)) * isz[i];
+        h = clampf_ref(h, -100.0f, 80.0f);
+        best = best > h ? best : h;
+    }
+    return best;
+}
+template <int MODE>
+__device__ int end_biome_from_height(const McbWorld *__restrict__ W, int cx, int cz, float hgt) {
+    const int bx = (cx * 2 + 1) * 8, bz = (cz * 2 + 1) * 8;
+    double h;
+    if (!MODE) h = ((double)hgt - 8.0) / 128.0;
+    else {
+        float dx = (float)(0 - bx), dz = (float)(0 - bz);
+        float len = (float)sqrt((double)(dx * dx + 0.0f * 0.0f + dz * dz));
+        float c = (clampf_ref(100.0f - len, -100.0f, 80.0f) - 8.0f) * 0.0078125f;
+        float e = (hgt - 8.0f) / 128.0f;
+        h = (double)(c > e ? c : e);
+    }
+    if (h > 0.25) return W->end_ids[1];
+    if (h >= -0.0625) return W->end_ids[2];
+    return h < -0.21875 ? W->end_ids[3] : W->end_ids[4];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_grid_end(const McbWorld *__restrict__ W, const McbSlot *__restrict__ slots,
+                                                  const McbRequest *__restrict__ reqs, const McbTile *__restrict__ tiles,
+                                                  int ntiles, int tile_w, int cpl, uint8_t *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm[256];
+    __shared__ int isl_x[END_CAP], isl_z[END_CAP];
+    __shared__ float isl_s[END_CAP];
+    __shared__ int isl_n;
+    const int tile_h = 256 / tile_w;
+    int staged = -1;
+    const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t_end = min(ntiles, ((int)blockIdx.x + 1) * per);
+    for (int ti = (int)blockIdx.x * per; ti < t_end; ti++) {
+        const McbTile tl = tiles[ti];
+        const McbRequest &R = reqs[tl.req];
+        __syncthreads();                                  // previous tile's list fully consumed
+        if (R.slot != staged) {
+            stage_perm(perm, slots + R.slot, 1);
+            staged = R.slot;
+        }
+        if (threadIdx.x == 0) isl_n = 0;
+        // quart, chunk and island-cell extent of this tile
+        const int ix_lo = tl.tx * tile_w, iz_lo = tl.tz * tile_h * cpl;
+        const int ix_hi = min(ix_lo + tile_w, R.nx) - 1, iz_hi = min(iz_lo + tile_h * cpl, R.nz) - 1;
+        const int cx_lo = (R.x0 + ix_lo) >> 2, cx_hi = (R.x0 + ix_hi) >> 2;
+        const int cz_lo = (R.z0 + iz_lo) >> 2, cz_hi = (R.z0 + iz_hi) >> 2;
+        // chunk c is sampled at section 2c+1, whose island cells are (2c+1)/2 + [-12, 12]
+        const int wx_lo = (2 * cx_lo + 1) / 2 - 12, wx_hi = (2 * cx_hi + 1) / 2 + 12;
+        const int wz_lo = (2 * cz_lo + 1) / 2 - 12, wz_hi = (2 * cz_hi + 1) / 2 + 12;
+        const int wxn = wx_hi - wx_lo + 1, ncell = wxn * (wz_hi - wz_lo + 1);
+        __syncthreads();
+        for (int c = (int)threadIdx.x; c < ncell; c += 256) {
+            const int64_t tx = wx_lo + c % wxn, tz = wz_lo + c / wxn;
+            if (tx * tx + tz * tz <= 4096) continue;
+            const double nv = simplex2(perm, (double)tx, (double)tz);
+            const bool island = MODE ? ((float)nv < -0.9f) : (nv < (double)-0.9f);
+            if (!island) continue;
+            const int k = atomicAdd(&isl_n, 1);
+            if (k < END_CAP) {
+                isl_x[k] = (int)tx;
+                isl_z[k] = (int)tz;
+                isl_s[k] = fmodf(fabsf((float)tx) * 3439.0f + fabsf((float)tz) * 147.0f, 13.0f) + 9.0f;
+            }
+        }
+        __syncthreads();
+        const int n = isl_n;
+        const int ix = ix_lo + (int)threadIdx.x % tile_w;
+        const int iz0 = (tl.tz * tile_h + (int)threadIdx.x / tile_w) * cpl;
+        if (ix >= R.nx || iz0 >= R.nz) continue;
+        const int64_t plane = (int64_t)R.nx * R.nz;
+        const int iz_end = min(iz0 + cpl, R.nz);
+        for (int iz = iz0; iz < iz_end; iz++) {
+            const int qx = R.x0 + ix, qz = R.z0 + iz;
+            const int cx = qx >> 2, cz = qz >> 2;
+            int b;
+            if ((int64_t)cx * cx + (int64_t)cz * cz <= 4096) b = W->end_ids[0];
+            else if (n > END_CAP) b = end_biome<MODE>(W, perm, qx, qz);      // list overflow: direct scan
+            else b = end_biome_from_height<MODE>(W, cx, cz, end_height_list<MODE>(isl_x, isl_z, isl_s, n, 2 * cx + 1, 2 * cz + 1));
+            for (int j = 0; j < R.ny; j++) out[R.out_off + j * plane + (int64_t)iz * R.nx + ix] = (uint8_t)b;
         }
     }
 }
@@ -832,31 +935,27 @@ __global__ __launch_bounds__(256) void k_zoom(const McbRequest *__restrict__ zre
     }
 }
 
-template <int MODE, int DIM, int MAXL, int CPL>
-hipError_t grid_launch4(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
-                        uint8_t *out, int tile_w, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL, CPL>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, slots, reqs, tiles,
-                       ntiles, tile_w, out);
-    return hipGetLastError();
-}
 template <int MODE, int DIM, int MAXL>
 hipError_t grid_launch3(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
                         uint8_t *out, int tile_w, int cpl, int blocks, hipStream_t st) {
-    switch (cpl) {
-        case 1: return grid_launch4<MODE, DIM, MAXL, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
-        case 2: return grid_launch4<MODE, DIM, MAXL, 2>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
-        default: return grid_launch4<MODE, DIM, MAXL, 4>(w, slots, reqs, tiles, ntiles, out, tile_w, blocks, st);
-    }
+    if constexpr (DIM == MCB_END)
+        hipLaunchKernelGGL((k_grid_end<MODE>), dim3(blocks), dim3(256), 0, st, w.d_world, slots, reqs, tiles, ntiles, tile_w,
+                           cpl, out);
+    else
+        hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, slots, reqs, tiles,
+                           ntiles, tile_w, cpl, out);
+    return hipGetLastError();
 }
 template <int MODE, int DIM>
 hipError_t grid_launch(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
                        uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, hipStream_t st) {
-    if constexpr (DIM != MCB_OVERWORLD) return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
-    else {
-    if (max_ny <= 1) return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
-    if (max_ny <= 2) return grid_launch3<MODE, DIM, 2>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
-    if (max_ny <= 4) return grid_launch3<MODE, DIM, 4>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
-    return grid_launch3<MODE, DIM, 8>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    if constexpr (DIM != MCB_OVERWORLD) {
+        return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    } else {
+        if (max_ny <= 1) return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+        if (max_ny <= 2) return grid_launch3<MODE, DIM, 2>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+        if (max_ny <= 4) return grid_launch3<MODE, DIM, 4>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+        return grid_launch3<MODE, DIM, 8>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
     }
 }
 template <int MODE, int DIM>
@@ -883,7 +982,7 @@ hipError_t launch_seed_init(const DevWorld &w, const int64_t *seeds, const int32
 hipError_t launch_grid(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
                        uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    if (tile_w <= 0 || tile_w > 256 || 256 % tile_w || (cpl != 1 && cpl != 2 && cpl != 4) || max_ny < 1 || max_ny > 8)
+    if (tile_w <= 0 || tile_w > 256 || 256 % tile_w || cpl < 1 || cpl > 64 || max_ny < 1 || max_ny > 8)
         return hipErrorInvalidValue;
     if (blocks <= 0) blocks = ntiles;
     const bool f = w.host.fmt_new != 0;

@@ -188,12 +188,15 @@ struct Group {
         return 0;
     }
 
-    int stage(int s, uint8_t *d_out, hipStream_t st) const {
+    // limit > 0 restricts the grid stages to a prefix of the tile lists (autotuner trials)
+    int stage(int s, uint8_t *d_out, hipStream_t st, int limit = 0) const {
         const DevWorld &w = e->w;
+        const int ng = limit > 0 ? std::min(ngtiles, limit) : ngtiles, nq = limit > 0 ? std::min(nqtiles, limit) : nqtiles;
+        const int bg = std::min(blocks_g, std::max(ng, 1)), bq = std::min(blocks_q, std::max(nq, 1));
         switch (s) {
             case 0: MCB_HIPCHK(launch_seed_init(w, d_seeds, d_slot_of, nseeds, d_slots, st)); break;
-            case 1: MCB_HIPCHK(launch_grid(w, d_slots, d_greqs, d_gtiles, ngtiles, d_out, tile_w, cpl, max_ny_g, blocks_g, st)); break;
-            case 2: MCB_HIPCHK(launch_grid(w, d_slots, d_qreqs, d_qtiles, nqtiles, d_quart, tile_w, cpl, max_ny_q, blocks_q, st)); break;
+            case 1: MCB_HIPCHK(launch_grid(w, d_slots, d_greqs, d_gtiles, ng, d_out, tile_w, cpl, max_ny_g, bg, st)); break;
+            case 2: MCB_HIPCHK(launch_grid(w, d_slots, d_qreqs, d_qtiles, nq, d_quart, tile_w, cpl, max_ny_q, bq, st)); break;
             default: MCB_HIPCHK(launch_zoom(d_zreqs, d_qreqs, d_ztiles, nztiles, d_quart, d_out, st)); break;
         }
         return 0;
@@ -229,7 +232,7 @@ static int build_plan(const std::vector<McbEngine *> &engine_of_world, const Mcb
     int cpl = (tile_w >> 16) & 0xFF;
     tile_w &= 0xFFFF;
     if (cpl == 0) cpl = 1;
-    if (cpl != 1 && cpl != 2 && cpl != 4) return fail(MCB_E_ARG, "columns per lane must be 1, 2 or 4");
+    if (cpl < 1 || cpl > 64) return fail(MCB_E_ARG, "columns per lane must be 1..64");
     if (!reqs || n <= 0 || !out) return fail(MCB_E_ARG, "bad arguments");
     *out = nullptr;
     if (tile_w == 0) tile_w = 16;
@@ -355,6 +358,10 @@ void mcb_plan_free(McbPlan *p) {
     delete p;
 }
 
+// trial launches run a prefix of each tile list, long enough to fill the device many times over; the
+// whole-plan time is estimated as trial time x (tiles / trial tiles)
+static const int kTuneTiles = 8192;
+
 int mcb_plan_autotune(McbPlan *p, uint8_t *d_out, void *stream, int *tile_w_out, double *best_ms_out) {
     if (!p || !d_out) return fail(MCB_E_ARG, "bad arguments");
     hipStream_t st = (hipStream_t)stream;
@@ -372,7 +379,7 @@ int mcb_plan_autotune(McbPlan *p, uint8_t *d_out, void *stream, int *tile_w_out,
     for (auto &G : p->groups) {
         if (int rc = G.stage(0, d_out, st)) return rc;
         const int shapes[4] = {16, 32, 64, 8};
-        const int cpls[3] = {1, 2, 4};
+        const int cpls[4] = {1, 2, 4, 16};
         const int per_cu[3] = {0, 2, 4};      // 0 = one workgroup per tile
         double best = 1e30;
         int best_tw = G.tile_w, best_cpl = G.cpl, best_blocks = G.blocks_arg;
@@ -384,13 +391,15 @@ int mcb_plan_autotune(McbPlan *p, uint8_t *d_out, void *stream, int *tile_w_out,
                     float ms = 0.f, acc = 0.f;
                     for (int rep = 0; rep < 3; rep++) {    // first rep warms caches, keep the best of the rest
                         MCB_HIPCHK(hipEventRecord(e0, st));
-                        if (int rc = G.stage(1, d_out, st)) return rc;
-                        if (int rc = G.stage(2, d_out, st)) return rc;
+                        if (int rc = G.stage(1, d_out, st, kTuneTiles)) return rc;
+                        if (int rc = G.stage(2, d_out, st, kTuneTiles)) return rc;
                         MCB_HIPCHK(hipEventRecord(e1, st));
                         MCB_HIPCHK(hipEventSynchronize(e1));
                         MCB_HIPCHK(hipEventElapsedTime(&ms, e0, e1));
                         if (rep == 1 || (rep > 1 && ms < acc)) acc = ms;
                     }
+                    const int trial = std::max(1, std::min(std::max(G.ngtiles, G.nqtiles), kTuneTiles));
+                    acc *= (float)(G.ngtiles + G.nqtiles) / (float)std::max(1, std::min(G.ngtiles, trial) + std::min(G.nqtiles, trial));
                     if (acc < best) { best = acc; best_tw = tw; best_cpl = cp; best_blocks = blocks; }
                 }
             }

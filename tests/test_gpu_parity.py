@@ -152,3 +152,27 @@ def test_wrap_branch_far_coordinates():
         want, wt = O.Oracle(version, "overworld").points(seeds, pts, want_tg=True)
         got, gt = mcbiome.Engine(version, "overworld").points(seeds, pts, want_tg=True)
         assert np.array_equal(gt, wt) and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("version", G.VERSIONS)
+def test_end_tiles_match_oracle(version):
+    """End tiles (island cells listed once per tile) over the centre boundary, negative chunks and far
+    coordinates, for several tile shapes and columns per lane."""
+    torch = pytest.importorskip("torch")
+    eng = mcbiome.Engine(version, "the_end")
+    orc = O.Oracle(version, "the_end")
+    cases = [(1, -300, -290, 70, 50), (-7048155917072976836, 250, -20, 40, 40), (2**63 - 1, -(2**20), 2**20 - 99, 37, 61)]
+    for tile in (16, 64 | 4 << 16, 8 | 16 << 16):
+        regs, off = [], 0
+        for seed, x0, z0, nx, nz in cases:
+            regs.append(mcbiome.make_region(seed, x0, z0, nx, nz, [16], False, off))
+            off += nx * nz
+        plan = eng.plan(regs, tile_w=tile)
+        out = torch.zeros(off, dtype=torch.uint8, device="cuda")
+        plan.run(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        host = out.cpu().numpy()
+        for r, (seed, x0, z0, nx, nz) in zip(regs, cases):
+            want = orc.grid(seed, x0, z0, nx, nz, 16, 0)
+            got = host[r.out_off: r.out_off + nx * nz].reshape(nz, nx)
+            assert np.array_equal(got, want), (tile, seed, int((got != want).sum()))
