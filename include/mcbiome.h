@@ -70,6 +70,17 @@ int mcb_device_count(void);
 int mcb_open(McbEngine **out, int version, int dim, const char *preset, const char *data_dir, int max_slots);
 void mcb_close(McbEngine *e);
 int mcb_engine_info(const McbEngine *e, int *version, int *dim, int *mode, int *tree_nodes);
+/* Overworld climate cell table that replaces the per-sample R-tree search of engine/mc_biomes.h:31-62
+ * (same (distance, pre-order index) result): cells, total list length and host build time (built once
+ * per tree per process).  Zero cells for the Nether and the End, which keep the tree walk. */
+int mcb_engine_cells(const McbEngine *e, int64_t *cells, int64_t *list_len, double *build_ms);
+
+/* Biomes of n quantized climate targets {t,h,c,e,d,w} (6 int32 each, the out_tg layout of
+ * mcg_ref_points), host buffers, synchronous: the search step alone (engine/mc_biomes.h:31-62; the
+ * quantized counterpart of libmcgen's world_biome_from_climate, libmcgen/src/mcgen_internal.h:166).
+ * Overworld and Nether; |target| <= MCB_TARGET_MAX (distances stay exact in double). */
+#define MCB_TARGET_MAX (1 << 24)
+int mcb_targets(McbEngine *e, const int32_t *tg6, int n, uint8_t *out_biome);
 
 /* Seeds x points, host buffers, synchronous (mcg_ref_points contract). */
 int mcb_points(McbEngine *e, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out_biome,

@@ -5,6 +5,8 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <map>
+#include <memory>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -57,6 +59,55 @@ int ensure_ident(McbEngine *e, int n) {
     for (int i = 0; i < n; i++) id[i] = i;
     MCB_HIPCHK(hipMemcpy(e->d_ident, id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
     return 0;
+}
+
+// Cell tables depend only on the tree (not on the seed or preset): one build per distinct tree per
+// process, shared by every engine and context world that loads it.
+std::shared_ptr<const CellTable> cells_for_tree(const std::vector<McbNode> &tree, std::string &err) {
+    static std::mutex mu;
+    static std::map<uint64_t, std::shared_ptr<const CellTable>> cache;
+    uint64_t h = 1469598103934665603ULL;
+    const uint8_t *p = reinterpret_cast<const uint8_t *>(tree.data());
+    for (size_t i = 0; i < tree.size() * sizeof(McbNode); i++) h = (h ^ p[i]) * 1099511628211ULL;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(h);
+    if (it != cache.end()) return it->second;
+    auto ct = std::make_shared<CellTable>();
+    if (build_cells(tree, *ct, err) != 0) return nullptr;
+    cache[h] = ct;
+    return ct;
+}
+
+// Fills the world's cell-table fields and uploads the entries and lists.
+int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {
+    std::string err;
+    auto ct = cells_for_tree(tree, err);
+    if (!ct) return fail(MCB_E_DATA, "cell table: %s", err.c_str());
+    McbWorld &W = e->w.host;
+    int n[6];
+    for (int k = 0; k < 6; k++) {
+        W.cell_nb[k] = ct->nb[k];
+        n[k] = ct->nb[k] + 1;
+        for (int i = 0; i < ct->nb[k]; i++) W.cell_bp[k][i] = ct->bp[ct->off[k] + i];
+    }
+    // index = (((((cT*nH + cH)*nC + cC)*nE + cE)*nW + cW)*nD + cD)
+    W.cell_stride[4] = 1;
+    W.cell_stride[5] = n[4];
+    W.cell_stride[3] = n[5] * W.cell_stride[5];
+    W.cell_stride[2] = n[3] * W.cell_stride[3];
+    W.cell_stride[1] = n[2] * W.cell_stride[2];
+    W.cell_stride[0] = n[1] * W.cell_stride[1];
+    if ((size_t)n[0] * W.cell_stride[0] != ct->entry.size()) return fail(MCB_E_DATA, "cell table size mismatch");
+    e->w.cells_build_ms = ct->build_ms;
+    if (hipMalloc((void **)&e->w.d_cells, ct->entry.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&e->w.d_cell_list, std::max<size_t>(ct->list.size(), 1) * sizeof(uint16_t)) != hipSuccess)
+        return fail(MCB_E_NOMEM, "cell table allocation");
+    MCB_HIPCHK(hipMemcpy(e->w.d_cells, ct->entry.data(), ct->entry.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!ct->list.empty())
+        MCB_HIPCHK(hipMemcpy(e->w.d_cell_list, ct->list.data(), ct->list.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    e->cell_entries = ct->entry.size();
+    e->cell_list_len = ct->list.size();
+    return MCB_OK;
 }
 
 // seeds are processed in chunks so the points grid stays within the y-dimension limit
@@ -115,6 +166,12 @@ int mcb_open(McbEngine **out, int version, int dim, const char *preset, const ch
     e->w.version = version;
     e->w.dim = dim;
     e->w.preset = preset ? preset : "normal";
+    if (dim == MCB_OVERWORLD) {
+        if (int rc = attach_cells(e, tree)) {
+            mcb_close(e);
+            return rc;
+        }
+    }
     if (hipGetDevice(&e->device) != hipSuccess || hipMalloc((void **)&e->w.d_world, sizeof(McbWorld)) != hipSuccess ||
         hipMemcpy(e->w.d_world, &e->w.host, sizeof(McbWorld), hipMemcpyHostToDevice) != hipSuccess) {
         mcb_close(e);
@@ -135,6 +192,8 @@ void mcb_close(McbEngine *e) {
     if (!e) return;
     (void)hipFree(e->w.d_world);
     (void)hipFree(e->w.d_tree);
+    (void)hipFree(e->w.d_cells);
+    (void)hipFree(e->w.d_cell_list);
     (void)hipFree(e->d_slots);
     (void)hipFree(e->d_ident);
     (void)hipFree(e->d_seeds);
@@ -150,6 +209,29 @@ int mcb_engine_info(const McbEngine *e, int *version, int *dim, int *mode, int *
     if (dim) *dim = e->w.dim;
     if (mode) *mode = e->w.host.fmt_new;
     if (tree_nodes) *tree_nodes = e->w.host.tree_n;
+    return MCB_OK;
+}
+
+int mcb_engine_cells(const McbEngine *e, int64_t *cells, int64_t *list_len, double *build_ms) {
+    if (!e) return fail(MCB_E_ARG, "null engine");
+    if (cells) *cells = (int64_t)e->cell_entries;
+    if (list_len) *list_len = (int64_t)e->cell_list_len;
+    if (build_ms) *build_ms = e->w.cells_build_ms;
+    return MCB_OK;
+}
+
+int mcb_targets(McbEngine *e, const int32_t *tg6, int n, uint8_t *out_biome) {
+    if (!e || n < 0 || (n && (!tg6 || !out_biome))) return fail(MCB_E_ARG, "bad arguments");
+    if (e->w.dim == MCB_END) return fail(MCB_E_ARG, "the End has no climate targets");
+    for (int64_t i = 0; i < (int64_t)n * 6; i++)
+        if (tg6[i] < -MCB_TARGET_MAX || tg6[i] > MCB_TARGET_MAX) return fail(MCB_E_ARG, "target %lld out of range", (long long)i);
+    if (n == 0) return MCB_OK;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (int rc = dev_grow(&e->d_tg, &e->tg_cap, (size_t)n * 6)) return rc;
+    if (int rc = dev_grow(&e->d_out, &e->out_cap, (size_t)n)) return rc;
+    MCB_HIPCHK(hipMemcpy(e->d_tg, tg6, (size_t)n * 6 * sizeof(int32_t), hipMemcpyHostToDevice));
+    MCB_HIPCHK(launch_targets(e->w, e->d_tg, n, e->d_out, nullptr));
+    MCB_HIPCHK(hipMemcpy(out_biome, e->d_out, (size_t)n, hipMemcpyDeviceToHost));
     return MCB_OK;
 }
 

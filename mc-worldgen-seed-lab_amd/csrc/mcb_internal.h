@@ -20,6 +20,22 @@ int biome_id(const char *name);
 int build_world(const std::string &root, int version, int dim, const std::string &preset, McbWorld *w,
                 std::vector<McbNode> &tree, std::string &err);
 
+// Climate cell table (mcb_cells.cpp): per cell of the breakpoint grid, the ascending pre-order
+// indices of the leaves that can be the (distance, index) minimum of a target inside it.
+// Cell index = (((((cT*nH + cH)*nC + cC)*nE + cE)*nW + cW)*nD + cD), n = breakpoints + 1, cell i of a
+// dimension = [bp[i-1], bp[i]-1] with the outer cells bounded by +-kCellTMax.
+constexpr int kCellTMax = 32767;
+constexpr int kCellMaxBp = MCB_CELL_MAX_BP;
+struct CellTable {
+    int nb[6] = {0, 0, 0, 0, 0, 0};    // breakpoints per tree dimension (0 T, 1 H, 2 C, 3 E, 4 depth, 5 W)
+    int off[6] = {0, 0, 0, 0, 0, 0};   // first breakpoint of each dimension in bp
+    std::vector<int32_t> bp;
+    std::vector<uint32_t> entry;       // (first << 8) | count; count 0 = walk the tree
+    std::vector<uint16_t> list;
+    double build_ms = 0.0;
+};
+int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &err);
+
 // BiomeManager.obfuscateSeed (SHA-256 of the seed's little-endian bytes, first 8 digest bytes LE)
 int64_t hashed_seed(int64_t seed);
 
@@ -28,6 +44,9 @@ struct DevWorld {
     McbWorld host;
     McbWorld *d_world = nullptr;
     McbNode *d_tree = nullptr;
+    uint32_t *d_cells = nullptr;      // Overworld cell table entries and lists (null elsewhere)
+    uint16_t *d_cell_list = nullptr;
+    double cells_build_ms = 0.0;
     int version = 0, dim = 0;
     std::string preset;
 };
@@ -61,6 +80,8 @@ hipError_t launch_grid(const DevWorld &w, const McbSlot *d_slots, const McbReque
                        int ntiles, uint8_t *d_out, int tile_w, int cpl, int max_ny, int grid_blocks, hipStream_t stream);
 hipError_t launch_points(const DevWorld &w, const McbSlot *d_slots, const int32_t *d_slot_of, int ns,
                          const int32_t *d_pts, int np, uint8_t *d_out, int32_t *d_tg, hipStream_t stream);
+// biomes of n quantized climate targets (6 int32 each: t,h,c,e,d,w); Overworld and Nether only
+hipError_t launch_targets(const DevWorld &w, const int32_t *d_tg, int n, uint8_t *d_out, hipStream_t stream);
 hipError_t launch_zoom(const McbRequest *d_zreqs, const McbRequest *d_qreqs, const McbTile *d_tiles, int ntiles,
                        const uint8_t *d_quart, uint8_t *d_out, hipStream_t stream);
 
@@ -96,6 +117,7 @@ struct McbEngine {
     size_t out_cap = 0;
     int32_t *d_tg = nullptr;
     size_t tg_cap = 0;
+    size_t cell_entries = 0, cell_list_len = 0;
 };
 
 #endif

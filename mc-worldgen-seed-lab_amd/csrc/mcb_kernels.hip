@@ -520,6 +520,52 @@ __device__ __forceinline__ void tree_search_layers(const McbNode *__restrict__ N
         if (j < ny) { cand[j] = bi[j]; biome[j] = N[bi[j]].leaf & 0xFF; }
 }
 
+// ================================================================================================
+// climate cell table (built by mcb_cells.cpp): the cell's ascending list of the leaves that can be the
+// (distance, pre-order index) minimum inside it.  Evaluating the list with a strict < keeps the first
+// minimum, which is the tree search's result.  Targets beyond +-kCellTMax walk the tree instead.
+// ================================================================================================
+#define CELL_TMAX 32767
+// cell of target t along tree dimension k: the number of breakpoints <= t (breakpoints are uniform)
+__device__ __forceinline__ int cell_coord(const McbWorld *__restrict__ W, int k, int t) {
+    const int n = W->cell_nb[k];
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < MCB_CELL_MAX_BP; i++)
+        if (i < n) c += t >= W->cell_bp[k][i] ? 1 : 0;
+    return c;
+}
+__device__ __forceinline__ bool cell_in_range(int t) { return t >= -CELL_TMAX && t <= CELL_TMAX; }
+// t5 = temperature, humidity, continentalness, erosion, weirdness; depth separately (tree dimension 4)
+__device__ __forceinline__ uint32_t cell_base(const McbWorld *__restrict__ W, const int *t5) {
+    return (uint32_t)(cell_coord(W, 0, t5[0]) * W->cell_stride[0] + cell_coord(W, 1, t5[1]) * W->cell_stride[1] +
+                      cell_coord(W, 2, t5[2]) * W->cell_stride[2] + cell_coord(W, 3, t5[3]) * W->cell_stride[3] +
+                      cell_coord(W, 5, t5[4]) * W->cell_stride[5]);
+}
+// biome of the least (distance, index) leaf of a cell entry (count > 0)
+__device__ __forceinline__ int cell_best(const McbNode *__restrict__ N, const uint16_t *__restrict__ list, uint32_t e,
+                                         const int *t5, int dep) {
+    const int cnt = (int)(e & 255u);
+    const uint16_t *l = list + (e >> 8);
+    double bd = __builtin_inf();
+    int biome = 0;
+    for (int q = 0; q < cnt; q++) {
+        const McbNode nd = N[l[q]];
+        double d = (double)nd.d6;
+        d = dim_sq(t5[0], nd.lo[0], nd.hi[0], d);
+        d = dim_sq(t5[1], nd.lo[1], nd.hi[1], d);
+        d = dim_sq(t5[2], nd.lo[2], nd.hi[2], d);
+        d = dim_sq(t5[3], nd.lo[3], nd.hi[3], d);
+        d = dim_sq(t5[4], nd.lo[5], nd.hi[5], d);
+        d = dim_sq(dep, nd.lo[4], nd.hi[4], d);
+        if (d < bd) {
+            bd = d;
+            biome = nd.leaf & 0xFF;
+        }
+    }
+    return biome;
+}
+
 __device__ __forceinline__ int quantize(float v) { return (int)(v * 10000.0f); }   // mc_climate.h:93
 
 // ================================================================================================
@@ -681,6 +727,7 @@ __device__ __forceinline__ void stage_perm(uint8_t *lds, const McbSlot *slot, in
 // columns (lx, lz*cpl + c) in order, each column's search warm-started from the previous one.
 template <int MODE, int DIM, int MAXL>
 __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                              const uint32_t *__restrict__ cells, const uint16_t *__restrict__ clist,
                                               const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
                                               const McbTile *__restrict__ tiles, int ntiles, int tile_w, int cpl,
                                               uint8_t *__restrict__ out) {
@@ -721,9 +768,25 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                 ow_column<MODE>(W, perm, oct, qx, qz, col);
                 const int t5[5] = {col.t, col.h, col.c, col.e, col.w};
                 int dep[MAXL], b[MAXL];
+                bool in = cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) && cell_in_range(t5[3]) &&
+                          cell_in_range(t5[4]);
 #pragma unroll
-                for (int j = 0; j < MAXL; j++) dep[j] = j < ny ? ow_depth<MODE>(W, col, R.y[j]) : 0;
-                tree_search_layers<MAXL>(tree, tree_n, t5, dep, ny, cand, b);
+                for (int j = 0; j < MAXL; j++) {
+                    dep[j] = j < ny ? ow_depth<MODE>(W, col, R.y[j]) : 0;
+                    in = in && cell_in_range(dep[j]);
+                }
+                if (in) {
+                    const uint32_t c5 = cell_base(W, t5);
+#pragma unroll
+                    for (int j = 0; j < MAXL; j++) {
+                        if (j < ny) {
+                            const uint32_t e = cells[c5 + (uint32_t)cell_coord(W, 4, dep[j])];
+                            if (e & 255u) b[j] = cell_best(tree, clist, e, t5, dep[j]);
+                            else in = false;
+                        }
+                    }
+                }
+                if (!in) tree_search_layers<MAXL>(tree, tree_n, t5, dep, ny, cand, b);
 #pragma unroll
                 for (int j = 0; j < MAXL; j++)
                     if (j < ny) o[j * plane] = (uint8_t)b[j];
@@ -847,6 +910,7 @@ __global__ __launch_bounds__(256) void k_grid_end(const McbWorld *__restrict__ W
 // ================================================================================================
 template <int MODE, int DIM>
 __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                                const uint32_t *__restrict__ cells, const uint16_t *__restrict__ clist,
                                                 const McbSlot *__restrict__ slots, const int32_t *__restrict__ slot_of,
                                                 const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
                                                 int32_t *__restrict__ tg) {
@@ -868,8 +932,17 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
         ow_column<MODE>(W, perm, oct, qx, qz, c);
         t[0] = c.t; t[1] = c.h; t[2] = c.c; t[3] = c.e; t[5] = c.w;
         t[4] = ow_depth<MODE>(W, c, qy);
-        int cand = -1;
-        b = tree_search(tree, W->tree_n, t, cand);
+        const int t5[5] = {c.t, c.h, c.c, c.e, c.w};
+        uint32_t e = 0;
+        if (cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) && cell_in_range(t5[3]) &&
+            cell_in_range(t5[4]) && cell_in_range(t[4]))
+            e = cells[cell_base(W, t5) + (uint32_t)cell_coord(W, 4, t[4])];
+        if (e & 255u) {
+            b = cell_best(tree, clist, e, t5, t[4]);
+        } else {
+            int cand = -1;
+            b = tree_search(tree, W->tree_n, t, cand);
+        }
     } else if (DIM == MCB_NETHER) {
         ne_column<MODE>(W, perm, oct, qx, qz, t);
         int cand = -1;
@@ -880,6 +953,33 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
     out[o] = (uint8_t)b;
     if (tg)
         for (int k = 0; k < 6; k++) tg[o * 6 + k] = t[k];
+}
+
+// ================================================================================================
+// biomes of quantized climate targets {t,h,c,e,d,w} (the search step alone, engine/mc_biomes.h:31-62)
+// ================================================================================================
+template <int DIM>
+__global__ __launch_bounds__(256) void k_targets(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                                 const uint32_t *__restrict__ cells, const uint16_t *__restrict__ clist,
+                                                 const int32_t *__restrict__ tg, int n, uint8_t *__restrict__ out) {
+    const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (i >= n) return;
+    int t[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) t[k] = tg[6 * i + k];
+    int b;
+    uint32_t e = 0;
+    const int t5[5] = {t[0], t[1], t[2], t[3], t[5]};
+    if (DIM == MCB_OVERWORLD && cell_in_range(t[0]) && cell_in_range(t[1]) && cell_in_range(t[2]) && cell_in_range(t[3]) &&
+        cell_in_range(t[4]) && cell_in_range(t[5]))
+        e = cells[cell_base(W, t5) + (uint32_t)cell_coord(W, 4, t[4])];
+    if (e & 255u) {
+        b = cell_best(tree, clist, e, t5, t[4]);
+    } else {
+        int cand = -1;
+        b = tree_search(tree, W->tree_n, t, cand);
+    }
+    out[i] = (uint8_t)b;
 }
 
 // ================================================================================================
@@ -942,8 +1042,8 @@ hipError_t grid_launch3(const DevWorld &w, const McbSlot *slots, const McbReques
         hipLaunchKernelGGL((k_grid_end<MODE>), dim3(blocks), dim3(256), 0, st, w.d_world, slots, reqs, tiles, ntiles, tile_w,
                            cpl, out);
     else
-        hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, slots, reqs, tiles,
-                           ntiles, tile_w, cpl, out);
+        hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
+                           w.d_cell_list, slots, reqs, tiles, ntiles, tile_w, cpl, out);
     return hipGetLastError();
 }
 template <int MODE, int DIM>
@@ -961,8 +1061,8 @@ hipError_t grid_launch(const DevWorld &w, const McbSlot *slots, const McbRequest
 template <int MODE, int DIM>
 hipError_t points_launch(const DevWorld &w, const McbSlot *slots, const int32_t *slot_of, int ns, const int32_t *pts, int np,
                          uint8_t *out, int32_t *tg, hipStream_t st) {
-    hipLaunchKernelGGL((k_points<MODE, DIM>), dim3((np + 255) / 256, ns), dim3(256), 0, st, w.d_world, w.d_tree, slots,
-                       slot_of, pts, np, out, tg);
+    hipLaunchKernelGGL((k_points<MODE, DIM>), dim3((np + 255) / 256, ns), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
+                       w.d_cell_list, slots, slot_of, pts, np, out, tg);
     return hipGetLastError();
 }
 
@@ -1011,6 +1111,19 @@ hipError_t launch_points(const DevWorld &w, const McbSlot *slots, const int32_t 
         default: return f ? points_launch<1, MCB_END>(w, slots, slot_of, ns, pts, np, out, tg, st)
                           : points_launch<0, MCB_END>(w, slots, slot_of, ns, pts, np, out, tg, st);
     }
+}
+
+hipError_t launch_targets(const DevWorld &w, const int32_t *tg, int n, uint8_t *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (w.dim == MCB_OVERWORLD)
+        hipLaunchKernelGGL(k_targets<MCB_OVERWORLD>, dim3((n + 255) / 256), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
+                           w.d_cell_list, tg, n, out);
+    else if (w.dim == MCB_NETHER)
+        hipLaunchKernelGGL(k_targets<MCB_NETHER>, dim3((n + 255) / 256), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
+                           w.d_cell_list, tg, n, out);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_zoom(const McbRequest *zreqs, const McbRequest *qreqs, const McbTile *tiles, int ntiles, const uint8_t *quart,

@@ -49,6 +49,8 @@ def lib():
         L.mcb_open.argtypes = [C.POINTER(P), C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int]
         L.mcb_close.argtypes = [P]
         L.mcb_engine_info.argtypes = [P, i32p, i32p, i32p, i32p]
+        L.mcb_engine_cells.argtypes = [P, i64p, i64p, C.POINTER(C.c_double)]
+        L.mcb_targets.argtypes = [P, i32p, C.c_int, u8p]
         L.mcb_points.argtypes = [P, i64p, C.c_int, i32p, C.c_int, u8p, i32p]
         L.mcb_points_dev.argtypes = [P, P, C.c_int, P, C.c_int, P, P, P]
         L.mcb_regions_bytes.restype = C.c_int64
@@ -201,6 +203,19 @@ class Engine:
         v, d, m, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
         _check(lib().mcb_engine_info(self.h, C.byref(v), C.byref(d), C.byref(m), C.byref(n)))
         self.mode, self.tree_nodes = m.value, n.value
+
+    def cells(self):
+        """(cells, list entries, build ms) of the Overworld climate cell table; zeros elsewhere."""
+        c, l, ms = C.c_int64(), C.c_int64(), C.c_double()
+        _check(lib().mcb_engine_cells(self.h, C.byref(c), C.byref(l), C.byref(ms)))
+        return c.value, l.value, ms.value
+
+    def targets(self, tg):
+        """Biomes of quantized climate targets (n x {t,h,c,e,d,w})."""
+        t = np.ascontiguousarray(tg, dtype=np.int32).reshape(-1, 6)
+        out = np.zeros(t.shape[0], np.uint8)
+        _check(lib().mcb_targets(self.h, _p(t, C.c_int32), t.shape[0], _p(out, C.c_uint8)))
+        return out
 
     def points(self, seeds, pts, want_tg=False):
         seeds = np.ascontiguousarray(seeds, dtype=np.int64)

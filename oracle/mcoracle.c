@@ -1186,6 +1186,14 @@ int or_tree_serialize(const OrWorld *w, int64_t *buf, int cap) {
     return n < 0 ? -1 : n / 16;
 }
 
+void or_targets(const OrWorld *w, const int32_t *tg6, int n, uint8_t *out) {   /* engine/mc_biomes.h:31-62 */
+    #pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        i64 t[RT_DIM] = { tg6[6*i], tg6[6*i+1], tg6[6*i+2], tg6[6*i+3], tg6[6*i+4], tg6[6*i+5], 0 };
+        out[i] = (uint8_t)w->tree.node[rt_search(&w->tree, w->tree.root, t, -1)].biome;
+    }
+}
+
 int or_tie_mask(const OrWorld *w, const int32_t tg6[6], uint64_t mask[2]) {   /* engine/cuda/mcg_host.c:85-96 */
     i64 t[RT_DIM] = { tg6[0], tg6[1], tg6[2], tg6[3], tg6[4], tg6[5], 0 };
     i64 best = INT64_MAX; int cnt = 0;

@@ -80,6 +80,8 @@ void or_zoom(int64_t hashed, int x, int y, int z, int32_t q[3]);
 int or_tree_serialize(const OrWorld *w, int64_t *buf, int cap);
 /* Brute-force: all leaves at minimum distance; returns the count, biome mask in mask2[2]. */
 int or_tie_mask(const OrWorld *w, const int32_t tg6[6], uint64_t mask2[2]);
+/* biome of quantized targets {t,h,c,e,d,w} by the tree search (engine/mc_biomes.h:31-62), n targets */
+void or_targets(const OrWorld *w, const int32_t *tg6, int n, uint8_t *out);
 /* Leaves visited per search (instrumentation): average over the given targets. */
 double or_tree_visits(const OrWorld *w, const int32_t *tg6, int n);
 

@@ -56,6 +56,7 @@ def lib():
         L.or_zoom.argtypes = [C.c_int64, C.c_int, C.c_int, C.c_int, i32p]
         L.or_tree_serialize.argtypes = [P, i64p, C.c_int]
         L.or_tie_mask.argtypes = [P, i32p, C.POINTER(C.c_uint64)]
+        L.or_targets.argtypes = [P, i32p, C.c_int, u8p]
         L.or_tree_visits.restype = C.c_double
         L.or_tree_visits.argtypes = [P, i32p, C.c_int]
         L.or_splitmix64_next.restype = C.c_uint64
@@ -173,6 +174,13 @@ class Oracle:
         m = (C.c_uint64 * 2)()
         cnt = lib().or_tie_mask(self.h, _p(t, C.c_int32), m)
         return cnt, (m[0], m[1])
+
+    def targets(self, tg):
+        """Biomes of quantized targets (n x {t,h,c,e,d,w}) by the restated tree search."""
+        t = np.ascontiguousarray(tg, dtype=np.int32).reshape(-1, 6)
+        out = np.zeros(t.shape[0], np.uint8)
+        lib().or_targets(self.h, _p(t, C.c_int32), t.shape[0], _p(out, C.c_uint8))
+        return out
 
     def tree_visits(self, tg):
         t = np.ascontiguousarray(tg, dtype=np.int32).reshape(-1, 6)
