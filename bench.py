@@ -374,6 +374,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--mixed-requests", type=int, default=10000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (reference for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -383,6 +385,12 @@ def main():
     import mcbiome
     assert torch.cuda.is_available(), "bench needs a HIP device"
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        src = torch.ones(256 << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve}[args.workload]
     print(json.dumps(fn(args, torch, mcbiome)))
 
