@@ -85,11 +85,7 @@ int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {
     if (!ct) return fail(MCB_E_DATA, "cell table: %s", err.c_str());
     McbWorld &W = e->w.host;
     int n[6];
-    for (int k = 0; k < 6; k++) {
-        W.cell_nb[k] = ct->nb[k];
-        n[k] = ct->nb[k] + 1;
-        for (int i = 0; i < ct->nb[k]; i++) W.cell_bp[k][i] = ct->bp[ct->off[k] + i];
-    }
+    for (int k = 0; k < 6; k++) n[k] = ct->nb[k] + 1;
     // index = (((((cT*nH + cH)*nC + cC)*nE + cE)*nW + cW)*nD + cD)
     W.cell_stride[4] = 1;
     W.cell_stride[5] = n[4];
@@ -98,6 +94,25 @@ int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {
     W.cell_stride[1] = n[2] * W.cell_stride[2];
     W.cell_stride[0] = n[1] * W.cell_stride[1];
     if ((size_t)n[0] * W.cell_stride[0] != ct->entry.size()) return fail(MCB_E_DATA, "cell table size mismatch");
+    // locator buckets of 256 values; the table is used only when no bucket holds two breakpoints and
+    // every cell offset fits 20 bits
+    W.cell_on = ct->entry.size() <= (1u << 20);
+    for (int k = 0; k < 6; k++) {
+        const int32_t *bp = ct->bp.data() + ct->off[k];
+        for (int b = 0; b < 256; b++) {
+            const int t0 = b * 256 - 32768;
+            int below = 0, inside = 0, off = 256;
+            for (int i = 0; i < ct->nb[k]; i++) {
+                if (bp[i] <= t0) below++;
+                else if (bp[i] <= t0 + 255) {
+                    inside++;
+                    off = bp[i] - t0;
+                }
+            }
+            if (inside > 1) W.cell_on = 0;
+            W.cell_lut[k][b] = (uint32_t)below * (uint32_t)W.cell_stride[k] | (uint32_t)off << 20;
+        }
+    }
     e->w.cells_build_ms = ct->build_ms;
     if (hipMalloc((void **)&e->w.d_cells, ct->entry.size() * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc((void **)&e->w.d_cell_list, std::max<size_t>(ct->list.size(), 1) * sizeof(uint16_t)) != hipSuccess)

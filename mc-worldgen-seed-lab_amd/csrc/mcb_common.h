@@ -15,6 +15,7 @@
 #define MCB_SPL_NODES 320
 #define MCB_SPL_PTS 320
 #define MCB_CELL_MAX_BP 24      /* breakpoints per climate dimension in the cell table */
+#define MCB_CELL_LUT (6 * 256)  /* cell locator entries */
 
 enum { MCB_V26_1 = 0, MCB_V26_2 = 1, MCB_V26_3 = 2 };
 enum { MCB_OVERWORLD = 0, MCB_NETHER = 1, MCB_END = 2 };
@@ -66,9 +67,12 @@ typedef struct {
     float rf_f[3];
     int32_t end_ids[5];
     int32_t tree_n;                 /* nodes in the pre-order array (root = 0) */
-    int32_t cell_nb[6];             /* cell table: breakpoints per tree dimension */
+    int32_t cell_on;                /* 1: Overworld cell table present (else every search walks the tree) */
     int32_t cell_stride[6];         /* cell index = sum of cell_stride[k] * (cell of dimension k) */
-    int32_t cell_bp[6][MCB_CELL_MAX_BP];
+    /* cell locator, per tree dimension k and bucket b of 256 target values (t + 32768) >> 8:
+     * (cells below the bucket's first value) * cell_stride[k] | (offset in the bucket of the one
+     * breakpoint inside it, 256 = none) << 20 */
+    uint32_t cell_lut[6][256];
     uint64_t jm262, ja262;          /* LCG jump by 262 draws (LegacyFbm skipped octave) */
     uint64_t jm_end, ja_end;        /* LCG jump by 17292 draws (End island noise) */
     McbSpline spline;

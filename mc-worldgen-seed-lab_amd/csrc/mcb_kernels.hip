@@ -207,9 +207,14 @@ __global__ __launch_bounds__(64) void k_seed_end(const McbWorld *__restrict__ W,
 }
 
 // ================================================================================================
-// octave evaluation (engine/mc_noise.h:54-110) on a permutation table in LDS
+// octave evaluation (engine/mc_noise.h:54-110) on permutation tables in LDS
 // ================================================================================================
+// The grid and points kernels stage each octave's 256-entry permutation twice in a row (512 bytes),
+// so P2[i] = perm[i & 255] for every index the lattice hash forms (a + y + 1 <= 511): no masks, and the
+// +1 neighbours are immediate offsets of the same address.  The End's simplex noise keeps the plain
+// 256-byte table.
 #define PERM(i) ((int)P[(i) & 255])
+#define OCT_BYTES 512
 
 // gradient table of ImprovedNoise (16 x {gx,gy,gz}), each component a 2-bit two's complement field
 constexpr int kGrad[16][3] = {{1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
@@ -222,13 +227,29 @@ constexpr uint32_t grad_bits(int c) {
 constexpr uint32_t kGX = grad_bits(0), kGY = grad_bits(1), kGZ = grad_bits(2);
 __device__ __forceinline__ int gcomp(uint32_t bits, int h) { return __builtin_amdgcn_sbfe((int)bits, 2 * h, 2); }
 
-__device__ __forceinline__ float gdot_f(int h, float x, float y, float z) {
-    h &= 15;
-    return (float)gcomp(kGX, h) * x + (float)gcomp(kGY, h) * y + (float)gcomp(kGZ, h) * z;
+// LDS gradient entry for a permutation value h (the vector of h & 15): gx as f32, gy and gz as f16
+// (exact for -1, 0, 1), so one 8-byte read feeds the dot product through mixed-precision FMAs.
+struct GradEnt {
+    float x;
+    _Float16 y, z;
+};
+__device__ __forceinline__ void init_grad(GradEnt *G) {
+    for (int h = threadIdx.x; h < 256; h += blockDim.x) {
+        const int k = h & 15;
+        G[h].x = (float)gcomp(kGX, k);
+        G[h].y = (_Float16)gcomp(kGY, k);
+        G[h].z = (_Float16)gcomp(kGZ, k);
+    }
+}
+
+// g . (x, y, z) as the reference's gx*x + gy*y + gz*z.  With g in {-1, 0, 1} every product is exact, so
+// a fused multiply-add rounds exactly like the separate product and sum (same value, same signed zero).
+__device__ __forceinline__ float gdot_f(const GradEnt &g, float x, float y, float z) {
+    return fmaf((float)g.z, z, fmaf((float)g.y, y, g.x * x));
 }
 __device__ __forceinline__ double gdot_d(int h, double x, double y, double z) {
     h &= 15;
-    return (double)gcomp(kGX, h) * x + (double)gcomp(kGY, h) * y + (double)gcomp(kGZ, h) * z;
+    return fma((double)gcomp(kGZ, h), z, fma((double)gcomp(kGY, h), y, (double)gcomp(kGX, h) * x));
 }
 __device__ __forceinline__ float lerp_f(float t, float a, float b) { return a + t * (b - a); }
 __device__ __forceinline__ double lerp_d(double t, double a, double b) { return a + t * (b - a); }
@@ -239,85 +260,70 @@ __device__ __forceinline__ double wrapc(double x) {
     return x;
 }
 
-// Octave whose y coordinate is the octave offset (climate noises, ShiftA): x, z vary.
 template <int MODE> struct R;
 template <> struct R<1> { typedef float T; };
 template <> struct R<0> { typedef double T; };
 
+// Trilinear blend of the 8 lattice corners (the reference's nesting: x, then y, then z).  hXYZ = hash
+// of corner (X, Y, Z); f* = fractions; u, v, w = fades along x, y, z.
 template <int MODE>
-__device__ __forceinline__ typename R<MODE>::T oct_yc(const uint8_t *P, const McbOct &o, double X, double Z) {
-    typedef typename R<MODE>::T T;
-    double x = wrapc(X) + o.xo, z = wrapc(Z) + o.zo;
-    double xf = floor(x), zf = floor(z);
-    int xi = (int)xf, zi = (int)zf, yi = o.yi;
-    T fx = (T)(x - xf), fz = (T)(z - zf);
-    T fy = MODE ? (T)o.fy_f : (T)o.fy_d, wy = MODE ? (T)o.wy_f : (T)o.wy_d;
-    int a = PERM(xi), b = PERM(xi + 1);
-    int aa = PERM(a + yi), ab = PERM(a + yi + 1), ba = PERM(b + yi), bb = PERM(b + yi + 1);
-    T c000, c100, c010, c110, c001, c101, c011, c111;
-    if (MODE) {
-        c000 = gdot_f(PERM(aa + zi), fx, fy, fz);
-        c100 = gdot_f(PERM(ba + zi), fx - 1.0f, fy, fz);
-        c010 = gdot_f(PERM(ab + zi), fx, fy - 1.0f, fz);
-        c110 = gdot_f(PERM(bb + zi), fx - 1.0f, fy - 1.0f, fz);
-        c001 = gdot_f(PERM(aa + zi + 1), fx, fy, fz - 1.0f);
-        c101 = gdot_f(PERM(ba + zi + 1), fx - 1.0f, fy, fz - 1.0f);
-        c011 = gdot_f(PERM(ab + zi + 1), fx, fy - 1.0f, fz - 1.0f);
-        c111 = gdot_f(PERM(bb + zi + 1), fx - 1.0f, fy - 1.0f, fz - 1.0f);
-        float u = fade_f(fx), w = fade_f(fz);
-        return lerp_f(w, lerp_f(wy, lerp_f(u, c000, c100), lerp_f(u, c010, c110)),
-                      lerp_f(wy, lerp_f(u, c001, c101), lerp_f(u, c011, c111)));
+__device__ __forceinline__ typename R<MODE>::T corners(const GradEnt *G, int h000, int h100, int h010, int h110, int h001,
+                                                       int h101, int h011, int h111, typename R<MODE>::T fx,
+                                                       typename R<MODE>::T fy, typename R<MODE>::T fz,
+                                                       typename R<MODE>::T u, typename R<MODE>::T v, typename R<MODE>::T w) {
+    if constexpr (MODE == 1) {
+        const float gx = fx - 1.0f, gy = fy - 1.0f, gz = fz - 1.0f;
+        const float c000 = gdot_f(G[h000], fx, fy, fz), c100 = gdot_f(G[h100], gx, fy, fz);
+        const float c010 = gdot_f(G[h010], fx, gy, fz), c110 = gdot_f(G[h110], gx, gy, fz);
+        const float c001 = gdot_f(G[h001], fx, fy, gz), c101 = gdot_f(G[h101], gx, fy, gz);
+        const float c011 = gdot_f(G[h011], fx, gy, gz), c111 = gdot_f(G[h111], gx, gy, gz);
+        return lerp_f(w, lerp_f(v, lerp_f(u, c000, c100), lerp_f(u, c010, c110)),
+                      lerp_f(v, lerp_f(u, c001, c101), lerp_f(u, c011, c111)));
     } else {
-        c000 = gdot_d(PERM(aa + zi), fx, fy, fz);
-        c100 = gdot_d(PERM(ba + zi), fx - 1.0, fy, fz);
-        c010 = gdot_d(PERM(ab + zi), fx, fy - 1.0, fz);
-        c110 = gdot_d(PERM(bb + zi), fx - 1.0, fy - 1.0, fz);
-        c001 = gdot_d(PERM(aa + zi + 1), fx, fy, fz - 1.0);
-        c101 = gdot_d(PERM(ba + zi + 1), fx - 1.0, fy, fz - 1.0);
-        c011 = gdot_d(PERM(ab + zi + 1), fx, fy - 1.0, fz - 1.0);
-        c111 = gdot_d(PERM(bb + zi + 1), fx - 1.0, fy - 1.0, fz - 1.0);
-        double u = fade_d(fx), w = fade_d(fz);
-        return lerp_d(w, lerp_d(wy, lerp_d(u, c000, c100), lerp_d(u, c010, c110)),
-                      lerp_d(wy, lerp_d(u, c001, c101), lerp_d(u, c011, c111)));
+        const double gx = fx - 1.0, gy = fy - 1.0, gz = fz - 1.0;
+        const double c000 = gdot_d(h000, fx, fy, fz), c100 = gdot_d(h100, gx, fy, fz);
+        const double c010 = gdot_d(h010, fx, gy, fz), c110 = gdot_d(h110, gx, gy, fz);
+        const double c001 = gdot_d(h001, fx, fy, gz), c101 = gdot_d(h101, gx, fy, gz);
+        const double c011 = gdot_d(h011, fx, gy, gz), c111 = gdot_d(h111, gx, gy, gz);
+        return lerp_d(w, lerp_d(v, lerp_d(u, c000, c100), lerp_d(u, c010, c110)),
+                      lerp_d(v, lerp_d(u, c001, c101), lerp_d(u, c011, c111)));
     }
+}
+
+// Octave whose y coordinate is the octave offset (climate noises, ShiftA): x, z vary.
+template <int MODE>
+__device__ __forceinline__ typename R<MODE>::T oct_yc(const uint8_t *P2, const GradEnt *G, const McbOct &o, double X,
+                                                      double Z) {
+    typedef typename R<MODE>::T T;
+    const double x = wrapc(X) + o.xo, z = wrapc(Z) + o.zo;
+    const double xf = floor(x), zf = floor(z);
+    const int xi = (int)xf & 255, zi = (int)zf & 255, yi = o.yi & 255;
+    const T fx = (T)(x - xf), fz = (T)(z - zf);
+    const T fy = MODE ? (T)o.fy_f : (T)o.fy_d, wy = MODE ? (T)o.wy_f : (T)o.wy_d;
+    const int a = P2[xi], b = P2[xi + 1];
+    const int aa = P2[a + yi], ab = P2[a + yi + 1], ba = P2[b + yi], bb = P2[b + yi + 1];
+    const T u = MODE ? (T)fade_f((float)fx) : (T)fade_d((double)fx);
+    const T w = MODE ? (T)fade_f((float)fz) : (T)fade_d((double)fz);
+    return corners<MODE>(G, P2[aa + zi], P2[ba + zi], P2[ab + zi], P2[bb + zi], P2[aa + zi + 1], P2[ba + zi + 1],
+                         P2[ab + zi + 1], P2[bb + zi + 1], fx, fy, fz, u, wy, w);
 }
 
 // Octave whose z coordinate is the octave offset (ShiftB): x, y vary.
 template <int MODE>
-__device__ __forceinline__ typename R<MODE>::T oct_zc(const uint8_t *P, const McbOct &o, double X, double Y) {
+__device__ __forceinline__ typename R<MODE>::T oct_zc(const uint8_t *P2, const GradEnt *G, const McbOct &o, double X,
+                                                      double Y) {
     typedef typename R<MODE>::T T;
-    double x = wrapc(X) + o.xo, y = wrapc(Y) + o.yo;
-    double xf = floor(x), yf = floor(y);
-    int xi = (int)xf, yi = (int)yf, zi = o.zi;
-    T fx = (T)(x - xf), fy = (T)(y - yf);
-    T fz = MODE ? (T)o.fz_f : (T)o.fz_d, wz = MODE ? (T)o.wz_f : (T)o.wz_d;
-    int a = PERM(xi), b = PERM(xi + 1);
-    int aa = PERM(a + yi), ab = PERM(a + yi + 1), ba = PERM(b + yi), bb = PERM(b + yi + 1);
-    if (MODE) {
-        float c000 = gdot_f(PERM(aa + zi), fx, fy, fz);
-        float c100 = gdot_f(PERM(ba + zi), fx - 1.0f, fy, fz);
-        float c010 = gdot_f(PERM(ab + zi), fx, fy - 1.0f, fz);
-        float c110 = gdot_f(PERM(bb + zi), fx - 1.0f, fy - 1.0f, fz);
-        float c001 = gdot_f(PERM(aa + zi + 1), fx, fy, fz - 1.0f);
-        float c101 = gdot_f(PERM(ba + zi + 1), fx - 1.0f, fy, fz - 1.0f);
-        float c011 = gdot_f(PERM(ab + zi + 1), fx, fy - 1.0f, fz - 1.0f);
-        float c111 = gdot_f(PERM(bb + zi + 1), fx - 1.0f, fy - 1.0f, fz - 1.0f);
-        float u = fade_f(fx), v = fade_f(fy);
-        return lerp_f(wz, lerp_f(v, lerp_f(u, c000, c100), lerp_f(u, c010, c110)),
-                      lerp_f(v, lerp_f(u, c001, c101), lerp_f(u, c011, c111)));
-    } else {
-        double c000 = gdot_d(PERM(aa + zi), fx, fy, fz);
-        double c100 = gdot_d(PERM(ba + zi), fx - 1.0, fy, fz);
-        double c010 = gdot_d(PERM(ab + zi), fx, fy - 1.0, fz);
-        double c110 = gdot_d(PERM(bb + zi), fx - 1.0, fy - 1.0, fz);
-        double c001 = gdot_d(PERM(aa + zi + 1), fx, fy, fz - 1.0);
-        double c101 = gdot_d(PERM(ba + zi + 1), fx - 1.0, fy, fz - 1.0);
-        double c011 = gdot_d(PERM(ab + zi + 1), fx, fy - 1.0, fz - 1.0);
-        double c111 = gdot_d(PERM(bb + zi + 1), fx - 1.0, fy - 1.0, fz - 1.0);
-        double u = fade_d(fx), v = fade_d(fy);
-        return lerp_d(wz, lerp_d(v, lerp_d(u, c000, c100), lerp_d(u, c010, c110)),
-                      lerp_d(v, lerp_d(u, c001, c101), lerp_d(u, c011, c111)));
-    }
+    const double x = wrapc(X) + o.xo, y = wrapc(Y) + o.yo;
+    const double xf = floor(x), yf = floor(y);
+    const int xi = (int)xf & 255, yi = (int)yf & 255, zi = o.zi & 255;
+    const T fx = (T)(x - xf), fy = (T)(y - yf);
+    const T fz = MODE ? (T)o.fz_f : (T)o.fz_d, wz = MODE ? (T)o.wz_f : (T)o.wz_d;
+    const int a = P2[xi], b = P2[xi + 1];
+    const int aa = P2[a + yi], ab = P2[a + yi + 1], ba = P2[b + yi], bb = P2[b + yi + 1];
+    const T u = MODE ? (T)fade_f((float)fx) : (T)fade_d((double)fx);
+    const T v = MODE ? (T)fade_f((float)fy) : (T)fade_d((double)fy);
+    return corners<MODE>(G, P2[aa + zi], P2[ba + zi], P2[ab + zi], P2[bb + zi], P2[aa + zi + 1], P2[ba + zi + 1],
+                         P2[ab + zi + 1], P2[bb + zi + 1], fx, fy, fz, u, v, wz);
 }
 
 #define INPUT_FACTOR 1.0181268882175227
@@ -326,11 +332,11 @@ __device__ __forceinline__ typename R<MODE>::T oct_zc(const uint8_t *P, const Mc
 // accumulates in float (NoiseStack.get, mc_noise.h:303-312; legacy-nether order :313-321); 26.1/26.2
 // sums each stack in double, then (v1+v2)*valueFactor (mc_noise.h:286-301).
 template <int MODE, bool ZC>
-__device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm, const McbOct *oct, const McbNoise &N, int base,
-                                                       double X, double Y) {
+__device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm2, const GradEnt *G, const McbOct *oct,
+                                                       const McbNoise &N, int base, double X, double Y) {
     const int n = N.nlev;
     auto ev = [&](int o, double a, double b) {
-        return ZC ? oct_zc<MODE>(perm + o * 256, oct[o], a, b) : oct_yc<MODE>(perm + o * 256, oct[o], a, b);
+        return ZC ? oct_zc<MODE>(perm2 + o * OCT_BYTES, G, oct[o], a, b) : oct_yc<MODE>(perm2 + o * OCT_BYTES, G, oct[o], a, b);
     };
     if (MODE) {
         float v = 0.0f;
@@ -526,21 +532,23 @@ __device__ __forceinline__ void tree_search_layers(const McbNode *__restrict__ N
 // minimum, which is the tree search's result.  Targets beyond +-kCellTMax walk the tree instead.
 // ================================================================================================
 #define CELL_TMAX 32767
-// cell of target t along tree dimension k: the number of breakpoints <= t (breakpoints are uniform)
-__device__ __forceinline__ int cell_coord(const McbWorld *__restrict__ W, int k, int t) {
-    const int n = W->cell_nb[k];
-    int c = 0;
-#pragma unroll
-    for (int i = 0; i < MCB_CELL_MAX_BP; i++)
-        if (i < n) c += t >= W->cell_bp[k][i] ? 1 : 0;
-    return c;
-}
 __device__ __forceinline__ bool cell_in_range(int t) { return t >= -CELL_TMAX && t <= CELL_TMAX; }
+// Offset of target t's cell along tree dimension k (see McbWorld.cell_lut): the bucket entry gives the
+// cells below the bucket, plus one stride when t is at or past the bucket's breakpoint.
+__device__ __forceinline__ uint32_t cell_part(const uint32_t *lut, int k, int t, uint32_t stride) {
+    const uint32_t u = (uint32_t)(t + 32768);
+    const uint32_t e = lut[k * 256 + (int)(u >> 8)];
+    return (e & 0xFFFFFu) + ((u & 255u) >= (e >> 20) ? stride : 0u);
+}
 // t5 = temperature, humidity, continentalness, erosion, weirdness; depth separately (tree dimension 4)
-__device__ __forceinline__ uint32_t cell_base(const McbWorld *__restrict__ W, const int *t5) {
-    return (uint32_t)(cell_coord(W, 0, t5[0]) * W->cell_stride[0] + cell_coord(W, 1, t5[1]) * W->cell_stride[1] +
-                      cell_coord(W, 2, t5[2]) * W->cell_stride[2] + cell_coord(W, 3, t5[3]) * W->cell_stride[3] +
-                      cell_coord(W, 5, t5[4]) * W->cell_stride[5]);
+__device__ __forceinline__ uint32_t cell_base(const McbWorld *__restrict__ W, const uint32_t *lut, const int *t5) {
+    return cell_part(lut, 0, t5[0], W->cell_stride[0]) + cell_part(lut, 1, t5[1], W->cell_stride[1]) +
+           cell_part(lut, 2, t5[2], W->cell_stride[2]) + cell_part(lut, 3, t5[3], W->cell_stride[3]) +
+           cell_part(lut, 5, t5[4], W->cell_stride[5]);
+}
+__device__ __forceinline__ uint32_t cell_depth(const uint32_t *lut, int dep) { return cell_part(lut, 4, dep, 1u); }
+__device__ __forceinline__ void stage_lut(uint32_t *lds, const McbWorld *__restrict__ W) {
+    for (int i = threadIdx.x; i < MCB_CELL_LUT; i += blockDim.x) lds[i] = W->cell_lut[0][i];
 }
 // biome of the least (distance, index) leaf of a cell entry (count > 0)
 __device__ __forceinline__ int cell_best(const McbNode *__restrict__ N, const uint16_t *__restrict__ list, uint32_t e,
@@ -578,33 +586,33 @@ struct Column {
 };
 
 template <int MODE>
-__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const uint8_t *perm, const McbOct *oct, int qx, int qz,
-                                          Column &c) {
+__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const uint8_t *perm, const GradEnt *G,
+                                          const McbOct *oct, int qx, int qz, Column &c) {
     const McbNoise *N = W->noise;
     const double X = (double)qx, Z = (double)qz;     // blockX*0.25 with blockX = 4*quartX
     if (MODE) {
-        float sx = normal2<1, false>(perm, oct, N[0], W->oct_base[0], X, Z) * 4.0f;
-        float sz = normal2<1, true>(perm, oct, N[0], W->oct_base[0], Z, X) * 4.0f;
+        float sx = normal2<1, false>(perm, G, oct, N[0], W->oct_base[0], X, Z) * 4.0f;
+        float sz = normal2<1, true>(perm, G, oct, N[0], W->oct_base[0], Z, X) * 4.0f;
         double x = X + (double)sx, z = Z + (double)sz;
-        float T = normal2<1, false>(perm, oct, N[1], W->oct_base[1], x, z);
-        float H = normal2<1, false>(perm, oct, N[2], W->oct_base[2], x, z);
-        float C = normal2<1, false>(perm, oct, N[3], W->oct_base[3], x, z);
-        float E = normal2<1, false>(perm, oct, N[4], W->oct_base[4], x, z);
-        float Wr = normal2<1, false>(perm, oct, N[5], W->oct_base[5], x, z);
+        float T = normal2<1, false>(perm, G, oct, N[1], W->oct_base[1], x, z);
+        float H = normal2<1, false>(perm, G, oct, N[2], W->oct_base[2], x, z);
+        float C = normal2<1, false>(perm, G, oct, N[3], W->oct_base[3], x, z);
+        float E = normal2<1, false>(perm, G, oct, N[4], W->oct_base[4], x, z);
+        float Wr = normal2<1, false>(perm, G, oct, N[5], W->oct_base[5], x, z);
         float a = fabsf(Wr) + W->rf_f[0];
         float rf = (fabsf(a) + W->rf_f[1]) * W->rf_f[2];
         float in4[4] = {C, E, Wr, rf};
         c.off_f = W->offc_f + spline_eval(W->spline, 0, in4);
         c.t = quantize(T); c.h = quantize(H); c.c = quantize(C); c.e = quantize(E); c.w = quantize(Wr);
     } else {
-        double sx = normal2<0, false>(perm, oct, N[0], W->oct_base[0], X, Z) * 4.0;
-        double sz = normal2<0, true>(perm, oct, N[0], W->oct_base[0], Z, X) * 4.0;
+        double sx = normal2<0, false>(perm, G, oct, N[0], W->oct_base[0], X, Z) * 4.0;
+        double sz = normal2<0, true>(perm, G, oct, N[0], W->oct_base[0], Z, X) * 4.0;
         double x = X + sx, z = Z + sz;
-        double T = normal2<0, false>(perm, oct, N[1], W->oct_base[1], x, z);
-        double H = normal2<0, false>(perm, oct, N[2], W->oct_base[2], x, z);
-        double C = normal2<0, false>(perm, oct, N[3], W->oct_base[3], x, z);
-        double E = normal2<0, false>(perm, oct, N[4], W->oct_base[4], x, z);
-        double Wr = normal2<0, false>(perm, oct, N[5], W->oct_base[5], x, z);
+        double T = normal2<0, false>(perm, G, oct, N[1], W->oct_base[1], x, z);
+        double H = normal2<0, false>(perm, G, oct, N[2], W->oct_base[2], x, z);
+        double C = normal2<0, false>(perm, G, oct, N[3], W->oct_base[3], x, z);
+        double E = normal2<0, false>(perm, G, oct, N[4], W->oct_base[4], x, z);
+        double Wr = normal2<0, false>(perm, G, oct, N[5], W->oct_base[5], x, z);
         double rf = (fabs(fabs(Wr) + W->rf_d[0]) + W->rf_d[1]) * W->rf_d[2];
         float in4[4] = {(float)C, (float)E, (float)Wr, (float)rf};
         float sp = spline_eval(W->spline, 0, in4);
@@ -631,11 +639,11 @@ __device__ __forceinline__ int ow_depth(const McbWorld *__restrict__ W, const Co
 
 // Nether climate: two legacy NormalNoises at (bx*0.25, by*0.0, bz*0.25) (engine/cuda/mcg_host.c:63-74)
 template <int MODE>
-__device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, const uint8_t *perm, const McbOct *oct, int qx, int qz,
-                                          int *t) {
+__device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, const uint8_t *perm, const GradEnt *G,
+                                          const McbOct *oct, int qx, int qz, int *t) {
     const double X = (double)qx, Z = (double)qz;
-    float T = (float)normal2<MODE, false>(perm, oct, W->noise[0], W->oct_base[0], X, Z);
-    float H = (float)normal2<MODE, false>(perm, oct, W->noise[1], W->oct_base[1], X, Z);
+    float T = (float)normal2<MODE, false>(perm, G, oct, W->noise[0], W->oct_base[0], X, Z);
+    float H = (float)normal2<MODE, false>(perm, G, oct, W->noise[1], W->oct_base[1], X, Z);
     t[0] = quantize(T); t[1] = quantize(H);
     t[2] = t[3] = t[4] = t[5] = 0;
 }
@@ -713,6 +721,18 @@ __device__ int end_biome(const McbWorld *__restrict__ W, const uint8_t *P, int q
 // ================================================================================================
 // LDS staging of a seed slot's permutation tables
 // ================================================================================================
+// doubled tables: octave o at lds[o*512], its 256 bytes twice in a row
+__device__ __forceinline__ void stage_perm2(uint8_t *lds, const McbSlot *slot, int n_oct) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(slot->perm[0]);
+    uint4 *dst = reinterpret_cast<uint4 *>(lds);
+    const int n16 = n_oct * 16;
+    for (int i = threadIdx.x; i < n16; i += blockDim.x) {
+        const uint4 v = src[i];
+        const int o = i >> 4, q = i & 15;
+        dst[o * 32 + q] = v;
+        dst[o * 32 + 16 + q] = v;
+    }
+}
 __device__ __forceinline__ void stage_perm(uint8_t *lds, const McbSlot *slot, int n_oct) {
     const uint4 *src = reinterpret_cast<const uint4 *>(slot->perm[0]);
     uint4 *dst = reinterpret_cast<uint4 *>(lds);
@@ -731,9 +751,13 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                                               const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
                                               const McbTile *__restrict__ tiles, int ntiles, int tile_w, int cpl,
                                               uint8_t *__restrict__ out) {
-    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_SLOT_PERM_BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
+    __shared__ GradEnt G[256];
+    __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
     const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT;
     const int tree_n = W->tree_n;
+    init_grad(G);                                         // both visible after the first staging barrier
+    if (DIM == MCB_OVERWORLD) stage_lut(clut, W);
     const int tile_h = 256 / tile_w;
     int staged = -1;
     // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables
@@ -745,7 +769,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
         const int slot = R.slot;
         if (slot != staged) {
             __syncthreads();
-            stage_perm(perm, slots + slot, n_oct);
+            stage_perm2(perm, slots + slot, n_oct);
             __syncthreads();
             staged = slot;
         }
@@ -765,22 +789,22 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
             uint8_t *o = out + R.out_off + (int64_t)iz * R.nx + ix;
             if (DIM == MCB_OVERWORLD) {
                 Column col;
-                ow_column<MODE>(W, perm, oct, qx, qz, col);
+                ow_column<MODE>(W, perm, G, oct, qx, qz, col);
                 const int t5[5] = {col.t, col.h, col.c, col.e, col.w};
                 int dep[MAXL], b[MAXL];
-                bool in = cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) && cell_in_range(t5[3]) &&
-                          cell_in_range(t5[4]);
+                bool in = W->cell_on && cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) &&
+                          cell_in_range(t5[3]) && cell_in_range(t5[4]);
 #pragma unroll
                 for (int j = 0; j < MAXL; j++) {
                     dep[j] = j < ny ? ow_depth<MODE>(W, col, R.y[j]) : 0;
                     in = in && cell_in_range(dep[j]);
                 }
                 if (in) {
-                    const uint32_t c5 = cell_base(W, t5);
+                    const uint32_t c5 = cell_base(W, clut, t5);
 #pragma unroll
                     for (int j = 0; j < MAXL; j++) {
                         if (j < ny) {
-                            const uint32_t e = cells[c5 + (uint32_t)cell_coord(W, 4, dep[j])];
+                            const uint32_t e = cells[c5 + cell_depth(clut, dep[j])];
                             if (e & 255u) b[j] = cell_best(tree, clist, e, t5, dep[j]);
                             else in = false;
                         }
@@ -792,7 +816,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                     if (j < ny) o[j * plane] = (uint8_t)b[j];
             } else {
                 int t[6];
-                ne_column<MODE>(W, perm, oct, qx, qz, t);
+                ne_column<MODE>(W, perm, G, oct, qx, qz, t);
                 uint8_t b = (uint8_t)tree_search(tree, tree_n, t, cand[0]);
                 for (int j = 0; j < ny; j++) o[j * plane] = b;
             }
@@ -914,11 +938,16 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
                                                 const McbSlot *__restrict__ slots, const int32_t *__restrict__ slot_of,
                                                 const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
                                                 int32_t *__restrict__ tg) {
-    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_SLOT_PERM_BYTES];
+    __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
+    __shared__ GradEnt G[256];
     const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
     const int s = blockIdx.y;
     const int slot = slot_of[s];
-    stage_perm(perm, slots + slot, n_oct);
+    __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
+    if (DIM == MCB_END) stage_perm(perm, slots + slot, 1);
+    else stage_perm2(perm, slots + slot, n_oct);
+    init_grad(G);
+    if (DIM == MCB_OVERWORLD) stage_lut(clut, W);
     __syncthreads();
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= np) return;
@@ -929,14 +958,14 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
     int b;
     if (DIM == MCB_OVERWORLD) {
         Column c;
-        ow_column<MODE>(W, perm, oct, qx, qz, c);
+        ow_column<MODE>(W, perm, G, oct, qx, qz, c);
         t[0] = c.t; t[1] = c.h; t[2] = c.c; t[3] = c.e; t[5] = c.w;
         t[4] = ow_depth<MODE>(W, c, qy);
         const int t5[5] = {c.t, c.h, c.c, c.e, c.w};
         uint32_t e = 0;
-        if (cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) && cell_in_range(t5[3]) &&
+        if (W->cell_on && cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) && cell_in_range(t5[3]) &&
             cell_in_range(t5[4]) && cell_in_range(t[4]))
-            e = cells[cell_base(W, t5) + (uint32_t)cell_coord(W, 4, t[4])];
+            e = cells[cell_base(W, clut, t5) + cell_depth(clut, t[4])];
         if (e & 255u) {
             b = cell_best(tree, clist, e, t5, t[4]);
         } else {
@@ -944,7 +973,7 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
             b = tree_search(tree, W->tree_n, t, cand);
         }
     } else if (DIM == MCB_NETHER) {
-        ne_column<MODE>(W, perm, oct, qx, qz, t);
+        ne_column<MODE>(W, perm, G, oct, qx, qz, t);
         int cand = -1;
         b = tree_search(tree, W->tree_n, t, cand);
     } else {
@@ -970,9 +999,9 @@ __global__ __launch_bounds__(256) void k_targets(const McbWorld *__restrict__ W,
     int b;
     uint32_t e = 0;
     const int t5[5] = {t[0], t[1], t[2], t[3], t[5]};
-    if (DIM == MCB_OVERWORLD && cell_in_range(t[0]) && cell_in_range(t[1]) && cell_in_range(t[2]) && cell_in_range(t[3]) &&
-        cell_in_range(t[4]) && cell_in_range(t[5]))
-        e = cells[cell_base(W, t5) + (uint32_t)cell_coord(W, 4, t[4])];
+    if (DIM == MCB_OVERWORLD && W->cell_on && cell_in_range(t[0]) && cell_in_range(t[1]) && cell_in_range(t[2]) &&
+        cell_in_range(t[3]) && cell_in_range(t[4]) && cell_in_range(t[5]))
+        e = cells[cell_base(W, W->cell_lut[0], t5) + cell_depth(W->cell_lut[0], t[4])];
     if (e & 255u) {
         b = cell_best(tree, clist, e, t5, t[4]);
     } else {
