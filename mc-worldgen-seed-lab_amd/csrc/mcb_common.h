@@ -14,6 +14,7 @@
 #define MCB_NE_OCT 8            /* Nether: 2 noises x 2 stacks x 2 levels */
 #define MCB_SPL_NODES 320
 #define MCB_SPL_PTS 320
+#define MCB_SPL_MAXD 6          /* device spline recursion levels: splines nest at most 5 deep */
 #define MCB_CELL_MAX_BP 24      /* breakpoints per climate dimension in the cell table */
 #define MCB_CELL_LUT (6 * 256)  /* cell locator entries */
 
@@ -42,6 +43,8 @@ typedef struct {
     float value[MCB_SPL_NODES];
     float loc[MCB_SPL_PTS], der[MCB_SPL_PTS];
     int32_t child[MCB_SPL_PTS];
+    uint32_t hdr[MCB_SPL_NODES];    /* packed for the device: multipoint << 31 | coord << 24 | count << 16 | first */
+    int32_t depth;                  /* multipoint levels on the deepest path */
 } McbSpline;
 
 /* R-tree node in DFS pre-order: the first child of node i is i+1, and `escape` is the index just past

@@ -227,25 +227,25 @@ constexpr uint32_t grad_bits(int c) {
 constexpr uint32_t kGX = grad_bits(0), kGY = grad_bits(1), kGZ = grad_bits(2);
 __device__ __forceinline__ int gcomp(uint32_t bits, int h) { return __builtin_amdgcn_sbfe((int)bits, 2 * h, 2); }
 
-// LDS gradient entry for a permutation value h (the vector of h & 15): gx as f32, gy and gz as f16
-// (exact for -1, 0, 1), so one 8-byte read feeds the dot product through mixed-precision FMAs.
-struct GradEnt {
-    float x;
-    _Float16 y, z;
-};
+// LDS gradient entry for a permutation value h (the vector of h & 15), 8 bytes: gx as f32 in the low
+// word, gy and gz as f16 (exact for -1, 0, 1) in the high word, so one ds_read_b64 feeds the dot product
+// through mixed-precision FMAs.
+typedef uint64_t GradEnt;
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void init_grad(GradEnt *G) {
     for (int h = threadIdx.x; h < 256; h += blockDim.x) {
         const int k = h & 15;
-        G[h].x = (float)gcomp(kGX, k);
-        G[h].y = (_Float16)gcomp(kGY, k);
-        G[h].z = (_Float16)gcomp(kGZ, k);
+        const half2v yz = {(_Float16)gcomp(kGY, k), (_Float16)gcomp(kGZ, k)};
+        G[h] = (uint64_t)__float_as_uint((float)gcomp(kGX, k)) | (uint64_t)__builtin_bit_cast(uint32_t, yz) << 32;
     }
 }
 
 // g . (x, y, z) as the reference's gx*x + gy*y + gz*z.  With g in {-1, 0, 1} every product is exact, so
 // a fused multiply-add rounds exactly like the separate product and sum (same value, same signed zero).
-__device__ __forceinline__ float gdot_f(const GradEnt &g, float x, float y, float z) {
-    return fmaf((float)g.z, z, fmaf((float)g.y, y, g.x * x));
+__device__ __forceinline__ float gdot_f(GradEnt g, float x, float y, float z) {
+    const float gx = __uint_as_float((uint32_t)g);
+    const half2v yz = __builtin_bit_cast(half2v, (uint32_t)(g >> 32));
+    return fmaf((float)yz.y, z, fmaf((float)yz.x, y, gx * x));
 }
 __device__ __forceinline__ double gdot_d(int h, double x, double y, double z) {
     h &= 15;
@@ -375,66 +375,59 @@ __device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm2, con
 // ================================================================================================
 // offset spline (engine/mc_climate.h:40-66), float
 // ================================================================================================
-// Iterative evaluation with a bounded frame stack (the offset splines nest 5 deep): phase 0 = enter
-// node, 1 = first child returned, 2 = second child returned.
-#define SPL_DEPTH 8
-__device__ float spline_eval(const McbSpline &S, int root, const float *in4) {
-    int nd[SPL_DEPTH], ph[SPL_DEPTH], iv[SPL_DEPTH];
-    float y1[SPL_DEPTH];
-    int sp = 0;
-    nd[0] = root;
-    ph[0] = 0;
-    float ret = 0.0f;
-    for (;;) {
-        const int id = nd[sp];
-        const int first = S.first[id], last = S.count[id] - 1;
-        const float x = in4[S.coord[id]];
-        if (ph[sp] == 0) {
-            if (S.kind[id] == 0) {
-                ret = S.value[id];
-                if (sp == 0) return ret;
-                sp--;
-                continue;
-            }
-            int lo = 0, hi = last + 1;                 // first i with x < loc[i], minus one
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (x < S.loc[first + mid]) hi = mid; else lo = mid + 1;
-            }
-            const int i = lo - 1;
-            iv[sp] = i;
-            ph[sp] = 1;
-            if (sp + 1 >= SPL_DEPTH) return __builtin_nanf("");
-            nd[sp + 1] = S.child[first + (i < 0 ? 0 : i)];
-            ph[sp + 1] = 0;
-            sp++;
-            continue;
+// The spline lives in LDS (packed headers, locations, derivatives, children).  Evaluation recurses
+// through MCB_SPL_MAXD compile-time levels; each level has one call site (a loop over the one or two
+// children the node needs), so the code grows linearly with the depth.
+struct SplLds {
+    uint32_t hdr[MCB_SPL_NODES];
+    float val[MCB_SPL_NODES];
+    float loc[MCB_SPL_PTS];
+    float der[MCB_SPL_PTS];
+    uint16_t child[MCB_SPL_PTS];
+};
+__device__ __forceinline__ void stage_spline(SplLds *L, const McbSpline &S) {
+    for (int i = threadIdx.x; i < MCB_SPL_NODES; i += blockDim.x) {
+        L->hdr[i] = S.hdr[i];
+        L->val[i] = S.value[i];
+    }
+    for (int i = threadIdx.x; i < MCB_SPL_PTS; i += blockDim.x) {
+        L->loc[i] = S.loc[i];
+        L->der[i] = S.der[i];
+        L->child[i] = (uint16_t)S.child[i];
+    }
+}
+template <int D>
+__device__ __forceinline__ float spline_eval(const SplLds &S, int id, float c, float e, float w, float r) {
+    if constexpr (D == 0) {
+        return __builtin_nanf("");                     // unreachable: the host bounds the nesting depth
+    } else {
+        const uint32_t h = S.hdr[id];
+        if (!(h >> 31)) return S.val[id];
+        const int first = (int)(h & 0xFFFFu), cnt = (int)((h >> 16) & 0xFFu), coord = (int)((h >> 24) & 3u);
+        const float x = coord == 0 ? c : (coord == 1 ? e : (coord == 2 ? w : r));
+        int lo = 0, hi = cnt;                          // first i with x < loc[i], minus one
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (x < S.loc[first + mid]) hi = mid; else lo = mid + 1;
         }
-        const int i = iv[sp];
-        if (ph[sp] == 1) {
-            if (i < 0 || i == last) {                 // linear extension beyond the end points
-                const int k = i < 0 ? 0 : last;
-                const float d = S.der[first + k];
-                ret = d == 0.0f ? ret : ret + d * (x - S.loc[first + k]);
-                if (sp == 0) return ret;
-                sp--;
-                continue;
-            }
-            y1[sp] = ret;
-            ph[sp] = 2;
-            nd[sp + 1] = S.child[first + i + 1];
-            ph[sp + 1] = 0;
-            sp++;
-            continue;
+        const int i = lo - 1, last = cnt - 1;
+        const bool edge = i < 0 || i == last;          // linear extension beyond the end points
+        const int base = first + (i < 0 ? 0 : i);
+        float ya = 0.0f, yb = 0.0f;
+#pragma unroll 1
+        for (int q = 0; q < (edge ? 1 : 2); q++) {
+            const float v = spline_eval<D - 1>(S, S.child[base + q], c, e, w, r);
+            if (q == 0) ya = v; else yb = v;
         }
-        const float ya = y1[sp], yb = ret;
-        const float x1 = S.loc[first + i], x2 = S.loc[first + i + 1];
+        if (edge) {
+            const float d = S.der[base];
+            return d == 0.0f ? ya : ya + d * (x - S.loc[base]);
+        }
+        const float x1 = S.loc[base], x2 = S.loc[base + 1];
         const float t = (x - x1) / (x2 - x1);
-        const float a = S.der[first + i] * (x2 - x1) - (yb - ya);
-        const float b = -S.der[first + i + 1] * (x2 - x1) + (yb - ya);
-        ret = lerp_f(t, ya, yb) + t * (1.0f - t) * lerp_f(t, a, b);
-        if (sp == 0) return ret;
-        sp--;
+        const float a = S.der[base] * (x2 - x1) - (yb - ya);
+        const float b = -S.der[base + 1] * (x2 - x1) + (yb - ya);
+        return lerp_f(t, ya, yb) + t * (1.0f - t) * lerp_f(t, a, b);
     }
 }
 
@@ -587,7 +580,7 @@ struct Column {
 
 template <int MODE>
 __device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const uint8_t *perm, const GradEnt *G,
-                                          const McbOct *oct, int qx, int qz, Column &c) {
+                                          const SplLds *SP, const McbOct *oct, int qx, int qz, Column &c) {
     const McbNoise *N = W->noise;
     const double X = (double)qx, Z = (double)qz;     // blockX*0.25 with blockX = 4*quartX
     if (MODE) {
@@ -601,8 +594,7 @@ __device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const 
         float Wr = normal2<1, false>(perm, G, oct, N[5], W->oct_base[5], x, z);
         float a = fabsf(Wr) + W->rf_f[0];
         float rf = (fabsf(a) + W->rf_f[1]) * W->rf_f[2];
-        float in4[4] = {C, E, Wr, rf};
-        c.off_f = W->offc_f + spline_eval(W->spline, 0, in4);
+        c.off_f = W->offc_f + spline_eval<MCB_SPL_MAXD>(*SP, 0, C, E, Wr, rf);
         c.t = quantize(T); c.h = quantize(H); c.c = quantize(C); c.e = quantize(E); c.w = quantize(Wr);
     } else {
         double sx = normal2<0, false>(perm, G, oct, N[0], W->oct_base[0], X, Z) * 4.0;
@@ -614,8 +606,7 @@ __device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const 
         double E = normal2<0, false>(perm, G, oct, N[4], W->oct_base[4], x, z);
         double Wr = normal2<0, false>(perm, G, oct, N[5], W->oct_base[5], x, z);
         double rf = (fabs(fabs(Wr) + W->rf_d[0]) + W->rf_d[1]) * W->rf_d[2];
-        float in4[4] = {(float)C, (float)E, (float)Wr, (float)rf};
-        float sp = spline_eval(W->spline, 0, in4);
+        float sp = spline_eval<MCB_SPL_MAXD>(*SP, 0, (float)C, (float)E, (float)Wr, (float)rf);
         c.off_d = 0.0 + (W->offc_d + (double)sp);
         c.t = quantize((float)T); c.h = quantize((float)H); c.c = quantize((float)C); c.e = quantize((float)E);
         c.w = quantize((float)Wr);
@@ -752,12 +743,17 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                                               const McbTile *__restrict__ tiles, int ntiles, int tile_w, int cpl,
                                               uint8_t *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
-    __shared__ GradEnt G[256];
+    __shared__ __attribute__((aligned(16))) GradEnt G[256];
     __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
+    __shared__ SplLds spl[1];
+    SplLds *SP = spl;
     const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT;
     const int tree_n = W->tree_n;
-    init_grad(G);                                         // both visible after the first staging barrier
-    if (DIM == MCB_OVERWORLD) stage_lut(clut, W);
+    init_grad(G);                                         // all visible after the first staging barrier
+    if (DIM == MCB_OVERWORLD) {
+        stage_lut(clut, W);
+        stage_spline(SP, W->spline);
+    }
     const int tile_h = 256 / tile_w;
     int staged = -1;
     // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables
@@ -789,7 +785,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
             uint8_t *o = out + R.out_off + (int64_t)iz * R.nx + ix;
             if (DIM == MCB_OVERWORLD) {
                 Column col;
-                ow_column<MODE>(W, perm, G, oct, qx, qz, col);
+                ow_column<MODE>(W, perm, G, SP, oct, qx, qz, col);
                 const int t5[5] = {col.t, col.h, col.c, col.e, col.w};
                 int dep[MAXL], b[MAXL];
                 bool in = W->cell_on && cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) &&
@@ -939,15 +935,20 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
                                                 const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
                                                 int32_t *__restrict__ tg) {
     __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
-    __shared__ GradEnt G[256];
+    __shared__ __attribute__((aligned(16))) GradEnt G[256];
     const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
     const int s = blockIdx.y;
     const int slot = slot_of[s];
     __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
+    __shared__ SplLds spl[1];
+    SplLds *SP = spl;
     if (DIM == MCB_END) stage_perm(perm, slots + slot, 1);
     else stage_perm2(perm, slots + slot, n_oct);
     init_grad(G);
-    if (DIM == MCB_OVERWORLD) stage_lut(clut, W);
+    if (DIM == MCB_OVERWORLD) {
+        stage_lut(clut, W);
+        stage_spline(SP, W->spline);
+    }
     __syncthreads();
     const int p = blockIdx.x * 256 + threadIdx.x;
     if (p >= np) return;
@@ -958,7 +959,7 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
     int b;
     if (DIM == MCB_OVERWORLD) {
         Column c;
-        ow_column<MODE>(W, perm, G, oct, qx, qz, c);
+        ow_column<MODE>(W, perm, G, SP, oct, qx, qz, c);
         t[0] = c.t; t[1] = c.h; t[2] = c.c; t[3] = c.e; t[5] = c.w;
         t[4] = ow_depth<MODE>(W, c, qy);
         const int t5[5] = {c.t, c.h, c.c, c.e, c.w};

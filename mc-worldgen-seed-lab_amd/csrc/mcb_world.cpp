@@ -11,6 +11,7 @@
 // doubles and floats as the reference CPU engine.
 #include "mcb_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -241,6 +242,23 @@ int spline_parse(McbSpline &S, const std::vector<std::string> &t, size_t &pos) {
     return id;
 }
 
+// Packed node headers and nesting depth for the device evaluator.
+int spline_pack(McbSpline &S, int id) {
+    if (S.kind[id] == 0) {
+        S.hdr[id] = 0;
+        return 0;
+    }
+    if (S.count[id] < 1 || S.count[id] > 255 || S.first[id] > 0xFFFF || S.coord[id] < 0 || S.coord[id] > 3) return -1;
+    S.hdr[id] = 1u << 31 | (uint32_t)S.coord[id] << 24 | (uint32_t)S.count[id] << 16 | (uint32_t)S.first[id];
+    int d = 0;
+    for (int i = 0; i < S.count[id]; i++) {
+        const int c = spline_pack(S, S.child[S.first[id] + i]);
+        if (c < 0) return -1;
+        d = std::max(d, c);
+    }
+    return d + 1;
+}
+
 // LCG jump: state' = A^n*state + C_n (mod 2^48)
 void lcg_jump(uint64_t n, uint64_t *mul, uint64_t *add) {
     const uint64_t M = (1ULL << 48) - 1;
@@ -380,6 +398,8 @@ int build_world(const std::string &root, int version, int dim, const std::string
             size_t pos = 2;
             w->spline.n_nodes = w->spline.n_pts = 0;
             if (spline_parse(w->spline, t, pos) != 0) { err = "bad offset spline"; return -1; }
+            w->spline.depth = spline_pack(w->spline, 0);
+            if (w->spline.depth < 0 || w->spline.depth >= MCB_SPL_MAXD) { err = "offset spline too deep"; return -1; }
             have_spline = true;
         } else if (t[0] == "ridges_folded" && t.size() >= 4) {
             for (int i = 0; i < 3; i++) {
