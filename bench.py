@@ -10,7 +10,9 @@ measures the plain per-call path (one synchronous request per seed), the serving
 stream, and the reference CPU engine on this host (cpu_baseline).  Prints ONE JSON line.
 
 Other workloads (--workload): plain (one seed, 256x256 quarts at qy 16), mixed (10,000 uneven requests of
-every version and dimension, 10% zoomed), serve (Poisson arrivals at 200/1000/5000 req/s).
+every version and dimension, 10% zoomed), serve (Poisson arrivals at 200/1000/5000 req/s), search (seed
+search over a candidate range against 8 reference observations), endfield (End island height field of
+256 seeds over 2048x2048 blocks).
 """
 import argparse
 import json
@@ -42,6 +44,10 @@ FLOPS = {
 }
 PER_SAMPLE = {"26.3": {"f32": 3}, "26.1": {"f64": 3}}
 PEAK = {"f32": 157.3, "f64": 78.6}     # MI355X published vector peaks, TFLOP/s (FMA counted as 2)
+# HBM bytes per launch of the 26.3 grid kernel on this workload, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch) scaled by the same counters on a 256 MiB device copy (FETCH x2.0, WRITE
+# x1.0): profiles/k_grid_26.3_batched_4096x64x64x4.md.  (167,968 x 2.0 + 142,464) KiB.
+TRAFFIC_26_3 = (167968 * 2 + 142464) * 1024
 
 
 def splitmix64_stream(k, n):
@@ -203,32 +209,49 @@ def bench_batched(args, torch, mcbiome):
             tuned[v] = {"tile_w": tw & 0xFFFF, "cols_per_lane": tw >> 16, "grid_ms": round(ms, 3)}
     samples_per_step = sum(plans[v].samples for v in VERSIONS)
 
-    def step():
+    # A step runs each version's plan; without --graph the stages are launched one by one so that HIP
+    # events on the plan's stream bracket the dominant kernel (stage 1, the quart-grid launch) inside the
+    # timed region.  With --graph the step is one graph replay and the kernel is timed after the loop.
+    ev = {v: [] for v in VERSIONS}
+
+    def step(record):
         for v in VERSIONS:
-            plans[v].run(outs[v].data_ptr(), sp, graph=args.graph)
+            if args.graph:
+                plans[v].run(outs[v].data_ptr(), sp, graph=True)
+                continue
+            plans[v].run_stage(0, outs[v].data_ptr(), sp)
+            if record:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+            plans[v].run_stage(1, outs[v].data_ptr(), sp)
+            if record:
+                e1.record(stream)
+                ev[v].append((e0, e1))
+            plans[v].run_stage(2, outs[v].data_ptr(), sp)
+            plans[v].run_stage(3, outs[v].data_ptr(), sp)
 
     for _ in range(args.warmup):
-        step()
+        step(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        step(True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = samples_per_step * args.steps / dt
 
-    # dominant kernel: the quart-grid launch (stage 1) per version, timed with events on its stream
     kern = {}
     for v in VERSIONS:
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        plans[v].run_stage(0, outs[v].data_ptr(), sp)
-        reps = 3
-        e0.record(stream)
-        for _ in range(reps):
-            plans[v].run_stage(1, outs[v].data_ptr(), sp)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
+        if args.graph:        # graph mode: time stage 1 on its own after the timed loop
+            plans[v].run_stage(0, outs[v].data_ptr(), sp)
+            for _ in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                plans[v].run_stage(1, outs[v].data_ptr(), sp)
+                e1.record(stream)
+                ev[v].append((e0, e1))
+            torch.cuda.synchronize()
+        ms = sum(a.elapsed_time(b) for a, b in ev[v]) / len(ev[v])
         cols, samp = plans[v].columns, plans[v].samples
         f32 = FLOPS[v]["f32"] * cols + PER_SAMPLE[v].get("f32", 0) * samp
         f64 = FLOPS[v]["f64"] * cols + PER_SAMPLE[v].get("f64", 0) * samp
@@ -236,7 +259,7 @@ def bench_batched(args, torch, mcbiome):
                    "samples_per_s": samp / (ms / 1e3), "out_gbps": samp / (ms / 1e3) / 1e9}
     k = kern["26.3"]
     roofline = {"bound": "valu", "achieved": round(k["f32_tflops"], 3), "peak": PEAK["f32"], "unit": "TFLOP/s",
-                "frac": round(k["f32_tflops"] / PEAK["f32"], 4), "traffic": None,
+                "frac": round(k["f32_tflops"] / PEAK["f32"], 4), "traffic": TRAFFIC_26_3,
                 "kernel": "k_grid<26.3 float, overworld>", "kernel_ms": round(k["ms"], 3),
                 "f64_side_tflops": round(k["f64_tflops"], 3),
                 "issue_ceiling_note": "FMA-free unpacked FP32/FP64 issue = 1 op/lane/clk = 39.3 T op/s (25% of 157.3); "
@@ -248,12 +271,14 @@ def bench_batched(args, torch, mcbiome):
 
     # plain per-call path: one synchronous request per seed (allocate, launch, copy back, free)
     npc = min(args.percall_seeds, N_SEEDS)
-    e = engines["26.3"]
-    e.region(work[0][0], work[0][1], work[0][2], SIDE, SIDE, QYS)
-    t1 = time.perf_counter()
-    for s, qx0, qz0 in work[:npc]:
-        e.region(s, qx0, qz0, SIDE, SIDE, QYS)
-    per_call = npc * SIDE * SIDE * len(QYS) / (time.perf_counter() - t1)
+    per_call = None
+    if npc > 0:
+        e = engines["26.3"]
+        e.region(work[0][0], work[0][1], work[0][2], SIDE, SIDE, QYS)
+        t1 = time.perf_counter()
+        for s, qx0, qz0 in work[:npc]:
+            e.region(s, qx0, qz0, SIDE, SIDE, QYS)
+        per_call = npc * SIDE * SIDE * len(QYS) / (time.perf_counter() - t1)
 
     lat = run_serving(mcbiome, args.serve_rate, args.serve_requests) if args.serve_requests > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(work)
@@ -266,13 +291,93 @@ def bench_batched(args, torch, mcbiome):
         "config": {"workload": "batched Overworld quart grids: 4096 seeds x 64x64 x qy{-16,0,16,48}, 26.3 and 26.1",
                    "samples_per_step": samples_per_step, "columns_per_step": sum(plans[v].columns for v in VERSIONS),
                    "autotuned": tuned, "graph": bool(args.graph), "parallelism": "single GPU"},
-        "per_call_samples_per_s": round(per_call, 1),
-        "batched_over_per_call": round(value / per_call, 2),
+        "per_call_samples_per_s": round(per_call, 1) if per_call else None,
+        "batched_over_per_call": round(value / per_call, 2) if per_call else None,
         "per_version_kernel_samples_per_s": {v: round(kern[v]["samples_per_s"], 1) for v in VERSIONS},
         "tile_latency_ms": lat,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+
+
+def _kat_obs(mcbiome):
+    """The 8 Java-oracle observations of seed 8675309 (Overworld 26.3, quart points), a fixture copied from
+    crack/src/gpu_biomes/results/example_obs_26.3_overworld_seed8675309.txt."""
+    import oracle
+    names = oracle.biome_names()
+    obs = []
+    for line in open(os.path.join(REPO, "tests", "golden", "search", "example_obs_26.3_overworld_seed8675309.txt")):
+        if line.startswith("#") or not line.strip():
+            continue
+        x, y, z, b = line.split()
+        obs.append((int(x), int(y), int(z), b.replace("minecraft:", "")))
+    return [mcbiome.make_obs(x, y, z, [names.index(b)]) for x, y, z, b in obs], \
+        [(x, y, z, False, {names.index(b)}) for x, y, z, b in obs]
+
+
+def bench_search(args, torch, mcbiome):
+    """Seed search: candidates 0 .. n-1 against the 8 observations of seed 8675309 (the reference's
+    full int32 scan with them runs at 1.72e7 candidates/s on an RTX 4080 SUPER and finds only 8675309,
+    crack/src/gpu_biomes/results/research_full32_ow_26.3.txt:2-5).  A step = one search over all n."""
+    n = args.search_candidates
+    obs, obs_ref = _kat_obs(mcbiome)
+    eng = mcbiome.Engine("26.3", "overworld")
+    for _ in range(args.warmup):
+        eng.search(obs, base=0, n=min(n, 1 << 22))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kms = []
+    for _ in range(args.steps):
+        found, reached, ms = eng.search(obs, base=0, n=n)
+        kms.append(ms)
+    dt = (time.perf_counter() - t0) / args.steps
+    assert 8675309 in found.tolist() if n > 8675309 else True
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        orc = oracle.Oracle("26.3", "overworld")
+        sample = 1 << 18
+        t1 = time.perf_counter()
+        orc.search(np.arange(sample, dtype=np.int64), obs_ref)
+        ct = time.perf_counter() - t1
+        cpu = {"value": round(sample / ct, 1), "unit": "candidates/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": "candidates 0 .. 2^18-1, the oracle's C search (OpenMP over candidates, all host threads)"}
+    return {"metric": "seed-search candidates/s, 8 observations, Overworld 26.3", "value": round(n / dt, 1),
+            "unit": "candidates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f32+f64 (26.3) / u64 RNG", "data": "reference observations (Java oracle), candidate range",
+            "config": {"workload": "search: candidates 0..%d, 8 observations of seed 8675309" % (n - 1)},
+            "kernel_ms": round(sum(kms) / len(kms), 3), "found": [int(v) for v in found[:8]],
+            "reached": [int(v) for v in reached], "reference_published": {"value": 1.72e7, "hardware": "RTX 4080 SUPER"},
+            "cpu_baseline": cpu}
+
+
+def bench_endfield(args, torch, mcbiome):
+    """End island field (the terrain's End density term): the first 256 seeds of splitmix64(4), each over
+    2048x2048 blocks at step 8 (256x256 height values, section = block / 8), origin at block 0."""
+    seeds = np.array([signed64(v) for v in splitmix64_stream(4, 256)], np.int64)
+    eng = mcbiome.Engine("26.3", "the_end")
+    for _ in range(args.warmup):
+        eng.end_field(seeds[:16], -1024, -1024, 256, 256, 8, 0)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = eng.end_field(seeds, -1024, -1024, 256, 256, 8, 0)
+    dt = (time.perf_counter() - t0) / args.steps
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        orc = oracle.Oracle("26.3", "the_end")
+        t1 = time.perf_counter()
+        orc.end_field(int(seeds[0]), -1024, -1024, 256, 256, 8, 0)
+        ct = time.perf_counter() - t1
+        cpu = {"value": round(65536 / ct, 1), "unit": "samples/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": "one seed (65,536 height values), the oracle's C field (OpenMP over rows)"}
+    return {"metric": "End island height values/s (host round trip included)", "value": round(seeds.size * 65536 / dt, 1),
+            "unit": "samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f64 simplex / f32 islands", "data": "synthetic (splitmix64(4) seeds)",
+            "config": {"workload": "endfield: 256 seeds x 256x256 sections (2048x2048 blocks, step 8)"},
+            "checksum": float(np.float64(out.sum())), "cpu_baseline": cpu}
 
 
 def bench_plain(args, torch, mcbiome):
@@ -365,7 +470,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield"])
+    ap.add_argument("--search-candidates", type=int, default=1 << 28)
     ap.add_argument("--tile-w", type=int, default=0, help="fixed tile (width | cols_per_lane<<16); 0 = autotune")
     ap.add_argument("--graph", action="store_true", help="replay each plan as a hipGraph")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -391,7 +497,8 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve}[args.workload]
+    fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve,
+          "search": bench_search, "endfield": bench_endfield}[args.workload]
     print(json.dumps(fn(args, torch, mcbiome)))
 
 

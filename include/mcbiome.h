@@ -82,6 +82,49 @@ int mcb_engine_cells(const McbEngine *e, int64_t *cells, int64_t *list_len, doub
 #define MCB_TARGET_MAX (1 << 24)
 int mcb_targets(McbEngine *e, const int32_t *tg6, int n, uint8_t *out_biome);
 
+/* End island field (engine/mc_end.h:66-106) over an nx x nz block grid of every seed, blocks
+ * (bx0 + ix*step, bz0 + iz*step): kind 0 = getHeightValue at section (bx/8, bz/8) (C division),
+ * kind 1 = the End's erosion value at the block (what TheEndBiomeSource reads; the terrain's End
+ * island density term).  out_host[s*nx*nz + iz*nx + ix] as double (26.3 values are floats).  End
+ * engines only; host buffers, synchronous. */
+int mcb_end_field(McbEngine *e, const int64_t *seeds, int ns, int bx0, int bz0, int nx, int nz, int step, int kind,
+                  double *out_host);
+
+/* ================================================================================================
+ * Seed search (crack/src/gpu_biomes/gpu_biomes.h:55-70, gb_search): which candidate seeds show every
+ * observed biome.  An observation is a point and an accepted biome set; it passes when the point's
+ * biome is in the set (with MCB_SEARCH_TIES: when any least-distance leaf's biome is, the game's
+ * tie nondeterminism).  Layout = McgObs (engine/cuda/mc_gpu.cuh:47): quart coordinates, or block
+ * coordinates seen through BiomeManager when flags has MCB_OBS_BLOCK; flags' low 8 bits = the End
+ * cell-window radius (12 = the game's window; smaller only where the set's answer cannot depend on
+ * farther cells, see mcb_end_obs_radius).
+ * ================================================================================================ */
+typedef struct McbObs {
+    int32_t x, y, z, flags;
+    uint64_t mask[2];        /* bit b = biome id b accepted */
+} McbObs;
+#define MCB_OBS_BLOCK 0x100
+#define MCB_MAX_OBS 64
+/* Candidates: kind 0 = list[0..n), 1 = structural seeds base | i << 48 for i < 65536 (n ignored),
+ * 2 = range base + i for i < n (McgSeedGen, engine/cuda/mc_gpu.cuh:695-702). */
+typedef struct McbSeedSource {
+    int32_t kind, pad;
+    const int64_t *list;     /* host pointer */
+    uint64_t base, n;
+} McbSeedSource;
+#define MCB_SEARCH_TIES 1    /* accept any least-distance leaf's biome */
+#define MCB_SEARCH_NOPRUNE 2 /* Overworld: no early rejection on partial climate targets */
+#define MCB_SEARCH_SLOTS 4   /* evaluate from full per-seed state (any dimension; the Nether and End always do) */
+/* Observations are checked in the given order; a candidate stops at its first failure.  reached[j]
+ * (nobs+1 entries) = candidates that passed exactly j observations; found = the candidates that
+ * passed all, sorted as unsigned, at most max_found of them (*n_found; reached[nobs] counts all);
+ * kernel_ms = device time.  Same result as gb_search over the same candidates. */
+int mcb_search(McbEngine *e, const McbSeedSource *src, const McbObs *obs, int nobs, int flags, int64_t *found,
+               uint32_t max_found, uint32_t *n_found, uint64_t *reached, double *kernel_ms);
+/* End window radius for an accepted set (the smallest cell window that decides membership,
+ * engine/cuda/mc_gpu.cuh:652-666); 0 when the answer does not depend on the island field. */
+int mcb_end_obs_radius(const McbEngine *e, const uint64_t mask[2]);
+
 /* Seeds x points, host buffers, synchronous (mcg_ref_points contract). */
 int mcb_points(McbEngine *e, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out_biome,
                int32_t *out_tg);

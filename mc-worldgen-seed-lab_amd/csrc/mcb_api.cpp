@@ -52,7 +52,10 @@ int device_ready() {
     return 0;
 }
 
-int ensure_ident(McbEngine *e, int n) {
+}  // namespace
+
+// identity slot map (slot i = seed i) of at least n entries
+int mcb::ensure_ident(McbEngine *e, int n) {
     if ((size_t)n <= e->ident_cap) return 0;
     if (int rc = dev_grow(&e->d_ident, &e->ident_cap, (size_t)n)) return rc;
     std::vector<int32_t> id(n);
@@ -60,6 +63,8 @@ int ensure_ident(McbEngine *e, int n) {
     MCB_HIPCHK(hipMemcpy(e->d_ident, id.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
     return 0;
 }
+
+namespace {
 
 // Cell tables depend only on the tree (not on the seed or preset): one build per distinct tree per
 // process, shared by every engine and context world that loads it.
@@ -109,7 +114,7 @@ int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {
                     off = bp[i] - t0;
                 }
             }
-            if (inside > 1) W.cell_on = 0;
+            if (inside > 1 || (ct->nb[k] && (bp[0] <= -32768 || bp[ct->nb[k] - 1] > 32767))) W.cell_on = 0;
             W.cell_lut[k][b] = (uint32_t)below * (uint32_t)W.cell_stride[k] | (uint32_t)off << 20;
         }
     }
@@ -122,6 +127,7 @@ int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {
         MCB_HIPCHK(hipMemcpy(e->w.d_cell_list, ct->list.data(), ct->list.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
     e->cell_entries = ct->entry.size();
     e->cell_list_len = ct->list.size();
+    e->cells = ct;
     return MCB_OK;
 }
 
@@ -192,6 +198,7 @@ int mcb_open(McbEngine **out, int version, int dim, const char *preset, const ch
         mcb_close(e);
         return fail(MCB_E_DEVICE, "world upload failed");
     }
+    e->tree_host = tree;
     if (!tree.empty()) {
         if (hipMalloc((void **)&e->w.d_tree, tree.size() * sizeof(McbNode)) != hipSuccess ||
             hipMemcpy(e->w.d_tree, tree.data(), tree.size() * sizeof(McbNode), hipMemcpyHostToDevice) != hipSuccess) {
@@ -215,6 +222,15 @@ void mcb_close(McbEngine *e) {
     (void)hipFree(e->d_pts);
     (void)hipFree(e->d_out);
     (void)hipFree(e->d_tg);
+    (void)hipFree(e->d_field);
+    (void)hipFree(e->d_feas);
+    (void)hipFree(e->d_coord_lut);
+    (void)hipFree(e->d_obs);
+    (void)hipFree(e->d_hist);
+    (void)hipFree(e->d_next);
+    (void)hipFree(e->d_nfound);
+    (void)hipFree(e->d_found);
+    (void)hipFree(e->d_list);
     delete e;
 }
 
@@ -247,6 +263,32 @@ int mcb_targets(McbEngine *e, const int32_t *tg6, int n, uint8_t *out_biome) {
     MCB_HIPCHK(hipMemcpy(e->d_tg, tg6, (size_t)n * 6 * sizeof(int32_t), hipMemcpyHostToDevice));
     MCB_HIPCHK(launch_targets(e->w, e->d_tg, n, e->d_out, nullptr));
     MCB_HIPCHK(hipMemcpy(out_biome, e->d_out, (size_t)n, hipMemcpyDeviceToHost));
+    return MCB_OK;
+}
+
+int mcb_end_field(McbEngine *e, const int64_t *seeds, int ns, int bx0, int bz0, int nx, int nz, int step, int kind,
+                  double *out) {
+    if (!e || ns < 0 || nx < 0 || nz < 0 || step < 1 || (kind != 0 && kind != 1) || (ns && nx && nz && (!seeds || !out)))
+        return fail(MCB_E_ARG, "bad arguments");
+    if (e->w.dim != MCB_END) return fail(MCB_E_ARG, "the End field needs an End engine");
+    if (ns == 0 || nx == 0 || nz == 0) return MCB_OK;
+    const int64_t xl = (int64_t)bx0 + (int64_t)(nx - 1) * step, zl = (int64_t)bz0 + (int64_t)(nz - 1) * step;
+    if (xl > INT32_MAX || zl > INT32_MAX) return fail(MCB_E_LIMIT, "grid beyond the int32 block range");
+    if ((int64_t)nx * nz > (1LL << 30)) return fail(MCB_E_LIMIT, "grid too large");
+    std::lock_guard<std::mutex> lk(e->mu);
+    const int chunk = (int)std::min<int64_t>(ns, std::max<int64_t>(1, std::min<int64_t>(65535, (256LL << 20) / ((int64_t)nx * nz))));
+    const size_t per = (size_t)nx * nz;
+    if (int rc = dev_grow(&e->d_slots, &e->slots_cap, (size_t)chunk)) return rc;
+    if (int rc = ensure_ident(e, chunk)) return rc;
+    if (int rc = dev_grow(&e->d_seeds, &e->seeds_cap, (size_t)chunk)) return rc;
+    if (int rc = dev_grow(&e->d_field, &e->field_cap, (size_t)chunk * per)) return rc;
+    for (int s0 = 0; s0 < ns; s0 += chunk) {
+        const int n = std::min(chunk, ns - s0);
+        MCB_HIPCHK(hipMemcpy(e->d_seeds, seeds + s0, n * sizeof(int64_t), hipMemcpyHostToDevice));
+        MCB_HIPCHK(launch_seed_init(e->w, e->d_seeds, e->d_ident, n, e->d_slots, nullptr));
+        MCB_HIPCHK(launch_end_field(e->w, e->d_slots, n, bx0, bz0, nx, nz, step, kind, e->d_field, nullptr));
+        MCB_HIPCHK(hipMemcpy(out + (size_t)s0 * per, e->d_field, (size_t)n * per * sizeof(double), hipMemcpyDeviceToHost));
+    }
     return MCB_OK;
 }
 

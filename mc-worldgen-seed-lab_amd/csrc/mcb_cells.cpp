@@ -142,18 +142,27 @@ struct Builder {
                 cur++;
             }
         }
-        // per depth cell: drop every leaf dominated by another leaf of the superset
+        // per depth cell: drop every leaf dominated by another leaf of the superset (first against the
+        // probe minima, then pairwise among what is left)
+        std::vector<int> C1;
         for (int dc = 0; dc < ncell[4]; dc++) {
             Iv Bd[6];
             for (int k = 0; k < 6; k++) Bd[k] = B[k];
             Bd[4] = cell_iv(4, dc);
-            std::vector<uint16_t> &out = lists[(size_t)c5 * ncell[4] + dc];
-            for (size_t a = 0; a < C.size(); a++) {
-                const McbNode &L = T[C[a]];
+            C1.clear();
+            for (int l : C) {
                 bool dom = false;
-                for (size_t b = 0; b < C.size() && !dom; b++)
-                    if (b != a) dom = dominates(T[C[b]], C[b], L, C[a], C[a] + 1, Bd);
-                if (!dom) out.push_back((uint16_t)C[a]);
+                for (size_t q = 0; q < D.size() && !dom; q++)
+                    if (D[q] != l) dom = dominates(T[D[q]], D[q], T[l], l, l + 1, Bd);
+                if (!dom) C1.push_back(l);
+            }
+            std::vector<uint16_t> &out = lists[(size_t)c5 * ncell[4] + dc];
+            for (size_t a = 0; a < C1.size(); a++) {
+                const McbNode &L = T[C1[a]];
+                bool dom = false;
+                for (size_t b = 0; b < C1.size() && !dom; b++)
+                    if (b != a) dom = dominates(T[C1[b]], C1[b], L, C1[a], C1[a] + 1, Bd);
+                if (!dom) out.push_back((uint16_t)C1[a]);
             }
         }
     }

@@ -3,6 +3,7 @@
 #define MCB_INTERNAL_H
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -23,8 +24,9 @@ int build_world(const std::string &root, int version, int dim, const std::string
 // Climate cell table (mcb_cells.cpp): per cell of the breakpoint grid, the ascending pre-order
 // indices of the leaves that can be the (distance, index) minimum of a target inside it.
 // Cell index = (((((cT*nH + cH)*nC + cC)*nE + cE)*nW + cW)*nD + cD), n = breakpoints + 1, cell i of a
-// dimension = [bp[i-1], bp[i]-1] with the outer cells bounded by +-kCellTMax.
-constexpr int kCellTMax = 32767;
+// dimension = [bp[i-1], bp[i]-1] with the outer cells bounded by +-kCellTMax (beyond every climate
+// noise's bound, so only a depth or a caller's target can fall outside and walk the tree).
+constexpr int kCellTMax = 1 << 17;
 constexpr int kCellMaxBp = MCB_CELL_MAX_BP;
 struct CellTable {
     int nb[6] = {0, 0, 0, 0, 0, 0};    // breakpoints per tree dimension (0 T, 1 H, 2 C, 3 E, 4 depth, 5 W)
@@ -82,6 +84,9 @@ hipError_t launch_points(const DevWorld &w, const McbSlot *d_slots, const int32_
                          const int32_t *d_pts, int np, uint8_t *d_out, int32_t *d_tg, hipStream_t stream);
 // biomes of n quantized climate targets (6 int32 each: t,h,c,e,d,w); Overworld and Nether only
 hipError_t launch_targets(const DevWorld &w, const int32_t *d_tg, int n, uint8_t *d_out, hipStream_t stream);
+// End island field of ns seeds (slots 0..ns-1) over an nx x nz block grid; kind 0 height, 1 erosion
+hipError_t launch_end_field(const DevWorld &w, const McbSlot *d_slots, int ns, int bx0, int bz0, int nx, int nz, int step,
+                            int kind, double *d_out, hipStream_t stream);
 hipError_t launch_zoom(const McbRequest *d_zreqs, const McbRequest *d_qreqs, const McbTile *d_tiles, int ntiles,
                        const uint8_t *d_quart, uint8_t *d_out, hipStream_t stream);
 
@@ -118,6 +123,29 @@ struct McbEngine {
     int32_t *d_tg = nullptr;
     size_t tg_cap = 0;
     size_t cell_entries = 0, cell_list_len = 0;
+    double *d_field = nullptr;
+    size_t field_cap = 0;
+    // host copies kept for the seed search's prefix tables
+    std::shared_ptr<const mcb::CellTable> cells;
+    std::vector<McbNode> tree_host;
+    // seed search (mcb_search.cpp), grow-only
+    uint4 *d_feas = nullptr;               // prefix biome sets
+    uint32_t *d_coord_lut = nullptr;
+    int feas_off[4] = {0, 0, 0, 0};
+    bool feas_ok = false;                  // pruning allowed (noise bounds inside the table)
+    McbObs *d_obs = nullptr;
+    unsigned long long *d_hist = nullptr;
+    uint64_t *d_next = nullptr;
+    uint32_t *d_nfound = nullptr;
+    int64_t *d_found = nullptr;
+    size_t found_cap = 0;
+    int64_t *d_list = nullptr;
+    size_t list_cap = 0;
 };
+
+namespace mcb {
+// identity slot map (slot i = seed i) of at least n entries in e->d_ident
+int ensure_ident(McbEngine *e, int n);
+}  // namespace mcb
 
 #endif

@@ -38,6 +38,33 @@ class McgHost(C.Structure):
 _lib = None
 
 
+class Obs(C.Structure):
+    """One observation (include/mcbiome.h McbObs): point, flags (OBS_BLOCK, End radius), biome set."""
+    _fields_ = [("x", C.c_int32), ("y", C.c_int32), ("z", C.c_int32), ("flags", C.c_int32),
+                ("mask", C.c_uint64 * 2)]
+
+
+class SeedSource(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32), ("list", C.POINTER(C.c_int64)), ("base", C.c_uint64),
+                ("n", C.c_uint64)]
+
+
+OBS_BLOCK = 0x100
+SEARCH_TIES, SEARCH_NOPRUNE, SEARCH_SLOTS = 1, 2, 4
+
+
+def make_obs(x, y, z, biomes, block=False, radius=12):
+    """Observation accepting any of `biomes` (ids) at quart (x, y, z), or block coordinates if block."""
+    o = Obs()
+    o.x, o.y, o.z = x, y, z
+    o.flags = (OBS_BLOCK if block else 0) | (radius & 0xFF)
+    m = [0, 0]
+    for b in biomes:
+        m[b >> 6] |= 1 << (b & 63)
+    o.mask[0], o.mask[1] = m
+    return o
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -51,6 +78,11 @@ def lib():
         L.mcb_engine_info.argtypes = [P, i32p, i32p, i32p, i32p]
         L.mcb_engine_cells.argtypes = [P, i64p, i64p, C.POINTER(C.c_double)]
         L.mcb_targets.argtypes = [P, i32p, C.c_int, u8p]
+        L.mcb_search.argtypes = [P, C.POINTER(SeedSource), C.POINTER(Obs), C.c_int, C.c_int, i64p, C.c_uint32,
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
+        L.mcb_end_obs_radius.argtypes = [P, C.POINTER(C.c_uint64)]
+        L.mcb_end_field.argtypes = [P, i64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_double)]
         L.mcb_points.argtypes = [P, i64p, C.c_int, i32p, C.c_int, u8p, i32p]
         L.mcb_points_dev.argtypes = [P, P, C.c_int, P, C.c_int, P, P, P]
         L.mcb_regions_bytes.restype = C.c_int64
@@ -216,6 +248,45 @@ class Engine:
         out = np.zeros(t.shape[0], np.uint8)
         _check(lib().mcb_targets(self.h, _p(t, C.c_int32), t.shape[0], _p(out, C.c_uint8)))
         return out
+
+    def search(self, obs, seeds=None, base=0, n=0, structural=False, flags=0, max_found=1 << 16):
+        """Seed search: candidates = `seeds` (list), or base|i<<48 (structural), or base+i for i < n.
+        Returns (found int64 array sorted as unsigned, reached uint64[len(obs)+1], device ms)."""
+        arr = (Obs * len(obs))(*obs)
+        src = SeedSource()
+        keep = None
+        if seeds is not None:
+            keep = np.ascontiguousarray(seeds, dtype=np.int64)
+            src.kind, src.list, src.n = 0, _p(keep, C.c_int64), keep.size
+        elif structural:
+            src.kind, src.base = 1, base & ((1 << 64) - 1)
+        else:
+            src.kind, src.base, src.n = 2, base & ((1 << 64) - 1), n
+        found = np.zeros(max(1, max_found), np.int64)
+        nf = C.c_uint32()
+        reached = np.zeros(len(obs) + 1, np.uint64)
+        ms = C.c_double()
+        _check(lib().mcb_search(self.h, C.byref(src), arr, len(obs), flags, _p(found, C.c_int64), max_found,
+                                C.byref(nf), reached.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(ms)))
+        return found[: nf.value].copy(), reached, ms.value
+
+    def end_obs_radius(self, biomes):
+        m = (C.c_uint64 * 2)()
+        for b in biomes:
+            m[b >> 6] |= 1 << (b & 63)
+        r = lib().mcb_end_obs_radius(self.h, m)
+        if r < 0:
+            raise EngineError(lib().mcb_last_error().decode())
+        return r
+
+    def end_field(self, seeds, bx0, bz0, nx, nz, step=1, kind=1):
+        """End island field: kind 0 height value at section (bx/8, bz/8), kind 1 erosion at the block;
+        returns (n_seeds, nz, nx) float64."""
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64).ravel()
+        out = np.zeros(seeds.size * nx * nz, np.float64)
+        _check(lib().mcb_end_field(self.h, _p(seeds, C.c_int64), seeds.size, bx0, bz0, nx, nz, step, kind,
+                                   _p(out, C.c_double)))
+        return out.reshape(seeds.size, nz, nx)
 
     def points(self, seeds, pts, want_tg=False):
         seeds = np.ascontiguousarray(seeds, dtype=np.int64)

@@ -1171,6 +1171,62 @@ double or_end_erosion(const OrWorld *w, int64_t seed, int bx, int bz) {
     return end_erosion(&o, w->fmt_new, bx, bz);
 }
 
+/* Seed search (crack/src/gpu_biomes/gpu_biomes.cu:242-266 semantics): per candidate the number of
+ * observations passed in order; observation i = point obs_xyz[3i..], block coordinates through
+ * BiomeManager when obs_block[i], accepted set masks[2i..2i+1]; ties: any least-distance leaf counts.
+ * reached[nobs+1]; found (at most max_found, in candidate order); returns the number found. */
+int64_t or_search(const OrWorld *w, const int64_t *seeds, int64_t n, const int32_t *obs_xyz, const int32_t *obs_block,
+                  const uint64_t *masks, int nobs, int ties, uint64_t *reached, int64_t *found, int64_t max_found) {
+    for (int j = 0; j <= nobs; j++) reached[j] = 0;
+    int64_t nf = 0;
+    #pragma omp parallel
+    {
+        SeedState *st = malloc(sizeof *st);
+        uint64_t *lr = calloc((size_t)nobs + 1, sizeof *lr);
+        #pragma omp for schedule(dynamic, 64)
+        for (int64_t c = 0; c < n; c++) {
+            seed_init(w, st, seeds[c]);
+            i64 hs = 0; int have = 0, j = 0;
+            for (; j < nobs; j++) {
+                i32 q[3] = { obs_xyz[3 * j], obs_xyz[3 * j + 1], obs_xyz[3 * j + 2] };
+                if (obs_block[j]) {
+                    if (!have) { hs = or_hashed_seed(seeds[c]); have = 1; }
+                    or_zoom(hs, obs_xyz[3 * j], obs_xyz[3 * j + 1], obs_xyz[3 * j + 2], q);
+                }
+                i32 tg[6];
+                int b = sample_biome(w, st, q[0], q[1], q[2], tg);
+                int ok = (masks[2 * j + (b >> 6)] >> (b & 63)) & 1;
+                if (!ok && ties && w->dim != OR_END) {
+                    uint64_t m[2];
+                    or_tie_mask(w, tg, m);
+                    ok = ((m[0] & masks[2 * j]) | (m[1] & masks[2 * j + 1])) != 0;
+                }
+                if (!ok) break;
+            }
+            lr[j]++;
+            if (j == nobs) {
+                #pragma omp critical
+                { if (nf < max_found) found[nf] = seeds[c]; nf++; }
+            }
+        }
+        #pragma omp critical
+        for (int j = 0; j <= nobs; j++) reached[j] += lr[j];
+        free(lr); free(st);
+    }
+    return nf;
+}
+
+void or_end_field(const OrWorld *w, int64_t seed, int bx0, int bz0, int nx, int nz, int step, int kind, double *out) {
+    Oct o; end_init(&o, seed);                                           /* mc_end.h:66-106 */
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int iz = 0; iz < nz; iz++)
+        for (int ix = 0; ix < nx; ix++) {
+            const i32 bx = bx0 + ix * step, bz = bz0 + iz * step;
+            out[(size_t)iz * nx + ix] = kind == 0 ? (double)end_height(&o, w->fmt_new, bx / 8, bz / 8)
+                                                  : end_erosion(&o, w->fmt_new, bx, bz);
+        }
+}
+
 static int ser_rec(const RTree *T, int ni, int64_t *buf, int cap, int pos) {
     const RNode *n = &T->node[ni];
     if (pos + 16 > cap) return -1;

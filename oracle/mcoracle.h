@@ -80,6 +80,11 @@ void or_zoom(int64_t hashed, int x, int y, int z, int32_t q[3]);
 int or_tree_serialize(const OrWorld *w, int64_t *buf, int cap);
 /* Brute-force: all leaves at minimum distance; returns the count, biome mask in mask2[2]. */
 int or_tie_mask(const OrWorld *w, const int32_t tg6[6], uint64_t mask2[2]);
+/* Seed search over n candidates (see mcoracle.c); returns the number found */
+int64_t or_search(const OrWorld *w, const int64_t *seeds, int64_t n, const int32_t *obs_xyz, const int32_t *obs_block,
+                  const uint64_t *masks, int nobs, int ties, uint64_t *reached, int64_t *found, int64_t max_found);
+/* End island field of one seed over a block grid: kind 0 height value at (bx/8, bz/8), 1 erosion */
+void or_end_field(const OrWorld *w, int64_t seed, int bx0, int bz0, int nx, int nz, int step, int kind, double *out);
 /* biome of quantized targets {t,h,c,e,d,w} by the tree search (engine/mc_biomes.h:31-62), n targets */
 void or_targets(const OrWorld *w, const int32_t *tg6, int n, uint8_t *out);
 /* Leaves visited per search (instrumentation): average over the given targets. */

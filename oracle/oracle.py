@@ -57,6 +57,11 @@ def lib():
         L.or_tree_serialize.argtypes = [P, i64p, C.c_int]
         L.or_tie_mask.argtypes = [P, i32p, C.POINTER(C.c_uint64)]
         L.or_targets.argtypes = [P, i32p, C.c_int, u8p]
+        L.or_search.restype = C.c_int64
+        L.or_search.argtypes = [P, i64p, C.c_int64, i32p, i32p, C.POINTER(C.c_uint64), C.c_int, C.c_int,
+                                C.POINTER(C.c_uint64), i64p, C.c_int64]
+        L.or_end_field.argtypes = [P, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(C.c_double)]
         L.or_tree_visits.restype = C.c_double
         L.or_tree_visits.argtypes = [P, i32p, C.c_int]
         L.or_splitmix64_next.restype = C.c_uint64
@@ -174,6 +179,28 @@ class Oracle:
         m = (C.c_uint64 * 2)()
         cnt = lib().or_tie_mask(self.h, _p(t, C.c_int32), m)
         return cnt, (m[0], m[1])
+
+    def search(self, seeds, obs, ties=False, max_found=1 << 16):
+        """obs: [(x, y, z, block, set_of_biome_ids)] -> (found sorted as unsigned, reached)."""
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64)
+        xyz = np.array([[o[0], o[1], o[2]] for o in obs], np.int32)
+        blk = np.array([1 if o[3] else 0 for o in obs], np.int32)
+        masks = np.zeros(2 * len(obs), np.uint64)
+        for i, o in enumerate(obs):
+            for b in o[4]:
+                masks[2 * i + (b >> 6)] |= np.uint64(1 << (b & 63))
+        reached = np.zeros(len(obs) + 1, np.uint64)
+        found = np.zeros(max(1, max_found), np.int64)
+        nf = lib().or_search(self.h, _p(seeds, C.c_int64), seeds.size, _p(xyz, C.c_int32), _p(blk, C.c_int32),
+                             masks.ctypes.data_as(C.POINTER(C.c_uint64)), len(obs), 1 if ties else 0,
+                             reached.ctypes.data_as(C.POINTER(C.c_uint64)), _p(found, C.c_int64), max_found)
+        f = sorted(found[: min(nf, max_found)].tolist(), key=lambda v: v & ((1 << 64) - 1))
+        return np.array(f, np.int64), reached
+
+    def end_field(self, seed, bx0, bz0, nx, nz, step=1, kind=1):
+        out = np.zeros(nx * nz, np.float64)
+        lib().or_end_field(self.h, seed, bx0, bz0, nx, nz, step, kind, _p(out, C.c_double))
+        return out.reshape(nz, nx)
 
     def targets(self, tg):
         """Biomes of quantized targets (n x {t,h,c,e,d,w}) by the restated tree search."""

@@ -1,6 +1,6 @@
 """The climate cell table (the per-sample replacement of the R-tree search) through mcb_targets: the
 device result equals the oracle's tree search on targets on, beside and between every breakpoint,
-inside every kind of cell, beyond the table's +-32767 range (tree-walk path) and at the API limit."""
+inside every kind of cell, in the outer cells past +-32767, beyond the table's +-2^17 range (tree-walk path) and at the API limit."""
 import numpy as np
 import pytest
 

@@ -176,3 +176,23 @@ def test_end_tiles_match_oracle(version):
             want = orc.grid(seed, x0, z0, nx, nz, 16, 0)
             got = host[r.out_off: r.out_off + nx * nz].reshape(nz, nx)
             assert np.array_equal(got, want), (tile, seed, int((got != want).sum()))
+
+
+@pytest.mark.parametrize("version", G.VERSIONS)
+def test_end_field_matches_oracle(version):
+    """mcb_end_field (End island height value and erosion over block grids, many seeds at once) = oracle,
+    bit for bit: near the centre island, negative coordinates, fine and coarse steps (a step of 1024
+    blocks exceeds the tile's listed window and takes the direct path), far coordinates."""
+    eng = mcbiome.Engine(version, "the_end")
+    orc = O.Oracle(version, "the_end")
+    seeds = np.array(EDGE_SEEDS[:5], np.int64)
+    for bx0, bz0, nx, nz, step in [(-700, -650, 40, 37, 8), (5000, -9000, 33, 20, 1), (-2**20, 2**20 - 4096, 18, 17, 37),
+                                   (-20000, -20000, 20, 20, 1024), (123456, 654321, 16, 16, 8)]:
+        for kind in (0, 1):
+            print("end_field", version, bx0, bz0, nx, nz, step, kind, flush=True)
+            got = eng.end_field(seeds, bx0, bz0, nx, nz, step, kind)
+            for i, s in enumerate(seeds):
+                want = orc.end_field(int(s), bx0, bz0, nx, nz, step, kind)
+                assert np.array_equal(got[i].view(np.uint64), want.view(np.uint64)), (bx0, step, kind, int(s))
+    with pytest.raises(mcbiome.EngineError):
+        mcbiome.Engine(version, "overworld").end_field(seeds, 0, 0, 4, 4)

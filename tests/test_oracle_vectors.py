@@ -127,3 +127,27 @@ def test_grid_hashes_cpu_subset(version):
         w = O.Oracle(version, dim)
         ids = w.grid(seed, qx0, qz0, n, n, qy, semantics=1)
         assert O.fnv1a_names(ids) == h, (version, dim, seed)
+
+
+@pytest.mark.parametrize("version", G.VERSIONS)
+def test_end_field_vectors(version):
+    """or_end_field (the End island field the GPU kernel is checked against): its erosion at every End
+    row's weird block gives the row's float bits, and its height value reproduces the row through the
+    erosion formula ((h - 8) / 128; in 26.3 wherever the island term beats the centre term)."""
+    w = O.Oracle(version, "the_end")
+    bad = 0
+    for seed, qx, qy, qz, hv, fbits, name in G.climate_rows(version, "the_end"):
+        cx, cz = (qx * 4) >> 4, (qz * 4) >> 4
+        if cx * cx + cz * cz <= 4096:
+            continue
+        bx, bz = (cx * 2 + 1) * 8, (cz * 2 + 1) * 8
+        e = w.end_field(seed, bx, bz, 1, 1, 1, 1)[0, 0]
+        h = w.end_field(seed, bx, bz, 1, 1, 1, 0)[0, 0]
+        bad += G.f32_bits(np.float32(e)) != (fbits & 0xFFFFFFFF)
+        if version == "26.3":
+            isl = np.float32((np.float32(h) - np.float32(8.0)) / np.float32(128.0))
+            bad += isl > np.float32(e)                     # the maximum is never below the island term
+            bad += isl == np.float32(e) and G.f32_bits(isl) != (fbits & 0xFFFFFFFF)
+        else:
+            bad += (h - 8.0) / 128.0 != hv
+    assert bad == 0
