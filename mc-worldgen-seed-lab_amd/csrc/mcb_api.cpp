@@ -119,12 +119,23 @@ int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {
         }
     }
     e->w.cells_build_ms = ct->build_ms;
-    if (hipMalloc((void **)&e->w.d_cells, ct->entry.size() * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void **)&e->w.d_cell_list, std::max<size_t>(ct->list.size(), 1) * sizeof(uint16_t)) != hipSuccess)
+    // device form: the listed leaves copied in list order (one load per candidate, no index hop); an
+    // entry whose leaves all carry one biome answers without any distance: (biome << 8) | 1
+    std::vector<McbNode> recs(std::max<size_t>(ct->list.size(), 1));
+    for (size_t i = 0; i < ct->list.size(); i++) recs[i] = tree[ct->list[i]];
+    std::vector<uint32_t> dent(ct->entry.size());
+    for (size_t c = 0; c < ct->entry.size(); c++) {
+        const uint32_t en = ct->entry[c], cnt = en & 255u, first = en >> 8;
+        bool uniform = cnt > 0;
+        for (uint32_t q = 1; q < cnt && uniform; q++)
+            uniform = (recs[first + q].leaf & 0xFF) == (recs[first].leaf & 0xFF);
+        dent[c] = uniform ? (uint32_t)(recs[first].leaf & 0xFF) << 8 | 1u : en;
+    }
+    if (hipMalloc((void **)&e->w.d_cells, dent.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&e->w.d_cell_recs, recs.size() * sizeof(McbNode)) != hipSuccess)
         return fail(MCB_E_NOMEM, "cell table allocation");
-    MCB_HIPCHK(hipMemcpy(e->w.d_cells, ct->entry.data(), ct->entry.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    if (!ct->list.empty())
-        MCB_HIPCHK(hipMemcpy(e->w.d_cell_list, ct->list.data(), ct->list.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+    MCB_HIPCHK(hipMemcpy(e->w.d_cells, dent.data(), dent.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    MCB_HIPCHK(hipMemcpy(e->w.d_cell_recs, recs.data(), recs.size() * sizeof(McbNode), hipMemcpyHostToDevice));
     e->cell_entries = ct->entry.size();
     e->cell_list_len = ct->list.size();
     e->cells = ct;
@@ -215,7 +226,7 @@ void mcb_close(McbEngine *e) {
     (void)hipFree(e->w.d_world);
     (void)hipFree(e->w.d_tree);
     (void)hipFree(e->w.d_cells);
-    (void)hipFree(e->w.d_cell_list);
+    (void)hipFree(e->w.d_cell_recs);
     (void)hipFree(e->d_slots);
     (void)hipFree(e->d_ident);
     (void)hipFree(e->d_seeds);

@@ -12,6 +12,10 @@
 #define MCB_MAX_LEV 12          /* levels per NormalNoise stack (continentalness has 9) */
 #define MCB_OW_OCT 46           /* octaves of one Overworld seed: 23 levels x 2 stacks (every preset) */
 #define MCB_NE_OCT 8            /* Nether: 2 noises x 2 stacks x 2 levels */
+/* Levels per noise (offset, temperature, vegetation, continentalness, erosion, ridges): the same in every
+ * supported version and preset (climate-26.x.txt); the kernels unroll them, the loader checks them. */
+#define MCB_OW_LEVELS {3, 2, 2, 9, 4, 3}
+#define MCB_NE_LEVELS {2, 2}
 #define MCB_SPL_NODES 320
 #define MCB_SPL_PTS 320
 #define MCB_SPL_MAXD 6          /* device spline recursion levels: splines nest at most 5 deep */

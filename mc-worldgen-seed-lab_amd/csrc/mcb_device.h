@@ -265,34 +265,40 @@ __device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const GradEnt *
     }
 }
 
-// NormalNoise with y fixed at the octave offsets.  26.3 interleaves first/second layers and
-// accumulates in float (NoiseStack.get, mc_noise.h:303-312; legacy-nether order :313-321); 26.1/26.2
-// sums each stack in double, then (v1+v2)*valueFactor (mc_noise.h:286-301).
-template <int MODE, bool ZC>
+// NormalNoise with y fixed at the octave offsets, NL levels whose octaves start at slot BASE (the
+// layout the loader checks), fully unrolled so every table and octave record sits at a constant
+// offset.  26.3 interleaves first/second layers and accumulates in float (NoiseStack.get,
+// mc_noise.h:303-312; legacy-nether order :313-321); 26.1/26.2 sums each stack in double, then
+// (v1+v2)*valueFactor (mc_noise.h:286-301).
+template <int MODE, bool ZC, int NL, int BASE, bool LEGACY>
 __device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm2, const GradEnt *G, const McbOct *oct,
-                                                       const McbNoise &N, int base, double X, double Y) {
-    const int n = N.nlev;
+                                                       const McbNoise &N, double X, double Y) {
     auto term = [&](int k, int st) {
-        const int o = base + st * n + k;
+        const int o = BASE + st * NL + k;
         return octave_term<MODE, ZC>(P2{perm2 + o * OCT_BYTES}, G, oct[o], N, k, st, X, Y);
     };
-    if (MODE) {
+    if constexpr (MODE == 1) {
         float v = 0.0f;
-        if (!N.legacy) {
-            for (int k = 0; k < n; k++) {
+        if constexpr (!LEGACY) {
+#pragma unroll 1
+            for (int k = 0; k < NL; k++) {
                 v += term(k, 0);
                 v += term(k, 1);
             }
         } else {
-            for (int k = 0; k < n; k++) v += term(k, 0);
-            for (int k = 0; k < n; k++) v += term(k, 1);
+#pragma unroll 1
+            for (int k = 0; k < NL; k++) v += term(k, 0);
+#pragma unroll 1
+            for (int k = 0; k < NL; k++) v += term(k, 1);
         }
-        return (typename R<MODE>::T)v;
+        return v;
     } else {
         double v1 = 0.0, v2 = 0.0;
-        for (int k = 0; k < n; k++) v1 += term(k, 0);
-        for (int k = 0; k < n; k++) v2 += term(k, 1);
-        return (typename R<MODE>::T)((v1 + v2) * N.value_factor);
+#pragma unroll 1
+        for (int k = 0; k < NL; k++) v1 += term(k, 0);
+#pragma unroll 1
+        for (int k = 0; k < NL; k++) v2 += term(k, 1);
+        return (v1 + v2) * N.value_factor;
     }
 }
 
@@ -469,28 +475,72 @@ __device__ __forceinline__ uint32_t cell_depth(const uint32_t *lut, int dep) { r
 __device__ __forceinline__ void stage_lut(uint32_t *lds, const McbWorld *__restrict__ W) {
     for (int i = threadIdx.x; i < MCB_CELL_LUT; i += blockDim.x) lds[i] = W->cell_lut[0][i];
 }
-// biome of the least (distance, index) leaf of a cell entry (count > 0)
-__device__ __forceinline__ int cell_best(const McbNode *__restrict__ N, const uint16_t *__restrict__ list, uint32_t e,
-                                         const int *t5, int dep) {
+// Device cell entry: count in bits 0-7 (0 = walk the tree); count 1 = one biome answers the whole cell
+// (one leaf, or leaves of one biome): biome in bits 8-15; else the first of `count` leaf records
+// (copies of the listed leaves, ascending pre-order index) in bits 8-31.
+__device__ __forceinline__ double leaf_dist5(const McbNode &nd, const int *t5, int dep) {
+    double d = (double)nd.d6;
+    d = dim_sq(t5[0], nd.lo[0], nd.hi[0], d);
+    d = dim_sq(t5[1], nd.lo[1], nd.hi[1], d);
+    d = dim_sq(t5[2], nd.lo[2], nd.hi[2], d);
+    d = dim_sq(t5[3], nd.lo[3], nd.hi[3], d);
+    d = dim_sq(t5[4], nd.lo[5], nd.hi[5], d);
+    return dim_sq(dep, nd.lo[4], nd.hi[4], d);
+}
+// biome of the least (distance, index) leaf of an entry (count > 0); strict < keeps the first minimum
+__device__ __forceinline__ int cell_best(const McbNode *__restrict__ recs, uint32_t e, const int *t5, int dep) {
     const int cnt = (int)(e & 255u);
-    const uint16_t *l = list + (e >> 8);
+    if (cnt == 1) return (int)((e >> 8) & 255u);
+    const McbNode *r = recs + (e >> 8);
     double bd = __builtin_inf();
     int biome = 0;
     for (int q = 0; q < cnt; q++) {
-        const McbNode nd = N[l[q]];
-        double d = (double)nd.d6;
-        d = dim_sq(t5[0], nd.lo[0], nd.hi[0], d);
-        d = dim_sq(t5[1], nd.lo[1], nd.hi[1], d);
-        d = dim_sq(t5[2], nd.lo[2], nd.hi[2], d);
-        d = dim_sq(t5[3], nd.lo[3], nd.hi[3], d);
-        d = dim_sq(t5[4], nd.lo[5], nd.hi[5], d);
-        d = dim_sq(dep, nd.lo[4], nd.hi[4], d);
+        const McbNode nd = r[q];
+        const double d = leaf_dist5(nd, t5, dep);
         if (d < bd) {
             bd = d;
             biome = nd.leaf & 0xFF;
         }
     }
     return biome;
+}
+// All layers of a column at once: the entries are loaded together, then candidate q of every layer in
+// the same step, so the loads of different layers overlap.  False when some layer must walk the tree.
+template <int MAXL>
+__device__ __forceinline__ bool cell_layers(const McbWorld *__restrict__ W, const uint32_t *clut,
+                                            const uint32_t *__restrict__ cells, const McbNode *__restrict__ recs,
+                                            const int *t5, const int *dep, int ny, int *b) {
+    const uint32_t c5 = cell_base(W, clut, t5);
+    uint32_t e[MAXL];
+    int cnt[MAXL], mx = 0;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < MAXL; j++) e[j] = j < ny ? cells[c5 + cell_depth(clut, dep[j])] : 1u;
+#pragma unroll
+    for (int j = 0; j < MAXL; j++) {
+        cnt[j] = (int)(e[j] & 255u);
+        ok = ok && cnt[j] != 0;
+        if (cnt[j] == 1) b[j] = (int)((e[j] >> 8) & 255u);
+        mx = max(mx, cnt[j] == 1 ? 0 : cnt[j]);
+    }
+    if (!ok) return false;
+    double bd[MAXL];
+#pragma unroll
+    for (int j = 0; j < MAXL; j++) bd[j] = __builtin_inf();
+    for (int q = 0; q < mx; q++) {
+#pragma unroll
+        for (int j = 0; j < MAXL; j++) {
+            if (q < cnt[j] && cnt[j] > 1) {
+                const McbNode nd = recs[(e[j] >> 8) + q];
+                const double d = leaf_dist5(nd, t5, dep[j]);
+                if (d < bd[j]) {
+                    bd[j] = d;
+                    b[j] = nd.leaf & 0xFF;
+                }
+            }
+        }
+    }
+    return true;
 }
 
 __device__ __forceinline__ int quantize(float v) { return (int)(v * 10000.0f); }   // mc_climate.h:93
@@ -545,15 +595,16 @@ __device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const 
     typedef typename R<MODE>::T T_;
     const McbNoise *N = W->noise;
     const double X = (double)qx, Z = (double)qz;     // blockX*0.25 with blockX = 4*quartX
-    const T_ a = normal2<MODE, false>(perm, G, oct, N[0], W->oct_base[0], X, Z);
-    const T_ b = normal2<MODE, true>(perm, G, oct, N[0], W->oct_base[0], Z, X);
+    // octave slots of the layout MCB_OW_LEVELS {3, 2, 2, 9, 4, 3}: bases 0, 6, 10, 14, 32, 40
+    const T_ a = normal2<MODE, false, 3, 0, false>(perm, G, oct, N[0], X, Z);
+    const T_ b = normal2<MODE, true, 3, 0, false>(perm, G, oct, N[0], Z, X);
     double x, z;
     shift_coords<MODE>(a, b, X, Z, x, z);
-    const T_ T = normal2<MODE, false>(perm, G, oct, N[1], W->oct_base[1], x, z);
-    const T_ H = normal2<MODE, false>(perm, G, oct, N[2], W->oct_base[2], x, z);
-    const T_ C = normal2<MODE, false>(perm, G, oct, N[3], W->oct_base[3], x, z);
-    const T_ E = normal2<MODE, false>(perm, G, oct, N[4], W->oct_base[4], x, z);
-    const T_ Wr = normal2<MODE, false>(perm, G, oct, N[5], W->oct_base[5], x, z);
+    const T_ T = normal2<MODE, false, 2, 6, false>(perm, G, oct, N[1], x, z);
+    const T_ H = normal2<MODE, false, 2, 10, false>(perm, G, oct, N[2], x, z);
+    const T_ C = normal2<MODE, false, 9, 14, false>(perm, G, oct, N[3], x, z);
+    const T_ E = normal2<MODE, false, 4, 32, false>(perm, G, oct, N[4], x, z);
+    const T_ Wr = normal2<MODE, false, 3, 40, false>(perm, G, oct, N[5], x, z);
     column_finish<MODE>(W, SP, T, H, C, E, Wr, c);
 }
 
@@ -577,8 +628,9 @@ template <int MODE>
 __device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, const uint8_t *perm, const GradEnt *G,
                                           const McbOct *oct, int qx, int qz, int *t) {
     const double X = (double)qx, Z = (double)qz;
-    float T = (float)normal2<MODE, false>(perm, G, oct, W->noise[0], W->oct_base[0], X, Z);
-    float H = (float)normal2<MODE, false>(perm, G, oct, W->noise[1], W->oct_base[1], X, Z);
+    // layout MCB_NE_LEVELS {2, 2}, legacy order: bases 0 and 4
+    float T = (float)normal2<MODE, false, 2, 0, true>(perm, G, oct, W->noise[0], X, Z);
+    float H = (float)normal2<MODE, false, 2, 4, true>(perm, G, oct, W->noise[1], X, Z);
     t[0] = quantize(T); t[1] = quantize(H);
     t[2] = t[3] = t[4] = t[5] = 0;
 }

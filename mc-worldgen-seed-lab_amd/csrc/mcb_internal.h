@@ -46,8 +46,9 @@ struct DevWorld {
     McbWorld host;
     McbWorld *d_world = nullptr;
     McbNode *d_tree = nullptr;
-    uint32_t *d_cells = nullptr;      // Overworld cell table entries and lists (null elsewhere)
-    uint16_t *d_cell_list = nullptr;
+    uint32_t *d_cells = nullptr;      // Overworld cell table (null elsewhere): entries, see cell_best
+    McbNode *d_cell_recs = nullptr;   // the listed leaves, copied in list order
+
     double cells_build_ms = 0.0;
     int version = 0, dim = 0;
     std::string preset;

@@ -119,7 +119,7 @@ __global__ __launch_bounds__(64) void k_seed_end(const McbWorld *__restrict__ W,
 // columns (lx, lz*cpl + c) in order, each column's search warm-started from the previous one.
 template <int MODE, int DIM, int MAXL>
 __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
-                                              const uint32_t *__restrict__ cells, const uint16_t *__restrict__ clist,
+                                              const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
                                               const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
                                               const McbTile *__restrict__ tiles, int ntiles, int tile_w, int cpl,
                                               uint8_t *__restrict__ out) {
@@ -176,17 +176,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                     dep[j] = j < ny ? ow_depth<MODE>(W, col, R.y[j]) : 0;
                     in = in && cell_in_range(dep[j]);
                 }
-                if (in) {
-                    const uint32_t c5 = cell_base(W, clut, t5);
-#pragma unroll
-                    for (int j = 0; j < MAXL; j++) {
-                        if (j < ny) {
-                            const uint32_t e = cells[c5 + cell_depth(clut, dep[j])];
-                            if (e & 255u) b[j] = cell_best(tree, clist, e, t5, dep[j]);
-                            else in = false;
-                        }
-                    }
-                }
+                if (in) in = cell_layers<MAXL>(W, clut, cells, crecs, t5, dep, ny, b);
                 if (!in) tree_search_layers<MAXL>(tree, tree_n, t5, dep, ny, cand, b);
 #pragma unroll
                 for (int j = 0; j < MAXL; j++)
@@ -328,7 +318,7 @@ __global__ __launch_bounds__(256) void k_end_field(const McbSlot *__restrict__ s
 // ================================================================================================
 template <int MODE, int DIM>
 __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
-                                                const uint32_t *__restrict__ cells, const uint16_t *__restrict__ clist,
+                                                const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
                                                 const McbSlot *__restrict__ slots, const int32_t *__restrict__ slot_of,
                                                 const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
                                                 int32_t *__restrict__ tg) {
@@ -366,7 +356,7 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
             cell_in_range(t5[4]) && cell_in_range(t[4]))
             e = cells[cell_base(W, clut, t5) + cell_depth(clut, t[4])];
         if (e & 255u) {
-            b = cell_best(tree, clist, e, t5, t[4]);
+            b = cell_best(crecs, e, t5, t[4]);
         } else {
             int cand = -1;
             b = tree_search(tree, W->tree_n, t, cand);
@@ -388,7 +378,7 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
 // ================================================================================================
 template <int DIM>
 __global__ __launch_bounds__(256) void k_targets(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
-                                                 const uint32_t *__restrict__ cells, const uint16_t *__restrict__ clist,
+                                                 const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
                                                  const int32_t *__restrict__ tg, int n, uint8_t *__restrict__ out) {
     const int i = (int)blockIdx.x * 256 + (int)threadIdx.x;
     if (i >= n) return;
@@ -402,7 +392,7 @@ __global__ __launch_bounds__(256) void k_targets(const McbWorld *__restrict__ W,
         cell_in_range(t[3]) && cell_in_range(t[4]) && cell_in_range(t[5]))
         e = cells[cell_base(W, W->cell_lut[0], t5) + cell_depth(W->cell_lut[0], t[4])];
     if (e & 255u) {
-        b = cell_best(tree, clist, e, t5, t[4]);
+        b = cell_best(crecs, e, t5, t[4]);
     } else {
         int cand = -1;
         b = tree_search(tree, W->tree_n, t, cand);
@@ -465,7 +455,7 @@ hipError_t grid_launch3(const DevWorld &w, const McbSlot *slots, const McbReques
                            cpl, out);
     else
         hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
-                           w.d_cell_list, slots, reqs, tiles, ntiles, tile_w, cpl, out);
+                           w.d_cell_recs, slots, reqs, tiles, ntiles, tile_w, cpl, out);
     return hipGetLastError();
 }
 template <int MODE, int DIM>
@@ -484,7 +474,7 @@ template <int MODE, int DIM>
 hipError_t points_launch(const DevWorld &w, const McbSlot *slots, const int32_t *slot_of, int ns, const int32_t *pts, int np,
                          uint8_t *out, int32_t *tg, hipStream_t st) {
     hipLaunchKernelGGL((k_points<MODE, DIM>), dim3((np + 255) / 256, ns), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
-                       w.d_cell_list, slots, slot_of, pts, np, out, tg);
+                       w.d_cell_recs, slots, slot_of, pts, np, out, tg);
     return hipGetLastError();
 }
 
@@ -539,10 +529,10 @@ hipError_t launch_targets(const DevWorld &w, const int32_t *tg, int n, uint8_t *
     if (n <= 0) return hipSuccess;
     if (w.dim == MCB_OVERWORLD)
         hipLaunchKernelGGL(k_targets<MCB_OVERWORLD>, dim3((n + 255) / 256), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
-                           w.d_cell_list, tg, n, out);
+                           w.d_cell_recs, tg, n, out);
     else if (w.dim == MCB_NETHER)
         hipLaunchKernelGGL(k_targets<MCB_NETHER>, dim3((n + 255) / 256), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
-                           w.d_cell_list, tg, n, out);
+                           w.d_cell_recs, tg, n, out);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

@@ -148,7 +148,7 @@ int mcb_search(McbEngine *e, const McbSeedSource *src, const McbObs *obs, int no
     A.W = e->w.d_world;
     A.tree = e->w.d_tree;
     A.cells = e->w.d_cells;
-    A.clist = e->w.d_cell_list;
+    A.crecs = e->w.d_cell_recs;
     A.feas = e->d_feas;
     for (int i = 0; i < 4; i++) A.feas_off[i] = e->feas_off[i];
     if (e->cells)

@@ -9,8 +9,8 @@ namespace mcb {
 struct SearchArgs {
     const McbWorld *W;
     const McbNode *tree;
-    const uint32_t *cells;          // Overworld cell table (entries, lists)
-    const uint16_t *clist;
+    const uint32_t *cells;          // Overworld cell table (entries, listed leaves)
+    const McbNode *crecs;
     const uint4 *feas;              // biome sets of prefix cells (temperature, +vegetation, +continents, +erosion)
     int feas_off[4];
     int ncell[6];                   // cells per tree dimension

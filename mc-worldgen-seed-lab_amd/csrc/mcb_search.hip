@@ -47,7 +47,7 @@ __device__ __forceinline__ int ow_biome(const SearchArgs &A, const uint32_t *clu
     if (A.W->cell_on && cell_in_range(t[0]) && cell_in_range(t[1]) && cell_in_range(t[2]) && cell_in_range(t[3]) &&
         cell_in_range(t[4]) && cell_in_range(t[5]))
         e = A.cells[cell_base(A.W, clut, t5) + cell_depth(clut, t[4])];
-    if (e & 255u) return cell_best(A.tree, A.clist, e, t5, t[4]);
+    if (e & 255u) return cell_best(A.crecs, e, t5, t[4]);
     int cand = -1;
     return tree_search(A.tree, A.W->tree_n, t, cand);
 }

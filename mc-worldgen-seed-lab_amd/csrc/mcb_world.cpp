@@ -428,6 +428,9 @@ int build_world(const std::string &root, int version, int dim, const std::string
             base += 2 * w->noise[i].nlev;
         }
         w->n_oct = base;
+        const int lv[6] = MCB_OW_LEVELS;
+        for (int i = 0; i < 6; i++)
+            if (w->noise[i].nlev != lv[i] || w->noise[i].legacy) { err = "unsupported Overworld noise layout"; return -1; }
         if (base != MCB_OW_OCT) { err = "unexpected octave count"; return -1; }
     } else if (dim == MCB_NETHER) {
         const Params *t = find("nether/temperature"), *v = find("nether/vegetation");
@@ -438,6 +441,11 @@ int build_world(const std::string &root, int version, int dim, const std::string
         w->oct_base[0] = 0;
         w->oct_base[1] = 2 * w->noise[0].nlev;
         w->n_oct = 2 * (w->noise[0].nlev + w->noise[1].nlev);
+        const int lv[2] = MCB_NE_LEVELS;
+        if (w->noise[0].nlev != lv[0] || w->noise[1].nlev != lv[1] || !w->noise[0].legacy || !w->noise[1].legacy) {
+            err = "unsupported Nether noise layout";
+            return -1;
+        }
         if (w->n_oct > MCB_NE_OCT) { err = "too many nether octaves"; return -1; }
     } else {
         w->n_noise = 0;
