@@ -109,7 +109,7 @@ def serve_requests(n, rate, seed_base=3):
     return [(float(t[i]), seeds[int(ranks[i])], int(tiles[i, 0]), int(tiles[i, 1])) for i in range(n)]
 
 
-def run_serving(mcbiome, rate, n, lanes=3, batch_mb=64):
+def run_serving(mcbiome, rate, n, lanes=4, batch_mb=8):
     ctx = mcbiome.Context()
     w = ctx.world("26.3", "overworld")
     srv = mcbiome.Server(ctx, batch_bytes=batch_mb << 20, lanes=lanes, slot_capacity=16384)

@@ -193,8 +193,9 @@ int mcb_context_world_engine(McbContext *c, int world, McbEngine **e);
  * into one of `lanes` in-flight batches while their output and zoom-staging bytes fit the lane's
  * device arena of `batch_bytes`; seed state is kept in a per-world device LRU of `slot_capacity`
  * slots (15,472 bytes each) so repeated seeds skip their initialisation; batches on different lanes
- * overlap on the device.  Results are copied into the caller's host buffer (ny*nx*nz bytes, layout of
- * McbRegion, out_off ignored).  Latency = submit to result in host memory.  No counterpart in the
+ * overlap on the device, and each lane has its own completion thread.  Results are copied into the
+ * caller's host buffer (ny*nx*nz bytes, layout of McbRegion, out_off ignored).  Latency = submit to
+ * result in the caller's buffer.  Defaults (argument <= 0): batch_bytes 8 MiB, lanes 4, 16384 slots.  No counterpart in the
  * reference (its closest is the terrain slot producer, libmcgen/src/gpu_bridge.c:605-760).
  * ================================================================================================ */
 typedef struct McbServer McbServer;

@@ -6,8 +6,9 @@
 // admission), looks their seeds up in a per-world device LRU of seed-state slots (only misses are
 // initialised), lowers the batch into kernel descriptors and launches it on the lane's stream
 // (seed state for misses, persistent quart-grid launch, zoom pass, copy-out).  Batches on different
-// lanes overlap on the device; a completion thread copies results to the callers' buffers and
-// records per-request latency (submit -> result in host memory).
+// lanes overlap on the device; each lane has its own completion thread, which copies the lane's
+// results to the callers' buffers and records per-request latency (submit -> result in the caller's
+// host buffer), so the host copies of different lanes run in parallel.
 //
 // The nearest precedent in the reference is the pinned-slot terrain producer of
 // libmcgen/src/gpu_bridge.c:605-760; the reference has no request server for biomes.
@@ -105,6 +106,8 @@ struct Lane {
     std::vector<Req> reqs;
     std::vector<std::pair<int, int>> pinned;              // (world, slot)
     bool busy = false;
+    bool launched = false;                                // batch on the device, results not yet copied
+    std::thread completer;
 };
 }  // namespace
 
@@ -118,12 +121,12 @@ struct McbServer {
     std::mutex mu;
     std::condition_variable cv_submit, cv_done, cv_lane;
     std::deque<Req> pending;
-    std::deque<int> inflight;                              // lane indices in launch order
+    int inflight = 0;                                      // lanes with a launched batch
     std::unordered_map<int64_t, std::pair<int, double>> results;   // ticket -> (status, latency ms)
     std::vector<double> latencies;
     int64_t next_ticket = 1;
     bool stop = false;
-    std::thread dispatcher, completer;
+    std::thread dispatcher;
     std::string fatal;
     int64_t batches = 0, admitted = 0;
 };
@@ -292,31 +295,36 @@ void dispatcher_main(McbServer *S) {
             S->cv_done.notify_all();
             continue;
         }
-        S->inflight.push_back(li);
+        L.launched = true;
+        S->inflight++;
         S->cv_lane.notify_all();
     }
 }
 
-void completer_main(McbServer *S) {
+// completion thread of lane li: waits for the lane's batch, copies each request's result out of the
+// lane's pinned buffer and stamps its latency once its bytes are in the caller's buffer
+void completer_main(McbServer *S, int li) {
+    Lane &L = S->lanes[li];
+    std::vector<double> ms;
     for (;;) {
-        int li;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_lane.wait(lk, [&] { return S->stop || !S->inflight.empty(); });
-            if (S->inflight.empty()) return;
-            li = S->inflight.front();
+            S->cv_lane.wait(lk, [&] { return S->stop || L.launched; });
+            if (!L.launched) return;
         }
-        Lane &L = S->lanes[li];
         hipError_t e = hipEventSynchronize(L.done);
-        const auto now = Clock::now();
-        if (e == hipSuccess)
-            for (auto &q : L.reqs) memcpy(q.host_out, L.h_out + q.r.out_off, (size_t)q.out_bytes);
+        ms.assign(L.reqs.size(), 0.0);
+        for (size_t i = 0; i < L.reqs.size(); i++) {
+            const Req &q = L.reqs[i];
+            if (e == hipSuccess) memcpy(q.host_out, L.h_out + q.r.out_off, (size_t)q.out_bytes);
+            ms[i] = std::chrono::duration<double, std::milli>(Clock::now() - q.t_submit).count();
+        }
         std::lock_guard<std::mutex> lk(S->mu);
-        S->inflight.pop_front();
-        for (auto &q : L.reqs) {
-            double ms = std::chrono::duration<double, std::milli>(now - q.t_submit).count();
-            S->results[q.ticket] = {e == hipSuccess ? 0 : (int)MCB_E_DEVICE, ms};
-            if (e == hipSuccess) S->latencies.push_back(ms);
+        L.launched = false;
+        S->inflight--;
+        for (size_t i = 0; i < L.reqs.size(); i++) {
+            S->results[L.reqs[i].ticket] = {e == hipSuccess ? 0 : (int)MCB_E_DEVICE, ms[i]};
+            if (e == hipSuccess) S->latencies.push_back(ms[i]);
         }
         for (auto &p : L.pinned) S->pools[p.first].release(p.second);
         L.busy = false;
@@ -332,8 +340,9 @@ extern "C" {
 int mcb_server_create(McbServer **out, McbContext *ctx, int64_t batch_bytes, int lanes, int slot_capacity) {
     if (!out || !ctx) return fail(MCB_E_ARG, "bad arguments");
     *out = nullptr;
-    if (batch_bytes <= 0) batch_bytes = 64ll << 20;
-    if (lanes <= 0) lanes = 3;
+    if (batch_bytes <= 0) batch_bytes = 8ll << 20;
+    if (lanes <= 0) lanes = 4;
+    if (lanes > 64) return fail(MCB_E_ARG, "at most 64 lanes");
     if (slot_capacity <= 0) slot_capacity = 16384;
     std::unique_ptr<McbServer> S(new McbServer());
     S->ctx = ctx;
@@ -354,7 +363,7 @@ int mcb_server_create(McbServer **out, McbContext *ctx, int64_t batch_bytes, int
     }
     McbServer *s = S.release();
     s->dispatcher = std::thread(dispatcher_main, s);
-    s->completer = std::thread(completer_main, s);
+    for (int i = 0; i < lanes; i++) s->lanes[i].completer = std::thread(completer_main, s, i);
     *out = s;
     return MCB_OK;
 }
@@ -394,7 +403,7 @@ int mcb_server_drain(McbServer *S) {
     if (!S) return fail(MCB_E_ARG, "null server");
     std::unique_lock<std::mutex> lk(S->mu);
     S->cv_done.wait(lk, [&] {
-        if (!S->pending.empty() || !S->inflight.empty()) return false;
+        if (!S->pending.empty() || S->inflight) return false;
         for (auto &L : S->lanes) if (L.busy) return false;
         return true;
     });
@@ -442,7 +451,7 @@ void mcb_server_destroy(McbServer *S) {
     S->cv_submit.notify_all();
     S->cv_lane.notify_all();
     S->dispatcher.join();
-    S->completer.join();
+    for (auto &L : S->lanes) L.completer.join();
     for (auto &L : S->lanes) {
         (void)hipStreamSynchronize(L.st);
         (void)hipStreamDestroy(L.st);

@@ -349,7 +349,7 @@ class Context:
 class Server:
     """Continuous-batching request server over a Context (mcb_server_*)."""
 
-    def __init__(self, context, batch_bytes=64 << 20, lanes=3, slot_capacity=16384):
+    def __init__(self, context, batch_bytes=8 << 20, lanes=4, slot_capacity=16384):
         self.context = context
         self.h = C.c_void_p()
         _check(lib().mcb_server_create(C.byref(self.h), context.h, batch_bytes, lanes, slot_capacity))
