@@ -1,0 +1,556 @@
+// mcg_plugin.cpp - host side of the libmcgen GPU plugin (include/mcgpu.h): device selection, errors,
+// compilation of a McgBiomeWorld into device form, and the grid / points entry points.
+//
+// Reference interface: libmcgen/gpu/mcgpu_core.cu:1-70 (device, errors), mcgpu_biome.cu:167-280 (world,
+// grid, points), bound by libmcgen/src/gpu_bridge.c:123-148.  Differences in how, not what:
+//  * the program is validated completely (operand order, index ranges, spline depth, tree shape) so a
+//    malformed description is an error instead of an out-of-range access on the device;
+//  * node values live in slots assigned by liveness (mcg_plugin.h), so there is no fixed cap on the
+//    number of nodes, only on the number of values alive at once (what fits in LDS);
+//  * a grid evaluates each quart cell (End: each chunk) it needs once and then applies the zoom,
+//    instead of evaluating a cell per output point; per point when the grid step makes that cheaper;
+//  * device buffers of a world persist between calls and grow on demand.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mcg_plugin.h"
+
+using namespace mcb::plug;
+
+namespace {
+
+thread_local char g_err[512];
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+std::mutex g_mu;
+int g_dev = -1;
+bool g_inited = false;
+
+bool ensure_init() {
+    if (g_inited) {
+        if (hipSetDevice(g_dev) != hipSuccess) { set_error("hipSetDevice(%d) failed", g_dev); return false; }
+        return true;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        set_error("no HIP devices: %s", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+        return false;
+    }
+    if (g_dev < 0) g_dev = 0;
+    if (g_dev >= n) { set_error("device %d does not exist (%d devices)", g_dev, n); return false; }
+    e = hipSetDevice(g_dev);
+    if (e != hipSuccess) { set_error("hipSetDevice(%d): %s", g_dev, hipGetErrorString(e)); return false; }
+    e = hipFree(nullptr);
+    if (e != hipSuccess) { set_error("device %d initialisation: %s", g_dev, hipGetErrorString(e)); return false; }
+    g_inited = true;
+    return true;
+}
+
+#define HCHK(call)                                                                                  \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            set_error("%s: %s", #call, hipGetErrorString(e_));                                     \
+            return false;                                                                           \
+        }                                                                                           \
+    } while (0)
+
+// ---------------------------------------------------------------------------------------------------
+// program compilation
+// ---------------------------------------------------------------------------------------------------
+enum { OP_A = 1, OP_B = 2, OP_C = 4, OP_NS = 8, OP_OCT = 16, OP_SPL = 32, OP_ISEL = 64 };
+
+// operands each kind reads (mcgpu_dev.cuh:307-373 and 390-435)
+int uses_new(int k) {
+    switch (k) {
+    case MCGK_CONST: case MCGK_DIST: case MCGK_GRAD_CLAMP: case MCGK_GRAD_REPEAT: case MCGK_GRAD_MIRROR:
+    case MCGK_CTX_ALPHA: case MCGK_CTX_OFFSET: case MCGK_CTX_BEARD: return 0;
+    case MCGK_NOISE: case MCGK_SHIFT_B: return OP_NS;
+    case MCGK_NOISE_XZ: return OP_NS | OP_A | OP_C;
+    case MCGK_NOISE_XYZ: return OP_NS | OP_A | OP_B | OP_C;
+    case MCGK_END: return OP_OCT;
+    case MCGK_ADD: case MCGK_SUB: case MCGK_MUL: case MCGK_DIV: case MCGK_MIN: case MCGK_MAX: return OP_A | OP_B;
+    case MCGK_LERP: case MCGK_RANGE: case MCGK_ISEL1: return OP_A | OP_B | OP_C;
+    case MCGK_LERP_CF: return OP_A | OP_C;
+    case MCGK_LERP_CS: return OP_A | OP_B;
+    case MCGK_SPLINE: return OP_SPL;
+    case MCGK_ISEL: return OP_A | OP_ISEL;
+    default: return OP_A;         // unary kinds, CACHE, BLEND_DENSITY, SLICE_*
+    }
+}
+int uses_old(int k) {
+    switch (k) {
+    case MCGO_CONST: case MCGO_Y_GRADIENT: case MCGO_BLEND_ALPHA: case MCGO_ZERO: return 0;
+    case MCGO_NOISE: case MCGO_SHIFT_A: case MCGO_SHIFT_B: case MCGO_SHIFT: return OP_NS;
+    case MCGO_SHIFTED_NOISE: return OP_NS | OP_A | OP_B | OP_C;
+    case MCGO_WEIRD_SCALED: return OP_NS | OP_A;
+    case MCGO_END_ISLANDS: return OP_OCT;
+    case MCGO_ADD: case MCGO_MUL: case MCGO_MIN: case MCGO_MAX: return OP_A | OP_B;
+    case MCGO_RANGE: return OP_A | OP_B | OP_C;
+    case MCGO_SPLINE: return OP_SPL;
+    case MCGO_ISEL: return OP_A | OP_ISEL;
+    default: return OP_A;         // unary kinds, CLAMP, CADD, CMUL, PASS
+    }
+}
+
+struct Compiled {
+    std::vector<PIns> ins;
+    std::vector<int32_t> arr;        // slot numbers
+    int nslot = 0, spl_depth = 1;
+    int root_slot[6] = {-1, -1, -1, -1, -1, -1};
+};
+
+bool spline_depth(const McgProg *p, int spi, int narr_node, int level, int *depth) {
+    if (spi < 0 || spi >= p->nsp) { set_error("spline index %d out of range", spi); return false; }
+    if (level > kSplineDepth) { set_error("spline nesting deeper than %d", kSplineDepth); return false; }
+    *depth = std::max(*depth, level);
+    const McgSpline &s = p->sp[spi];
+    if (s.is_const) return true;
+    if (s.n < 1 || s.loc0 < 0 || (int64_t)s.loc0 + 2 * (int64_t)s.n > p->nsplf || s.val0 < 0 ||
+        (int64_t)s.val0 + s.n > p->nspc || s.coord < 0 || s.coord >= narr_node) {
+        set_error("spline %d: points or coordinate out of range", spi);
+        return false;
+    }
+    for (int i = 0; i < s.n; i++)
+        if (!spline_depth(p, p->spc[s.val0 + i], narr_node, level + 1, depth)) return false;
+    return true;
+}
+
+bool check_noise(const McgProg *p, int ns) {
+    if (ns < 0 || ns >= p->nnoise) { set_error("noise index %d out of range", ns); return false; }
+    const McgNoise &mn = p->noise[ns];
+    auto range_ok = [&](int l0, int n, bool allow_empty) {
+        if (n < 0 || l0 < 0 || (int64_t)l0 + n > p->nlayers) return false;
+        for (int i = 0; i < n; i++) {
+            const int o = p->layers[l0 + i].oct;
+            if (o == -1 && allow_empty) continue;
+            if (o < 0 || o >= p->noct) return false;
+        }
+        return true;
+    };
+    const bool ok = p->old ? (range_ok(mn.l0, mn.n0, true) && range_ok(mn.l1, mn.n1, true)) : range_ok(mn.l0, mn.n0, false);
+    if (!ok) set_error("noise %d: layers or octaves out of range", ns);
+    return ok;
+}
+
+bool compile_prog(const McgProg *p, Compiled &C) {
+    const int n = p->nnodes;
+    if (n < 0 || (n && !p->nodes) || p->nnoise < 0 || p->nlayers < 0 || p->noct < 0 || p->nsp < 0 || p->nsplf < 0 ||
+        p->nspc < 0 || p->narr < 0 || p->nthr < 0 || (p->nnoise && !p->noise) || (p->nlayers && !p->layers) ||
+        (p->noct && !p->octs) || (p->nsp && !p->sp) || (p->nsplf && !p->splf) || (p->nspc && !p->spc) ||
+        (p->narr && !p->arr) || (p->nthr && !p->thr)) {
+        set_error("malformed program (counts or arrays)");
+        return false;
+    }
+    std::vector<int> last(n, -1);
+    auto use = [&](int user, int j) -> bool {
+        if (j < 0 || j >= user) { set_error("node %d reads node %d (must be an earlier node)", user, j); return false; }
+        last[j] = std::max(last[j], user);
+        return true;
+    };
+    for (int i = 0; i < n; i++) {
+        const McgNode &s = p->nodes[i];
+        if (p->old) {
+            if (s.k < 0 || s.k >= MCGO__COUNT) { set_error("unknown node kind %d", s.k); return false; }
+        } else {
+            switch (s.k) {
+            case MCGK_INTERP: case MCGK_FTS: case MCGK_LOG: case MCGK_POW_CB: case MCGK_POW_CE: case MCGK_POW:
+            case MCGK_ROUND_INT: case MCGK_ROUND:
+                set_error("node kind %d is not supported by the point evaluator", s.k);
+                return false;
+            default:
+                if (s.k < 0 || s.k >= MCGK__COUNT) { set_error("unknown node kind %d", s.k); return false; }
+            }
+            if ((s.k == MCGK_GRAD_REPEAT || s.k == MCGK_GRAD_MIRROR) && s.i2 == 0) {
+                set_error("node %d: gradient period 0", i);
+                return false;
+            }
+        }
+        const int u = p->old ? uses_old(s.k) : uses_new(s.k);
+        if ((u & OP_A) && !use(i, s.a)) return false;
+        if ((u & OP_B) && !use(i, s.b)) return false;
+        if ((u & OP_C) && !use(i, s.c)) return false;
+        if ((u & OP_NS) && !check_noise(p, s.ns)) return false;
+        if ((u & OP_OCT) && (s.oct < 0 || s.oct >= p->noct)) { set_error("node %d: octave out of range", i); return false; }
+        if (u & (OP_SPL | OP_ISEL)) {
+            if (s.narr < 1 || s.arr0 < 0 || (int64_t)s.arr0 + s.narr > p->narr) { set_error("node %d: list out of range", i); return false; }
+            for (int q = 0; q < s.narr; q++)
+                if (!use(i, p->arr[s.arr0 + q])) return false;
+        }
+        if ((u & OP_ISEL) && (s.nthr < 0 || s.thr0 < 0 || (int64_t)s.thr0 + s.nthr > p->nthr)) {
+            set_error("node %d: thresholds out of range", i);
+            return false;
+        }
+        if ((u & OP_ISEL) && s.nthr >= s.narr) { set_error("node %d: more thresholds than choices", i); return false; }
+        if ((u & OP_SPL) && !spline_depth(p, s.sp, s.narr, 1, &C.spl_depth)) return false;
+    }
+    for (int k = 0; k < 6; k++) {
+        const int r = k < p->nroots ? p->root[k] : -1;
+        if (r < -1 || r >= n) { set_error("root %d out of range", k); return false; }
+        if (r >= 0) last[r] = n;                              // live to the end
+    }
+    // linear-scan slot assignment: a value's slot is free again after its last reader
+    std::vector<int> slot(n, -1), free_slots;
+    std::vector<std::vector<int>> dies(n + 1);
+    for (int j = 0; j < n; j++)
+        if (last[j] >= 0 && last[j] < n) dies[last[j]].push_back(j);
+    int nslot = 0;
+    for (int i = 0; i < n; i++) {
+        for (int j : dies[i]) free_slots.push_back(slot[j]);  // read by node i: the kernel reads before it writes
+        std::sort(free_slots.begin(), free_slots.end(), std::greater<int>());
+        free_slots.erase(std::unique(free_slots.begin(), free_slots.end()), free_slots.end());
+        int d;
+        if (!free_slots.empty()) { d = free_slots.back(); free_slots.pop_back(); }
+        else d = nslot++;
+        slot[i] = d;
+        if (last[i] < 0) free_slots.push_back(d);             // never read: overwritten by a later node
+    }
+    C.nslot = std::max(nslot, 1);
+    C.arr.assign(std::max(p->narr, 1), 0);
+    C.ins.resize(n);
+    for (int i = 0; i < n; i++) {
+        const McgNode &s = p->nodes[i];
+        const int u = p->old ? uses_old(s.k) : uses_new(s.k);
+        PIns &q = C.ins[i];
+        memset(&q, 0, sizeof q);
+        q.op = s.k;
+        q.dst = slot[i];
+        q.a = (u & OP_A) ? slot[s.a] : 0;
+        q.b = (u & OP_B) ? slot[s.b] : 0;
+        q.c = (u & OP_C) ? slot[s.c] : 0;
+        q.ns = s.ns; q.sp = s.sp; q.arr0 = s.arr0; q.narr = s.narr; q.thr0 = s.thr0; q.nthr = s.nthr;
+        q.i0 = s.i0; q.i1 = s.i1; q.i2 = s.i2; q.i3 = s.i3;
+        q.cm = p->old ? 0 : s.cm; q.cx = s.cx; q.cy = s.cy; q.cz = s.cz; q.oct = s.oct;
+        q.f0 = s.f0; q.f1 = s.f1; q.f2 = s.f2; q.f3 = s.f3;
+        q.d0 = s.d0; q.d1 = s.d1;
+        if (u & (OP_SPL | OP_ISEL))
+            for (int k = 0; k < s.narr; k++) C.arr[s.arr0 + k] = slot[p->arr[s.arr0 + k]];
+    }
+    for (int k = 0; k < 6; k++) {
+        const int r = k < p->nroots ? p->root[k] : -1;
+        C.root_slot[k] = r >= 0 ? slot[r] : -1;
+    }
+    return true;
+}
+
+// BFS tree (children consecutive) -> pre-order with escape links, children in their stored order
+bool convert_tree(const McgTreeNode *T, int n, int root, std::vector<PNode> &out) {
+    if (n <= 0 || !T || root < 0 || root >= n) { set_error("malformed R-tree"); return false; }
+    std::vector<char> seen(n, 0);
+    std::vector<std::pair<int, int>> st;          // (node, next child) ; out index kept in order stack
+    std::vector<int> outidx;
+    out.clear();
+    auto enter = [&](int v) -> bool {
+        if (seen[v]) { set_error("R-tree node %d reached twice", v); return false; }
+        seen[v] = 1;
+        const McgTreeNode &m = T[v];
+        if (m.count < 0 || (m.count > 0 && (m.first < 0 || (int64_t)m.first + m.count > n))) {
+            set_error("R-tree node %d: children out of range", v);
+            return false;
+        }
+        PNode p;
+        memset(&p, 0, sizeof p);
+        for (int k = 0; k < 6; k++) { p.lo[k] = m.lo[k]; p.hi[k] = m.hi[k]; }
+        const int64_t above = -(int64_t)m.hi[6], below = (int64_t)m.lo[6];
+        const int64_t x = above > 0 ? above : (below > 0 ? below : 0);
+        p.d6 = (uint64_t)x * (uint64_t)x;
+        p.leaf = m.count == 0;
+        p.biome = m.biome;
+        outidx.push_back((int)out.size());
+        out.push_back(p);
+        st.push_back({v, 0});
+        return true;
+    };
+    if (!enter(root)) return false;
+    while (!st.empty()) {
+        auto &top = st.back();
+        const McgTreeNode &m = T[top.first];
+        if (top.second < m.count) {
+            const int c = m.first + top.second++;
+            if (!enter(c)) return false;
+        } else {
+            out[outidx.back()].escape = (int32_t)out.size();
+            outidx.pop_back();
+            st.pop_back();
+        }
+    }
+    return true;
+}
+
+struct GBiome {
+    PWorld W;
+    bool newf = false, perm_lds = false;
+    int lds = 0;
+    int source = 0;
+    hipStream_t st = nullptr;
+    char *d_data = nullptr;                 // program and tree, one allocation
+    uint8_t *d_out = nullptr, *d_lat = nullptr;
+    int32_t *d_xyz = nullptr, *d_bad = nullptr;
+    size_t out_cap = 0, lat_cap = 0, xyz_cap = 0;
+    void release() {
+        if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
+        (void)hipFree(d_data); (void)hipFree(d_out); (void)hipFree(d_lat); (void)hipFree(d_xyz); (void)hipFree(d_bad);
+        st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr;
+    }
+};
+
+template <typename T>
+bool grow(T **p, size_t *cap, size_t need) {
+    if (need <= *cap) return true;
+    size_t c = std::max<size_t>(need, *cap * 2);
+    (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    HCHK(hipMalloc((void **)p, c * sizeof(T)));
+    *cap = c;
+    return true;
+}
+
+// host half of mcgpu_biome_new: validation, compilation, tree layout (no device call)
+struct HostWorld {
+    Compiled C;
+    std::vector<PNode> tree;
+    const McgProg *p = nullptr;
+};
+bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
+    if (W->source != MCG_BS_MULTI && W->source != MCG_BS_END && W->source != MCG_BS_FIXED) {
+        set_error("unknown biome source %d", W->source);
+        return false;
+    }
+    if (W->qh <= 0) { set_error("world height must be positive"); return false; }
+    if (W->source == MCG_BS_FIXED) return true;
+    const McgProg *p = W->prog;
+    if (!p) { set_error("no program"); return false; }
+    if ((p->old != 0) == (W->newf != 0)) { set_error("program branch (old=%d) does not match newf=%d", p->old, W->newf); return false; }
+    if (!compile_prog(p, H.C)) return false;
+    if (W->source == MCG_BS_END && H.C.root_slot[3] < 0) { set_error("no erosion field"); return false; }
+    if (W->source == MCG_BS_MULTI)
+        for (int k = 0; k < 6; k++)
+            if (H.C.root_slot[k] < 0) { set_error("no climate field %d", k); return false; }
+    if (W->source == MCG_BS_MULTI && !convert_tree(W->tree, W->ntree, W->tree_root, H.tree)) return false;
+    const int vbytes = H.C.nslot * kWG * (W->newf ? 4 : 8);
+    if (vbytes > kMaxLdsBytes) {
+        set_error("program keeps %d values alive at once; at most %d fit", H.C.nslot, kMaxLdsBytes / (kWG * (W->newf ? 4 : 8)));
+        return false;
+    }
+    H.p = p;
+    return true;
+}
+
+// device half: buffers, stream, uploads
+bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
+    memset(&g->W, 0, sizeof g->W);
+    PWorld &P = g->W;
+    P.source = W->source;
+    P.fixed_biome = W->fixed_biome;
+    memcpy(P.end_ids, W->end_ids, sizeof P.end_ids);
+    P.min_qy = W->min_qy;
+    P.qh = W->qh;
+    P.zoom_seed = W->zoom_seed;
+    for (int k = 0; k < 6; k++) P.root_slot[k] = -1;
+    g->source = W->source;
+    g->newf = W->newf != 0;
+    HCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+    HCHK(hipMalloc((void **)&g->d_bad, sizeof(int32_t)));
+    if (W->source == MCG_BS_FIXED) return true;
+    const McgProg *p = H.p;
+    const Compiled &C = H.C;
+    // LDS: value slots always, permutation tables when they fit beside them
+    const int vbytes = C.nslot * kWG * (g->newf ? 4 : 8);
+    const int pbytes = (p->noct * 256 + 15) & ~15;
+    g->perm_lds = (int64_t)vbytes + pbytes <= kMaxLdsBytes;
+    g->lds = vbytes + (g->perm_lds ? pbytes : 0);
+    // one device allocation: ins, noise, layers, octs, sp, splf, spc, arr, thr, tree
+    struct Part { const void *src; size_t bytes; size_t off; };
+    std::vector<Part> parts = {
+        {C.ins.data(), C.ins.size() * sizeof(PIns), 0},          {p->noise, (size_t)p->nnoise * sizeof(McgNoise), 0},
+        {p->layers, (size_t)p->nlayers * sizeof(McgLayer), 0},   {p->octs, (size_t)p->noct * sizeof(McgOct), 0},
+        {p->sp, (size_t)p->nsp * sizeof(McgSpline), 0},          {p->splf, (size_t)p->nsplf * sizeof(float), 0},
+        {p->spc, (size_t)p->nspc * sizeof(int32_t), 0},          {C.arr.data(), C.arr.size() * sizeof(int32_t), 0},
+        {p->thr, (size_t)p->nthr * sizeof(double), 0},           {H.tree.data(), H.tree.size() * sizeof(PNode), 0}};
+    size_t total = 0;
+    for (auto &q : parts) { q.off = total; total += (q.bytes + 255) & ~(size_t)255; }
+    HCHK(hipMalloc((void **)&g->d_data, std::max<size_t>(total, 256)));
+    for (auto &q : parts)
+        if (q.bytes) HCHK(hipMemcpy(g->d_data + q.off, q.src, q.bytes, hipMemcpyHostToDevice));
+    auto at = [&](int i) { return (const void *)(g->d_data + parts[i].off); };
+    P.ins = (const PIns *)at(0);
+    P.noise = (const McgNoise *)at(1);
+    P.layers = (const McgLayer *)at(2);
+    P.octs = (const McgOct *)at(3);
+    P.sp = (const McgSpline *)at(4);
+    P.splf = (const float *)at(5);
+    P.spc = (const int32_t *)at(6);
+    P.arr = (const int32_t *)at(7);
+    P.thr = (const double *)at(8);
+    P.tree = (const PNode *)at(9);
+    P.nins = (int32_t)C.ins.size();
+    P.noct = p->noct;
+    P.ntree = (int32_t)H.tree.size();
+    P.nslot = C.nslot;
+    P.spl_depth = C.spl_depth;
+    for (int k = 0; k < 6; k++) P.root_slot[k] = C.root_slot[k];
+    return true;
+}
+
+int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsigned char *out, int *nunres) {
+    const int64_t n = (int64_t)nx * nz;
+    if (!grow(&g->d_out, &g->out_cap, (size_t)n)) return false;
+    HCHK(hipMemsetAsync(g->d_bad, 0, sizeof(int32_t), g->st));
+    PJob J;
+    memset(&J, 0, sizeof J);
+    J.nbad = g->d_bad;
+    // the zoomed quart cells lie in [(x0-2)>>2, ((xmax-2)>>2)+1] (mcgpu_biome.cu:215-218)
+    const int64_t xmax = (int64_t)x0 + (int64_t)(nx - 1) * step, zmax = (int64_t)z0 + (int64_t)(nz - 1) * step;
+    const int64_t qxa = (int64_t)(x0 - 2) >> 2, qxb = ((xmax - 2) >> 2) + 1, qza = (int64_t)(z0 - 2) >> 2, qzb = ((zmax - 2) >> 2) + 1;
+    const int py = (y - 2) >> 2, lo = g->W.min_qy, hi = g->W.min_qy + g->W.qh - 1;
+    int64_t ax0 = qxa, az0 = qza, nax = qxb - qxa + 1, naz = qzb - qza + 1;
+    if (g->source == MCG_BS_END) { ax0 = qxa >> 2; az0 = qza >> 2; nax = (qxb >> 2) - ax0 + 1; naz = (qzb >> 2) - az0 + 1; }
+    const int64_t nlat = 2 * nax * naz;
+    // the lattice path indexes by cell, so every block coordinate must be formed without int overflow
+    const bool fits = (int64_t)x0 - 2 >= INT32_MIN && (int64_t)z0 - 2 >= INT32_MIN && xmax <= INT32_MAX && zmax <= INT32_MAX;
+    if (g->source != MCG_BS_FIXED && fits && nlat <= n && nlat < (1ll << 31)) {
+        if (!grow(&g->d_lat, &g->lat_cap, (size_t)nlat)) return false;
+        J.gen = GEN_LATTICE;
+        J.n = nlat;
+        J.q0x = (int32_t)ax0; J.q0z = (int32_t)az0; J.nqx = (int32_t)nax; J.nqz = (int32_t)naz;
+        J.qys[0] = clampi(py, lo, hi);
+        J.qys[1] = clampi(py + 1, lo, hi);
+        J.out = g->d_lat;
+        J.nbad = nullptr;                    // counted per output point by the zoom pass
+        HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        HCHK(launch_zoom_lattice(g->W, J, x0, z0, nx, nz, step, y, g->d_out, g->d_bad, g->st));
+    } else {
+        J.gen = GEN_GRIDPTS;
+        J.n = n;
+        J.x0 = x0; J.z0 = z0; J.nx = nx; J.step = step; J.y = y;
+        J.out = g->d_out;
+        HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    }
+    int32_t bad = 0;
+    HCHK(hipMemcpyAsync(out, g->d_out, (size_t)n, hipMemcpyDeviceToHost, g->st));
+    HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
+    HCHK(hipStreamSynchronize(g->st));
+    if (nunres) *nunres = bad;
+    return true;
+}
+
+bool run_points(GBiome *g, int n, const int *xyz, unsigned char *out, int *nunres) {
+    if (n == 0) {
+        if (nunres) *nunres = 0;
+        return true;
+    }
+    if (!grow(&g->d_out, &g->out_cap, (size_t)n) || !grow(&g->d_xyz, &g->xyz_cap, (size_t)n * 3)) return false;
+    HCHK(hipMemsetAsync(g->d_bad, 0, sizeof(int32_t), g->st));
+    HCHK(hipMemcpyAsync(g->d_xyz, xyz, (size_t)n * 3 * sizeof(int32_t), hipMemcpyHostToDevice, g->st));
+    PJob J;
+    memset(&J, 0, sizeof J);
+    J.gen = GEN_POINTS;
+    J.n = n;
+    J.xyz = g->d_xyz;
+    J.out = g->d_out;
+    J.nbad = g->d_bad;
+    HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    int32_t bad = 0;
+    HCHK(hipMemcpyAsync(out, g->d_out, (size_t)n, hipMemcpyDeviceToHost, g->st));
+    HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
+    HCHK(hipStreamSynchronize(g->st));
+    if (nunres) *nunres = bad;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcgpu_abi(void) { return MCGPU_ABI; }
+
+int mcgpu_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    return n;
+}
+
+int mcgpu_device_info(int i, char *name, size_t namelen, int *cc_major, int *cc_minor, size_t *mem_mb) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, i) != hipSuccess) { (void)hipGetLastError(); return 1; }
+    if (name && namelen) { strncpy(name, p.name, namelen - 1); name[namelen - 1] = 0; }
+    if (cc_major) *cc_major = p.major;
+    if (cc_minor) *cc_minor = p.minor;
+    if (mem_mb) *mem_mb = p.totalGlobalMem >> 20;
+    return 0;
+}
+
+int mcgpu_select(int device) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_inited && device == g_dev) return 0;
+    if (g_inited) { set_error("device already selected (%d)", g_dev); return 1; }
+    g_dev = device;
+    return ensure_init() ? 0 : 1;
+}
+
+int mcgpu_selected(void) { return g_inited ? g_dev : -1; }
+
+const char *mcgpu_last_error(void) { return g_err; }
+
+int mcgpu_mem_info(size_t *free_mb, size_t *total_mb) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!ensure_init()) return 1;
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) { set_error("hipMemGetInfo failed"); return 1; }
+    if (free_mb) *free_mb = f >> 20;
+    if (total_mb) *total_mb = t >> 20;
+    return 0;
+}
+
+void *mcgpu_biome_new(const McgBiomeWorld *W, char *err, size_t errlen) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto fail = [&]() -> void * {
+        if (err && errlen) snprintf(err, errlen, "%s", g_err);
+        return nullptr;
+    };
+    if (!W) { set_error("null world"); return fail(); }
+    HostWorld H;
+    if (!prepare_world(W, H)) return fail();               // everything about the description, on the host
+    if (!ensure_init()) return fail();
+    std::unique_ptr<GBiome> g(new GBiome());
+    if (!build_world(g.get(), W, H)) { g->release(); return fail(); }
+    return g.release();
+}
+
+void mcgpu_biome_free(void *h) {
+    if (!h) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    GBiome *g = (GBiome *)h;
+    if (ensure_init()) g->release();
+    delete g;
+}
+
+int mcgpu_biome_grid(void *h, int x0, int z0, int nx, int nz, int step, int y, unsigned char *out, int *nunres) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!h || nx <= 0 || nz <= 0 || step <= 0 || !out) { set_error("bad arguments"); return 1; }
+    if (!ensure_init()) return 1;
+    return run_grid((GBiome *)h, x0, z0, nx, nz, step, y, out, nunres) ? 0 : 1;
+}
+
+int mcgpu_biome_points(void *h, int n, const int *xyz, unsigned char *out, int *nunres) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!h || n < 0 || (n > 0 && (!xyz || !out))) { set_error("bad arguments"); return 1; }
+    if (!ensure_init()) return 1;
+    return run_points((GBiome *)h, n, xyz, out, nunres) ? 0 : 1;
+}
+
+}  // extern "C"

@@ -1,0 +1,87 @@
+// mcg_plugin.h - internals of the libmcgen GPU plugin (include/mcgpu.h): the compiled form of a
+// density-function program and of the biome R-tree, shared by the host side (mcg_plugin.cpp) and the
+// kernels (mcg_plugin.hip).
+//
+// A McgProg arrives as a node list in evaluation order (children first).  The host compiles it once per
+// world into PIns records whose operands are value slots, not node numbers: a linear scan over the
+// nodes gives each value a slot from its definition to its last use, so a lane keeps only the values
+// that are still live (a climate program of ~100 nodes needs ~10 slots) and the slots fit in LDS.
+// Every lane of a wave evaluates the same instruction stream, so each record is read with scalar loads.
+#ifndef MCG_PLUGIN_H
+#define MCG_PLUGIN_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mcgpu.h"
+
+namespace mcb {
+namespace plug {
+
+constexpr int kWG = 256;                 // lanes per workgroup of the plugin kernels
+constexpr int kChainCap = 12;            // tied cells followed back along the fill order (reference: 12)
+constexpr int kSplineDepth = 16;         // spline nesting accepted (the reference's frame stack holds 16)
+constexpr int kMaxLdsBytes = 64 * 1024;  // LDS per workgroup the kernels may use
+
+// one program node, compiled (128 bytes; operand fields are value slots)
+struct PIns {
+    int32_t op, dst, a, b, c;
+    int32_t ns, sp, arr0, narr, thr0, nthr;
+    int32_t i0, i1, i2, i3;
+    int32_t cm, cx, cy, cz, oct;
+    float f0, f1, f2, f3;
+    double d0, d1;
+    int32_t pad[4];
+};
+static_assert(sizeof(PIns) == 128, "PIns layout");
+
+// R-tree node in pre-order with escape links (the reference's BFS array, walked depth-first in its child
+// order).  d6 = squared distance of the seventh (offset) parameter to a target offset of 0.
+struct PNode {
+    int32_t lo[6], hi[6];
+    uint64_t d6;
+    int32_t escape, leaf, biome, pad;
+};
+static_assert(sizeof(PNode) == 72, "PNode layout");
+
+// device view of one world
+struct PWorld {
+    const PIns *ins;
+    const McgNoise *noise;
+    const McgLayer *layers;
+    const McgOct *octs;            // offsets (and the permutation tables when they are not staged)
+    const McgSpline *sp;
+    const float *splf;
+    const int32_t *spc;
+    const int32_t *arr;            // slot numbers (ISEL choices, spline coordinates)
+    const double *thr;
+    const PNode *tree;
+    int32_t nins, noct, ntree, nslot, spl_depth;
+    int32_t root_slot[6];          // -1: field absent
+    int32_t source, fixed_biome, end_ids[5], min_qy, qh;
+    int64_t zoom_seed;
+};
+
+// work description of one launch
+enum { GEN_LATTICE = 0, GEN_POINTS = 1, GEN_GRIDPTS = 2 };
+struct PJob {
+    int32_t gen;
+    int64_t n;                     // items
+    // lattice: cells (q0x + i % nqx, qys[layer], q0z + (i / nqx) % nqz), layer = i / (nqx * nqz); for the
+    // End source a lattice of chunks (c0x + i % nqx, c0z + ..., qys[layer])
+    int32_t q0x, q0z, nqx, nqz, qys[2];
+    // grid points: block (x0 + ix * step, y, z0 + iz * step)
+    int32_t x0, z0, nx, step, y;
+    const int32_t *xyz;            // points: 3 ints per item
+    uint8_t *out;
+    int32_t *nbad;
+};
+
+// launchers (mcg_plugin.hip); lds = bytes of dynamic LDS, perm_lds = permutation tables staged in LDS
+hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st);
+// block grid from a lattice of cells (MULTI) or of chunks (END): out[iz*nx+ix]
+hipError_t launch_zoom_lattice(const PWorld &W, const PJob &L, int x0, int z0, int nx, int nz, int step, int y,
+                               uint8_t *out, int32_t *nbad, hipStream_t st);
+
+}  // namespace plug
+}  // namespace mcb
+#endif

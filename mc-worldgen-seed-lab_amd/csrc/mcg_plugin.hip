@@ -1,0 +1,706 @@
+// mcg_plugin.hip - kernels of the libmcgen GPU plugin (include/mcgpu.h).
+//
+// One lane per quart cell (or per point): BiomeManager zoom -> climate program -> R-tree search with the
+// chunk-fill-order tie chain.  Semantics follow the reference plugin and the libmcgen CPU code it mirrors
+// (libmcgen/gpu/mcgpu_dev.cuh:1-544 and mcgpu_biome.cu:13-122; CPU: libmcgen/src/noise.c:339-347,
+// df_new.c:655-836, df_old.c, biome.c:26-120), with the same order of every floating-point operation; the
+// build uses -ffp-contract=off and no fast-math, so each multiply and add rounds separately.  The fused
+// multiply-adds written below have an operand in {-1, 0, 1}: the product is exact, so they round like the
+// separate product and sum (same value, same signed zero).
+//
+// MI355X layout: the compiled program (PIns) is read with scalar loads, every lane running the same
+// instruction; live values sit in LDS slots, column per lane (slot s of lane l at s*256 + l, no bank
+// conflicts); the octave permutation tables are staged in LDS once per workgroup when they fit.
+#include "mcg_plugin.h"
+#include "mcb_device.h"
+
+namespace mcb {
+namespace plug {
+
+using dev::gcomp;
+using dev::kGX;
+using dev::kGY;
+using dev::kGZ;
+
+// ---------------------------------------------------------------------------------------------------
+// Java arithmetic (mcgpu_dev.cuh:24-58)
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int32_t jd2i(double v) {
+    if (v != v) return 0;
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return (int32_t)0x80000000u;
+    return (int32_t)v;
+}
+__device__ __forceinline__ int64_t jd2l(double v) {
+    if (v != v) return 0;
+    if (v >= 9223372036854775807.0) return INT64_MAX;
+    if (v <= -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)v;
+}
+__device__ __forceinline__ int32_t jfloor(double v) { return jd2i(floor(v)); }
+__device__ __forceinline__ int32_t jfloordiv(int32_t a, int32_t b) {
+    int32_t q = a / b;
+    if ((a % b != 0) && ((a < 0) != (b < 0))) q--;
+    return q;
+}
+__device__ __forceinline__ int32_t jfloormod(int32_t a, int32_t b) {
+    int32_t m = a % b;
+    if (m != 0 && ((m < 0) != (b < 0))) m += b;
+    return m;
+}
+__device__ __forceinline__ float jlerpf(float a, float p0, float p1) { return p0 + a * (p1 - p0); }
+__device__ __forceinline__ double jlerp(double a, double p0, double p1) { return p0 + a * (p1 - p0); }
+__device__ __forceinline__ float jmaxf(float a, float b) {
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && signbit(a)) return b;
+    return a >= b ? a : b;
+}
+__device__ __forceinline__ float jminf(float a, float b) {
+    if (a != a) return a;
+    if (a == 0.0f && b == 0.0f && signbit(b)) return b;
+    return a <= b ? a : b;
+}
+__device__ __forceinline__ double jmax(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0 && signbit(a)) return b;
+    return a >= b ? a : b;
+}
+__device__ __forceinline__ double jmin(double a, double b) {
+    if (a != a) return a;
+    if (a == 0.0 && b == 0.0 && signbit(b)) return b;
+    return a <= b ? a : b;
+}
+__device__ __forceinline__ float jclampf(float v, float lo, float hi) { return v < lo ? lo : jminf(v, hi); }
+__device__ __forceinline__ double jclamp(double v, double lo, double hi) { return v < lo ? lo : jmin(v, hi); }
+__device__ __forceinline__ float jsignf(float v) {
+    if (v != v || v == 0.0f) return v;
+    return v > 0.0f ? 1.0f : -1.0f;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// octaves (mcgpu_dev.cuh:64-155; CPU noise.c)
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float pdot_f(int h, float x, float y, float z) {
+    h &= 15;
+    return fmaf((float)gcomp(kGZ, h), z, fmaf((float)gcomp(kGY, h), y, (float)gcomp(kGX, h) * x));
+}
+__device__ __forceinline__ double pdot_d(int h, double x, double y, double z) {
+    h &= 15;
+    return fma((double)gcomp(kGZ, h), z, fma((double)gcomp(kGY, h), y, (double)gcomp(kGX, h) * x));
+}
+__device__ __forceinline__ float smooth_f(float x) { return x * x * x * (x * (x * 6.0f - 15.0f) + 10.0f); }
+__device__ __forceinline__ double smooth_d(double x) { return x * x * x * (x * (x * 6.0 - 15.0) + 10.0); }
+__device__ __forceinline__ double wrap_new(double x) {
+    const double HALF = 16777215.999999998;
+    return (x >= -HALF && x < HALF) ? x : x - floor(x / 3.3554432E7 + 0.5) * 3.3554432E7;
+}
+__device__ __forceinline__ double wrap_old(double x) { return x - (double)jd2l(floor(x / 3.3554432E7 + 0.5)) * 3.3554432E7; }
+
+#define PP(i) ((int)P[(i) & 0xFF])
+__device__ __forceinline__ float sample_lerp_f(const uint8_t *P, int x, int y, int z, float rx, float ry, float rz,
+                                               float ry_orig) {
+    const int x0 = PP(x), x1 = PP(x + 1);
+    const int xy00 = PP(x0 + y), xy01 = PP(x0 + y + 1), xy10 = PP(x1 + y), xy11 = PP(x1 + y + 1);
+    const float gx = rx - 1.0f, gy = ry - 1.0f, gz = rz - 1.0f;
+    const float d000 = pdot_f(PP(xy00 + z), rx, ry, rz), d100 = pdot_f(PP(xy10 + z), gx, ry, rz);
+    const float d010 = pdot_f(PP(xy01 + z), rx, gy, rz), d110 = pdot_f(PP(xy11 + z), gx, gy, rz);
+    const float d001 = pdot_f(PP(xy00 + z + 1), rx, ry, gz), d101 = pdot_f(PP(xy10 + z + 1), gx, ry, gz);
+    const float d011 = pdot_f(PP(xy01 + z + 1), rx, gy, gz), d111 = pdot_f(PP(xy11 + z + 1), gx, gy, gz);
+    const float a1 = smooth_f(rx), a2 = smooth_f(ry_orig), a3 = smooth_f(rz);
+    return jlerpf(a3, jlerpf(a2, jlerpf(a1, d000, d100), jlerpf(a1, d010, d110)),
+                  jlerpf(a2, jlerpf(a1, d001, d101), jlerpf(a1, d011, d111)));
+}
+// GradientNoise (26.3+): perlin (kind 0) or smeared (kind 1, y fudge fs)
+__device__ __forceinline__ float octave_f(const uint8_t *P, const McgOct *o, int kind, double fs, double _x, double _y,
+                                          double _z) {
+    const double x = wrap_new(_x) + o->xo, y = wrap_new(_y) + o->yo, z = wrap_new(_z) + o->zo;
+    const int32_t fx = jfloor(x), fy = jfloor(y), fz = jfloor(z);
+    const float rx = (float)(x - fx), rz = (float)(z - fz);
+    const double ry = y - fy;
+    float fry = (float)ry;
+    if (kind != 0) {
+        const double lim = (_y >= 0.0 && _y < ry) ? _y : ry;
+        fry = (float)(ry - (double)jfloor(lim / fs + (double)1.0E-7f) * fs);
+    }
+    return sample_lerp_f(P, fx, fy, fz, rx, fry, rz, (float)ry);
+}
+// ImprovedNoise.noise of the legacy path, coordinates already wrapped (yScale = yFudge = 0*factor)
+__device__ __forceinline__ double octave_d(const uint8_t *P, const McgOct *o, double _x, double _y, double _z,
+                                           double y_scale, double y_fudge) {
+    const double x = _x + o->xo, y = _y + o->yo, z = _z + o->zo;
+    const int32_t xf = jfloor(x), yf = jfloor(y), zf = jfloor(z);
+    const double xr = x - xf, yr = y - yf, zr = z - zf;
+    double yrf = 0.0;
+    if (y_scale != 0.0) {
+        const double lim = (y_fudge >= 0.0 && y_fudge < yr) ? y_fudge : yr;
+        yrf = (double)jfloor(lim / y_scale + (double)1.0E-7f) * y_scale;
+    }
+    const double yy = yr - yrf;
+    const int x0 = PP(xf), x1 = PP(xf + 1);
+    const int xy00 = PP(x0 + yf), xy01 = PP(x0 + yf + 1), xy10 = PP(x1 + yf), xy11 = PP(x1 + yf + 1);
+    const double d000 = pdot_d(PP(xy00 + zf), xr, yy, zr), d100 = pdot_d(PP(xy10 + zf), xr - 1.0, yy, zr);
+    const double d010 = pdot_d(PP(xy01 + zf), xr, yy - 1.0, zr), d110 = pdot_d(PP(xy11 + zf), xr - 1.0, yy - 1.0, zr);
+    const double d001 = pdot_d(PP(xy00 + zf + 1), xr, yy, zr - 1.0), d101 = pdot_d(PP(xy10 + zf + 1), xr - 1.0, yy, zr - 1.0);
+    const double d011 = pdot_d(PP(xy01 + zf + 1), xr, yy - 1.0, zr - 1.0);
+    const double d111 = pdot_d(PP(xy11 + zf + 1), xr - 1.0, yy - 1.0, zr - 1.0);
+    const double a1 = smooth_d(xr), a2 = smooth_d(yr), a3 = smooth_d(zr);
+    return jlerp(a3, jlerp(a2, jlerp(a1, d000, d100), jlerp(a1, d010, d110)),
+                 jlerp(a2, jlerp(a1, d001, d101), jlerp(a1, d011, d111)));
+}
+#undef PP
+
+template <bool PL>
+__device__ __forceinline__ const uint8_t *perm_of(const PWorld &W, const uint8_t *lds_perm, int oct) {
+    return PL ? lds_perm + oct * 256 : W.octs[oct].p;
+}
+// NoiseStack value (26.3+; mcgpu_dev.cuh:157-168, noise.c:339-347)
+template <bool PL>
+__device__ float ns_get(const PWorld &W, const uint8_t *lp, int nsi, double x, double y, double z) {
+    const McgNoise mn = W.noise[nsi];
+    float v = 0.0f;
+#pragma unroll 1
+    for (int i = 0; i < mn.n0; i++) {
+        const McgLayer L = W.layers[mn.l0 + i];
+        const double f = L.a;
+        const float g = octave_f(perm_of<PL>(W, lp, L.oct), W.octs + L.oct, L.kind, L.b, x * f, y * f, z * f);
+        v += L.amp_f * g;
+    }
+    return v;
+}
+// legacy PerlinNoise.getValue (yScale = yFudge = 0) over one layer range (mcgpu_dev.cuh:170-181)
+template <bool PL>
+__device__ double old_perlin(const PWorld &W, const uint8_t *lp, int l0, int n, double x, double y, double z) {
+    double value = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < n; i++) {
+        const McgLayer L = W.layers[l0 + i];
+        if (L.oct >= 0) {
+            const double factor = L.a;
+            const double nv = octave_d(perm_of<PL>(W, lp, L.oct), W.octs + L.oct, wrap_old(x * factor), wrap_old(y * factor),
+                                       wrap_old(z * factor), 0.0 * factor, 0.0 * factor);
+            value += L.amp_d * nv * L.b;
+        }
+    }
+    return value;
+}
+template <bool PL>
+__device__ double old_normal(const PWorld &W, const uint8_t *lp, int nsi, double x, double y, double z) {
+    const McgNoise mn = W.noise[nsi];
+    const double x2 = x * 1.0181268882175227, y2 = y * 1.0181268882175227, z2 = z * 1.0181268882175227;
+    return (old_perlin<PL>(W, lp, mn.l0, mn.n0, x, y, z) + old_perlin<PL>(W, lp, mn.l1, mn.n1, x2, y2, z2)) *
+           mn.value_factor;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// End islands height (mcgpu_dev.cuh:212-243; df_new.c:659-673, df_old.c:449-465) with Java's clamp and
+// max: a NaN centre term (the 26.1/26.2 section square wraps negative beyond |section| ~ 32768) stays NaN
+// ---------------------------------------------------------------------------------------------------
+template <bool OLD>
+__device__ float end_height_j(const uint8_t *P, int sx, int sz) {
+    const int cx = sx / 2, cz = sz / 2, subx = sx % 2, subz = sz % 2;
+    float doffs = -100.0f;
+    if (OLD) {
+        const int32_t sq = (int32_t)((uint32_t)sx * (uint32_t)sx + (uint32_t)sz * (uint32_t)sz);
+        doffs = jclampf(100.0f - (float)sqrt((double)(float)sq) * 8.0f, -100.0f, 80.0f);
+    }
+#pragma unroll 1
+    for (int xo = -12; xo <= 12; xo++)
+#pragma unroll 1
+        for (int zo = -12; zo <= 12; zo++) {
+            const int64_t tx = (int64_t)cx + xo, tz = (int64_t)cz + zo;
+            if (tx * tx + tz * tz <= 4096LL) continue;
+            const double nv = dev::simplex2(P, (double)tx, (double)tz);
+            if (OLD ? !(nv < (double)-0.9f) : !((float)nv < -0.9f)) continue;
+            const float size = fmodf(fabsf((float)tx) * 3439.0f + fabsf((float)tz) * 147.0f, 13.0f) + 9.0f;
+            const float xd = (float)(subx - xo * 2), zd = (float)(subz - zo * 2);
+            const float nd = jclampf(100.0f - (float)sqrt((double)(xd * xd + zd * zd)) * size, -100.0f, 80.0f);
+            doffs = jmaxf(doffs, nd);
+        }
+    return doffs;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// spline (mcgpu_dev.cuh:246-289, df_new.c:707-728): bounded compile-time recursion, one call site per
+// level; coordinates are the spline node's coordinate slots
+// ---------------------------------------------------------------------------------------------------
+template <typename VT, int D>
+__device__ float spline(const PWorld &W, int spi, const VT *vals, int arr0) {
+    if constexpr (D == 0) {
+        return __builtin_nanf("");                      // unreachable: the host bounds the depth
+    } else {
+        const McgSpline s = W.sp[spi];
+        if (s.is_const) return s.value;
+        const float in = (float)vals[W.arr[arr0 + s.coord] * kWG];
+        const float *loc = W.splf + s.loc0, *der = W.splf + s.loc0 + s.n;
+        int from = 0, len = s.n;
+        while (len > 0) {
+            const int half = len / 2, mid = from + half;
+            if (in < loc[mid]) len = half;
+            else { from = mid + 1; len -= half + 1; }
+        }
+        const int start = from - 1, last = s.n - 1;
+        const bool edge = start < 0 || start == last;
+        const int base = start < 0 ? 0 : start;
+        float ya = 0.0f, yb = 0.0f;
+#pragma unroll 1
+        for (int q = 0; q < (edge ? 1 : 2); q++) {
+            const float v = spline<VT, D - 1>(W, W.spc[s.val0 + base + q], vals, arr0);
+            if (q == 0) ya = v; else yb = v;
+        }
+        if (edge) {
+            const float d = der[base];
+            return d == 0.0f ? ya : ya + d * (in - loc[base]);
+        }
+        const float x1 = loc[start], x2 = loc[start + 1];
+        const float t = (in - x1) / (x2 - x1);
+        const float d1 = der[start], d2 = der[start + 1];
+        const float a = d1 * (x2 - x1) - (yb - ya);
+        const float b = -d2 * (x2 - x1) + (yb - ya);
+        return jlerpf(t, ya, yb) + t * (1.0f - t) * jlerpf(t, a, b);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// program evaluation: every node in order, values into the lane's LDS slots
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float squeeze_f(float v) {
+    const float x = jclampf(v, -1.0f, 1.0f);
+    return x / 2.0f - (x * x * x) / 24.0f;
+}
+__device__ __forceinline__ float grad_value(const PIns &s, int coord) {    // df_new.c:676-694
+    if (s.op == MCGK_GRAD_CLAMP) {
+        const int cc = coord < s.i2 ? s.i2 : (coord > s.i3 ? s.i3 : coord);
+        return s.f0 + (float)(cc - s.i1) * s.f1;
+    }
+    if (s.op == MCGK_GRAD_REPEAT) return s.f0 + (float)jfloormod(coord - s.i1, s.i2) * s.f1;
+    const int rel = coord - s.i1;
+    const int tile = jfloordiv(rel, s.i2);
+    const int local = rel - tile * s.i2;
+    return (tile & 1) == 0 ? s.f0 + (float)local * s.f1 : s.f0 + (float)(s.i2 - local) * s.f1;
+}
+__device__ __forceinline__ float dist_value(int m, float dx, float dy, float dz) {         // df_new.c:695-702
+    switch (m) {
+    case 0: return (float)sqrt((double)(dx * dx + dy * dy + dz * dz));
+    case 1: return dx * dx + dy * dy + dz * dz;
+    case 2: return fabsf(dx) + fabsf(dy) + fabsf(dz);
+    default: return jmaxf(jmaxf(fabsf(dx), fabsf(dy)), fabsf(dz));
+    }
+}
+
+// 26.3+ float program (mcgpu_dev.cuh:305-374; df_new.c:740-836 with caches off)
+template <bool PL, int SPD>
+__device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox, int oy, int oz) {
+#define V(x) vals[(x) * kWG]
+#pragma unroll 1
+    for (int i = 0; i < W.nins; i++) {
+        const PIns s = W.ins[i];
+        const int bx = (s.cm & 1) ? s.cx : ox, by = (s.cm & 2) ? s.cy : oy, bz = (s.cm & 4) ? s.cz : oz;
+        float r;
+        switch (s.op) {
+        case MCGK_CONST: r = s.f0; break;
+        case MCGK_NOISE: r = ns_get<PL>(W, lp, s.ns, bx * s.d0, by * s.d1, bz * s.d0); break;
+        case MCGK_NOISE_XZ: {
+            const double nx = bx * s.d0 + (double)V(s.a), ny = by * s.d1, nz = bz * s.d0 + (double)V(s.c);
+            r = ns_get<PL>(W, lp, s.ns, nx, ny, nz);
+            break;
+        }
+        case MCGK_NOISE_XYZ: {
+            const double nx = bx * s.d0 + (double)V(s.a), ny = by * s.d1 + (double)V(s.b), nz = bz * s.d0 + (double)V(s.c);
+            r = ns_get<PL>(W, lp, s.ns, nx, ny, nz);
+            break;
+        }
+        case MCGK_SHIFT_B: r = ns_get<PL>(W, lp, s.ns, bz * s.d0, bx * s.d0, 0.0) * 4.0f; break;
+        case MCGK_END: r = (end_height_j<false>(perm_of<PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0f) / 128.0f; break;
+        case MCGK_DIST: r = dist_value(s.i3, (float)(s.i0 - bx), (float)(s.i1 - by), (float)(s.i2 - bz)); break;
+        case MCGK_GRAD_CLAMP: case MCGK_GRAD_REPEAT: case MCGK_GRAD_MIRROR:
+            r = grad_value(s, s.i0 == 0 ? bx : (s.i0 == 1 ? by : bz));
+            break;
+        case MCGK_CTX_ALPHA: case MCGK_CTX_OFFSET: case MCGK_CTX_BEARD: r = s.f0; break;
+        case MCGK_ABS: r = fabsf(V(s.a)); break;
+        case MCGK_SQUARE: { const float v = V(s.a); r = v * v; break; }
+        case MCGK_CUBE: { const float v = V(s.a); r = v * v * v; break; }
+        case MCGK_SQRT: r = (float)sqrt((double)V(s.a)); break;
+        case MCGK_LEAKY: { const float v = V(s.a); r = v > 0.0f ? v : v * s.f0; break; }
+        case MCGK_RECIP: r = 1.0f / V(s.a); break;
+        case MCGK_NEG: r = -V(s.a); break;
+        case MCGK_SQUEEZE: r = squeeze_f(V(s.a)); break;
+        case MCGK_SIGN: r = jsignf(V(s.a)); break;
+        case MCGK_ADD: r = V(s.a) + V(s.b); break;
+        case MCGK_CADD: r = V(s.a) + s.f0; break;
+        case MCGK_CSUB: r = s.f0 - V(s.a); break;
+        case MCGK_SUB: r = V(s.a) - V(s.b); break;
+        case MCGK_MUL: { const float l = V(s.a); r = l == 0.0f ? 0.0f : l * V(s.b); break; }
+        case MCGK_CMUL: r = V(s.a) * s.f0; break;
+        case MCGK_DIV: { const float l = V(s.a); r = l == 0.0f ? 0.0f : l / V(s.b); break; }
+        case MCGK_CDIV: r = s.f0 / V(s.a); break;
+        case MCGK_MIN: { const float l = V(s.a); r = l <= s.f0 ? l : jminf(l, V(s.b)); break; }
+        case MCGK_CMIN: r = jminf(V(s.a), s.f0); break;
+        case MCGK_MAX: { const float l = V(s.a); r = l >= s.f0 ? l : jmaxf(l, V(s.b)); break; }
+        case MCGK_CMAX: r = jmaxf(V(s.a), s.f0); break;
+        case MCGK_SPLINE: r = spline<float, SPD>(W, s.sp, vals, s.arr0); break;
+        case MCGK_LERP: {
+            const float a = V(s.a);
+            r = a == 0.0f ? V(s.b) : (a == 1.0f ? V(s.c) : jlerpf(a, V(s.b), V(s.c)));
+            break;
+        }
+        case MCGK_LERP_CF: {
+            const float a = V(s.a);
+            r = a == 0.0f ? s.f0 : (a == 1.0f ? V(s.c) : jlerpf(a, s.f0, V(s.c)));
+            break;
+        }
+        case MCGK_LERP_CS: {
+            const float a = V(s.a);
+            r = a == 0.0f ? V(s.b) : (a == 1.0f ? s.f0 : jlerpf(a, V(s.b), s.f0));
+            break;
+        }
+        case MCGK_CLAMP: r = jclampf(V(s.a), s.f0, s.f1); break;
+        case MCGK_RANGE_C: { const float v = V(s.a); r = (v >= s.f0 && v < s.f1) ? s.f2 : s.f3; break; }
+        case MCGK_RANGE: { const float v = V(s.a); r = (v >= s.f0 && v < s.f1) ? V(s.b) : V(s.c); break; }
+        case MCGK_ISEL1: { const float v = V(s.a); r = v < s.f0 ? V(s.b) : V(s.c); break; }
+        case MCGK_ISEL: {
+            const float v = V(s.a);
+            int k = 0;
+            for (; k < s.nthr; k++)
+                if (v < (float)W.thr[s.thr0 + k]) break;
+            if (k == s.nthr) k = s.narr - 1;
+            r = V(W.arr[s.arr0 + k]);
+            break;
+        }
+        default: r = V(s.a); break;            // CACHE, BLEND_DENSITY, SLICE_*: the argument's value
+        }
+        V(s.dst) = r;
+    }
+#undef V
+}
+
+// 26.1/26.2 double program (mcgpu_dev.cuh:387-437; df_old.c point evaluation)
+__device__ __forceinline__ double clamped_map(double v, double a, double b, double ta, double tb) {
+    const double f = (v - a) / (b - a);
+    return f < 0.0 ? ta : (f > 1.0 ? tb : jlerp(f, ta, tb));
+}
+__device__ __forceinline__ double weird_rarity(int type, double r) {
+    if (type == 0) return r < -0.5 ? 0.75 : (r < 0.0 ? 1.0 : (r < 0.5 ? 1.5 : 2.0));
+    return r < -0.75 ? 0.5 : (r < -0.5 ? 0.75 : (r < 0.5 ? 1.0 : (r < 0.75 ? 2.0 : 3.0)));
+}
+template <bool PL, int SPD>
+__device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int bx, int by, int bz) {
+#define V(x) vals[(x) * kWG]
+#pragma unroll 1
+    for (int i = 0; i < W.nins; i++) {
+        const PIns s = W.ins[i];
+        double r;
+        switch (s.op) {
+        case MCGO_CONST: r = s.d0; break;
+        case MCGO_NOISE: r = old_normal<PL>(W, lp, s.ns, bx * s.d0, by * s.d1, bz * s.d0); break;
+        case MCGO_SHIFTED_NOISE: {
+            const double x = bx * s.d0 + V(s.a), y = by * s.d1 + V(s.b), z = bz * s.d0 + V(s.c);
+            r = old_normal<PL>(W, lp, s.ns, x, y, z);
+            break;
+        }
+        case MCGO_SHIFT_A: r = old_normal<PL>(W, lp, s.ns, (double)bx * s.d0, 0.0 * s.d0, (double)bz * s.d0) * 4.0; break;
+        case MCGO_SHIFT_B: r = old_normal<PL>(W, lp, s.ns, (double)bz * s.d0, (double)bx * s.d0, 0.0 * s.d0) * 4.0; break;
+        case MCGO_SHIFT: r = old_normal<PL>(W, lp, s.ns, (double)bx * s.d0, (double)by * s.d1, (double)bz * s.d0) * 4.0; break;
+        case MCGO_END_ISLANDS:
+            r = ((double)end_height_j<true>(perm_of<PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0) / 128.0;
+            break;
+        case MCGO_Y_GRADIENT: r = clamped_map((double)by, (double)s.i0, (double)s.i1, s.d0, s.d1); break;
+        case MCGO_WEIRD_SCALED: {
+            const double rr = weird_rarity(s.i0, V(s.a));
+            r = rr * fabs(old_normal<PL>(W, lp, s.ns, bx / rr, by / rr, bz / rr));
+            break;
+        }
+        case MCGO_BLEND_ALPHA: r = 1.0; break;
+        case MCGO_ZERO: r = 0.0; break;
+        case MCGO_ABS: r = fabs(V(s.a)); break;
+        case MCGO_SQUARE: { const double v = V(s.a); r = v * v; break; }
+        case MCGO_CUBE: { const double v = V(s.a); r = v * v * v; break; }
+        case MCGO_HALF_NEG: { const double v = V(s.a); r = v > 0.0 ? v : v * 0.5; break; }
+        case MCGO_QUARTER_NEG: { const double v = V(s.a); r = v > 0.0 ? v : v * 0.25; break; }
+        case MCGO_RECIP: r = 1.0 / V(s.a); break;
+        case MCGO_SQUEEZE: { const double c = jclamp(V(s.a), -1.0, 1.0); r = c / 2.0 - c * c * c / 24.0; break; }
+        case MCGO_CLAMP: r = jclamp(V(s.a), s.d0, s.d1); break;
+        case MCGO_ADD: r = V(s.a) + V(s.b); break;
+        case MCGO_MUL: { const double v = V(s.a); r = v == 0.0 ? 0.0 : v * V(s.b); break; }
+        case MCGO_MIN: { const double v = V(s.a); r = v < s.d1 ? v : jmin(v, V(s.b)); break; }
+        case MCGO_MAX: { const double v = V(s.a); r = v > s.d1 ? v : jmax(v, V(s.b)); break; }
+        case MCGO_CADD: r = V(s.a) + s.d0; break;
+        case MCGO_CMUL: r = V(s.a) * s.d0; break;
+        case MCGO_RANGE: { const double v = V(s.a); r = (v >= s.d0 && v < s.d1) ? V(s.b) : V(s.c); break; }
+        case MCGO_ISEL: {
+            const double v = V(s.a);
+            int k = 0;
+            for (; k < s.nthr; k++)
+                if (v < W.thr[s.thr0 + k]) break;
+            if (k == s.nthr) k = s.narr - 1;
+            r = V(W.arr[s.arr0 + k]);
+            break;
+        }
+        case MCGO_SPLINE: r = (double)spline<double, SPD>(W, s.sp, vals, s.arr0); break;
+        default: r = V(s.a); break;            // PASS
+        }
+        V(s.dst) = r;
+    }
+#undef V
+}
+
+template <bool NEWF> struct VT_of;
+template <> struct VT_of<true> { typedef float T; };
+template <> struct VT_of<false> { typedef double T; };
+
+template <bool NEWF, bool PL, int SPD>
+__device__ __forceinline__ void eval_prog(const PWorld &W, const uint8_t *lp, typename VT_of<NEWF>::T *vals, int bx, int by,
+                                          int bz) {
+    if constexpr (NEWF) eval_new<PL, SPD>(W, lp, vals, bx, by, bz);
+    else eval_old<PL, SPD>(W, lp, vals, bx, by, bz);
+}
+
+// ---------------------------------------------------------------------------------------------------
+// R-tree (mcgpu_dev.cuh:440-500): distances in wrapping 64-bit arithmetic as the reference's u64 sums
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t pdist(const PNode &n, const int64_t *tg) {
+    uint64_t d = n.d6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int64_t above = tg[k] - (int64_t)n.hi[k], below = (int64_t)n.lo[k] - tg[k];
+        const int64_t x = above > 0 ? above : (below > 0 ? below : 0);
+        d += (uint64_t)x * (uint64_t)x;
+    }
+    return d;
+}
+// first leaf of least distance in the reference's child order (its frame walk keeps one running best
+// and enters a child when best > distance); -1 when none is accepted
+__device__ int rt_find(const PNode *__restrict__ T, int n, const int64_t *tg) {
+    uint64_t best = 0x7fffffffffffffffULL;
+    int bl = -1, cur = 1;
+    while (cur < n) {
+        const PNode nd = T[cur];
+        const uint64_t cd = pdist(nd, tg);
+        if (best > cd) {
+            if (nd.leaf) { best = cd; bl = cur; cur = nd.escape; }
+            else cur++;
+        } else {
+            cur = nd.escape;
+        }
+    }
+    return bl;
+}
+// at least two leaves at distance d (children entered when their distance is <= d)
+__device__ bool rt_tie(const PNode *__restrict__ T, int n, const int64_t *tg, uint64_t d) {
+    if (n < 2 || T[0].leaf) return false;
+    int cnt = 0, cur = 1;
+    while (cur < n) {
+        const PNode nd = T[cur];
+        const uint64_t cd = pdist(nd, tg);
+        if (cd > d) { cur = nd.escape; continue; }
+        if (nd.leaf) {
+            if (cd == d && ++cnt >= 2) return true;
+            cur = nd.escape;
+        } else {
+            cur++;
+        }
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------------------------------
+// biome of one quart cell (world_biome_cell, biome.c:87-120; mcgpu_biome.cu:54-96) as a per-lane state
+// machine around one program evaluation, so the kernel has a single evaluation site whatever the lanes
+// are doing (first cell, walking back along a tie chain, resolving it forward, an End chunk)
+// ---------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int clamp_qy(const PWorld &W, int qy) {
+    const int lo = W.min_qy, hi = W.min_qy + W.qh - 1;
+    return qy < lo ? lo : (qy > hi ? hi : qy);
+}
+enum { ST_BACK = 0, ST_FWD = 1, ST_END = 2, ST_DONE = 3 };
+struct Cell {
+    int st;                 // ST_*
+    int bxq, bzq, idx0;     // MULTI: chunk corner (quarts) and fill index of the target cell
+    int k, n, leaf;         // walk position, chain length, leaf carried forward
+    int ex, ey, ez;         // END: evaluation point
+    int result;             // biome or -1
+};
+__device__ __forceinline__ void cell_start(const PWorld &W, int qx, int qy, int qz, Cell &c) {
+    qy = clamp_qy(W, qy);
+    c.k = 0; c.n = 0; c.leaf = -1; c.result = -1;
+    if (W.source == MCG_BS_FIXED) { c.st = ST_DONE; c.result = W.fixed_biome; return; }
+    if (W.source == MCG_BS_END) {
+        const int cx = qx >> 2, cz = qz >> 2;
+        if ((int64_t)cx * cx + (int64_t)cz * cz <= 4096LL) { c.st = ST_DONE; c.result = W.end_ids[0]; return; }
+        c.st = ST_END; c.ex = (cx * 2 + 1) * 8; c.ey = qy * 4; c.ez = (cz * 2 + 1) * 8;
+        return;
+    }
+    const int ly = qy - W.min_qy;
+    c.bxq = qx & ~3; c.bzq = qz & ~3;
+    c.idx0 = (ly >> 2) * 64 + (qx & 3) * 16 + (ly & 3) * 4 + (qz & 3);    // fill order of the chunk
+    c.st = ST_BACK;
+}
+// End chunk given directly (the grid's chunk lattice)
+__device__ __forceinline__ void chunk_start(const PWorld &W, int cx, int qy, int cz, Cell &c) {
+    c.k = 0; c.n = 0; c.leaf = -1; c.result = -1;
+    if ((int64_t)cx * cx + (int64_t)cz * cz <= 4096LL) { c.st = ST_DONE; c.result = W.end_ids[0]; return; }
+    c.st = ST_END; c.ex = (cx * 2 + 1) * 8; c.ey = qy * 4; c.ez = (cz * 2 + 1) * 8;
+}
+__device__ __forceinline__ void cell_point(const PWorld &W, const Cell &c, int &x, int &y, int &z) {
+    if (c.st == ST_END) { x = c.ex; y = c.ey; z = c.ez; return; }
+    const int idx = c.idx0 - c.k, s = idx >> 6, r = idx & 63;
+    x = (c.bxq + (r >> 4)) * 4;
+    y = (W.min_qy + s * 4 + ((r >> 2) & 3)) * 4;
+    z = (c.bzq + (r & 3)) * 4;
+}
+// consume the program's values at the point cell_point gave
+template <typename VT>
+__device__ __forceinline__ void cell_step(const PWorld &W, const VT *vals, Cell &c) {
+    if (c.st == ST_END) {                          // TheEndBiomeSource (biome.c:26-45)
+        const double h = (double)vals[W.root_slot[3] * kWG];
+        c.result = h > 0.25 ? W.end_ids[1] : (h >= -0.0625 ? W.end_ids[2] : (h < -0.21875 ? W.end_ids[3] : W.end_ids[4]));
+        c.st = ST_DONE;
+        return;
+    }
+    int64_t tg[6];
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const float f = (float)vals[W.root_slot[q] * kWG] * 10000.0f;
+        if (!(fabsf(f) < 9.0e18f)) ok = false;     // NaN / inf / beyond the exact range of the cast
+        tg[q] = ok ? (int64_t)f : 0;
+    }
+    if (!ok) { c.st = ST_DONE; c.result = -1; return; }
+    const int l0 = rt_find(W.tree, W.ntree, tg);
+    if (l0 < 0) { c.st = ST_DONE; c.result = -1; return; }
+    const uint64_t d = pdist(W.tree[l0], tg);
+    if (c.st == ST_BACK) {
+        const int idx = c.idx0 - c.k;
+        if (idx == 0 || !rt_tie(W.tree, W.ntree, tg, d)) {
+            c.leaf = l0;
+            c.n = c.k;
+            if (c.n == 0) { c.st = ST_DONE; c.result = W.tree[l0].biome; return; }
+            c.st = ST_FWD;
+            c.k = c.n - 1;                         // re-evaluate the tied cells forward
+            return;
+        }
+        if (c.k == kChainCap) { c.st = ST_DONE; c.result = -1; return; }
+        c.k++;
+        return;
+    }
+    // forward: a tied cell keeps the previous cell's leaf when it is at the least distance
+    c.leaf = pdist(W.tree[c.leaf], tg) == d ? c.leaf : l0;
+    if (c.k == 0) { c.st = ST_DONE; c.result = W.tree[c.leaf].biome; return; }
+    c.k--;
+}
+
+__device__ __forceinline__ void count_bad(int32_t *nbad, bool bad) {
+    const uint64_t m = __ballot(bad);
+    if (m && nbad && (int)(threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) atomicAdd(nbad, (int)__popcll(m));
+}
+
+// cells of a lattice, points, or grid points.  Each lane walks its items (stride gridDim*256) and runs
+// one program evaluation per loop trip; a lane whose item finishes starts its next item in the same trip.
+template <bool NEWF, bool PL, int SPD>
+__global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
+    typedef typename VT_of<NEWF>::T VT;
+    extern __shared__ __align__(16) uint8_t lds[];
+    const int pbytes = PL ? ((W.noct * 256 + 15) & ~15) : 0;
+    if (PL) {
+        uint32_t *dst = (uint32_t *)lds;
+        for (int i = threadIdx.x; i < W.noct * 64; i += kWG) dst[i] = ((const uint32_t *)W.octs[i >> 6].p)[i & 63];
+        __syncthreads();
+    }
+    VT *vals = (VT *)(lds + pbytes) + threadIdx.x;
+    const uint8_t *lp = lds;
+    const int64_t stride = (int64_t)gridDim.x * kWG;
+    int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x;
+    Cell c;
+    c.st = ST_DONE;
+    c.result = 0;
+    bool have = false;
+    for (;;) {
+        // finish the current item, start the next one (items whose biome needs no evaluation finish here)
+        while (c.st == ST_DONE) {
+            if (have) {
+                J.out[it] = c.result < 0 ? (uint8_t)255 : (uint8_t)c.result;
+                if (J.nbad && c.result < 0) atomicAdd(J.nbad, 1);
+                it += stride;
+            }
+            have = it < J.n;
+            if (!have) break;
+            if (J.gen == GEN_LATTICE) {
+                const int64_t per = (int64_t)J.nqx * J.nqz;
+                const int layer = (int)(it / per), r = (int)(it % per);
+                const int ax = J.q0x + r % J.nqx, az = J.q0z + r / J.nqx, qy = J.qys[layer];
+                if (W.source == MCG_BS_END) chunk_start(W, ax, qy, az, c);       // a lattice of chunks
+                else cell_start(W, ax, qy, az, c);
+            } else {
+                int x, y, z;
+                if (J.gen == GEN_POINTS) {
+                    x = J.xyz[3 * it]; y = J.xyz[3 * it + 1]; z = J.xyz[3 * it + 2];
+                } else {
+                    x = J.x0 + (int)(it % J.nx) * J.step; y = J.y; z = J.z0 + (int)(it / J.nx) * J.step;
+                }
+                int q[3];
+                dev::zoom_quart(W.zoom_seed, x, y, z, q);
+                cell_start(W, q[0], q[1], q[2], c);
+            }
+        }
+        if (!have) break;
+        int x, y, z;
+        cell_point(W, c, x, y, z);
+        eval_prog<NEWF, PL, SPD>(W, lp, vals, x, y, z);
+        cell_step<VT>(W, vals, c);
+    }
+}
+
+// block grid read from a lattice of cells (MULTI) or chunks (END)
+__global__ __launch_bounds__(kWG) void k_zoom_lattice(PWorld W, PJob L, int x0, int z0, int nx, int64_t n, int step, int y,
+                                                     uint8_t *out, int32_t *nbad) {
+    for (int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x; it - (int64_t)threadIdx.x < n; it += (int64_t)gridDim.x * kWG) {
+        const bool live = it < n;
+        bool bad = false;
+        if (live) {
+            const int x = x0 + (int)(it % nx) * step, z = z0 + (int)(it / nx) * step;
+            int q[3];
+            dev::zoom_quart(W.zoom_seed, x, y, z, q);
+            const int qy = clamp_qy(W, q[1]);
+            const int layer = qy == L.qys[0] ? 0 : 1;
+            int b;
+            if (W.source == MCG_BS_END) {
+                const int cx = q[0] >> 2, cz = q[2] >> 2;
+                if ((int64_t)cx * cx + (int64_t)cz * cz <= 4096LL) b = W.end_ids[0];
+                else b = L.out[((int64_t)layer * L.nqz + (cz - L.q0z)) * L.nqx + (cx - L.q0x)];
+            } else {
+                b = L.out[((int64_t)layer * L.nqz + (q[2] - L.q0z)) * L.nqx + (q[0] - L.q0x)];
+            }
+            bad = b == 255;
+            out[it] = (uint8_t)b;
+        }
+        count_bad(nbad, bad);
+    }
+}
+
+static int grid_blocks(int64_t n) {
+    const int64_t b = (n + kWG - 1) / kWG;
+    return (int)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+template <bool NEWF, bool PL>
+static hipError_t launch_t(const PWorld &W, const PJob &J, int spd, int lds, hipStream_t st) {
+    if (spd <= 6) hipLaunchKernelGGL((k_cells<NEWF, PL, 6>), dim3(grid_blocks(J.n)), dim3(kWG), lds, st, W, J);
+    else hipLaunchKernelGGL((k_cells<NEWF, PL, kSplineDepth>), dim3(grid_blocks(J.n)), dim3(kWG), lds, st, W, J);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st) {
+    if (J.n <= 0) return hipSuccess;
+    if (newf) return perm_lds ? launch_t<true, true>(W, J, W.spl_depth, lds, st) : launch_t<true, false>(W, J, W.spl_depth, lds, st);
+    return perm_lds ? launch_t<false, true>(W, J, W.spl_depth, lds, st) : launch_t<false, false>(W, J, W.spl_depth, lds, st);
+}
+
+hipError_t launch_zoom_lattice(const PWorld &W, const PJob &L, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out,
+                               int32_t *nbad, hipStream_t st) {
+    const int64_t n = (int64_t)nx * nz;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zoom_lattice, dim3(grid_blocks(n)), dim3(kWG), 0, st, W, L, x0, z0, nx, n, step, y, out, nbad);
+    return hipGetLastError();
+}
+
+}  // namespace plug
+}  // namespace mcb

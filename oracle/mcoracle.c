@@ -1272,3 +1272,220 @@ double or_tree_visits(const OrWorld *w, const int32_t *tg6, int n) {
     }
     return n ? (double)total / n : 0.0;
 }
+
+/* ================================================================================================
+ * The world in libmcgen's GPU-plugin form (McgBiomeWorld, libmcgen/gpu/mcgen_gpu_abi.h:1-123) for one
+ * seed: the climate of this file's sampler written as a density-function program, node for node in
+ * the order climate_eval computes it, so its values are climate_eval's bit for bit; octave tables from
+ * seed_init; the R-tree laid out breadth-first with consecutive children as libmcgen's exporter does
+ * (libmcgen/src/gpu_bridge.c:246-274).  The program's shape follows libmcgen's lowering of the
+ * vanilla router (libmcgen/src/df_new.c:405-560: shift_a = CMUL(NOISE(0.25, 0), 4), shift_b =
+ * SHIFT_B, shifted_noise = NOISE_XZ, constant operands folded into CADD / CMUL, caches as pass-through
+ * nodes, blend_alpha / blend_offset as context constants).
+ * ================================================================================================ */
+#include "../include/mcgpu.h"
+
+typedef struct { McgBiomeWorld W; McgProg P; McgTreeNode *tree; } McgOwned;
+
+static McgNode mg_node(int k) {
+    McgNode n; memset(&n, 0, sizeof n);
+    n.k = k; n.a = n.b = n.c = -1; n.ns = -1; n.sp = -1; n.oct = -1;
+    return n;
+}
+static int mg_add(McgProg *P, McgNode n) { P->nodes[P->nnodes] = n; return P->nnodes++; }
+static int mg_un(McgProg *P, int k, int a) { McgNode n = mg_node(k); n.a = a; return mg_add(P, n); }
+static int mg_unf(McgProg *P, int k, int a, float f0, double d0) { McgNode n = mg_node(k); n.a = a; n.f0 = f0; n.d0 = d0; return mg_add(P, n); }
+static int mg_oct(McgProg *P, const Oct *o) {
+    McgOct *m = &P->octs[P->noct];
+    m->xo = o->xo; m->yo = o->yo; m->zo = o->zo; memcpy(m->p, o->p, 256);
+    return P->noct++;
+}
+/* one NormalNoise as a McgNoise: 26.3 stack (first and second octave of a level interleaved, or for the
+ * legacy Nether form first stack then second), 26.1/26.2 two PerlinNoises */
+static int mg_noise(McgProg *P, const Spec *s, const Normal *nn, int fmt_new) {
+    McgNoise *mn = &P->noise[P->nnoise];
+    memset(mn, 0, sizeof *mn);
+    const int n = s->nlev;
+    if (fmt_new) {
+        mn->l0 = P->nlayers; mn->n0 = 2 * n;
+        for (int i = 0; i < 2 * n; i++) {
+            /* interleaved (level k: first stack, second stack) or, legacy, first stack then second */
+            const int stack = s->legacy ? i / n : i & 1, k = s->legacy ? i % n : i >> 1;
+            McgLayer *L = &P->layers[P->nlayers++];
+            memset(L, 0, sizeof *L);
+            L->a = stack ? s->freq[k] * INPUT_FACTOR : s->freq[k];
+            L->amp_f = s->amp_f[k]; L->kind = 0;
+            L->oct = mg_oct(P, &nn->o[stack * n + k]);
+        }
+    } else {
+        for (int stack = 0; stack < 2; stack++) {
+            if (stack == 0) mn->l0 = P->nlayers; else mn->l1 = P->nlayers;
+            for (int k = 0; k < n; k++) {
+                McgLayer *L = &P->layers[P->nlayers++];
+                memset(L, 0, sizeof *L);
+                L->a = s->freq[k]; L->b = s->vf_d[k]; L->amp_d = s->amp_d[k];
+                L->oct = mg_oct(P, &nn->o[stack * n + k]);
+            }
+            if (stack == 0) mn->n0 = n; else mn->n1 = n;
+        }
+        mn->value_factor = s->value_factor;
+    }
+    return P->nnoise++;
+}
+static int mg_spline(McgProg *P, const Spline *S) {
+    for (int id = 0; id < S->nn; id++) {
+        const SplNode *sn = &S->node[id];
+        McgSpline *m = &P->sp[P->nsp++];
+        memset(m, 0, sizeof *m);
+        if (sn->is_const) { m->is_const = 1; m->value = sn->value; continue; }
+        m->coord = sn->coord; m->n = sn->count;
+        m->loc0 = P->nsplf;
+        for (int i = 0; i < sn->count; i++) P->splf[P->nsplf++] = S->loc[sn->first + i];
+        for (int i = 0; i < sn->count; i++) P->splf[P->nsplf++] = S->der[sn->first + i];
+        m->val0 = P->nspc;
+        for (int i = 0; i < sn->count; i++) P->spc[P->nspc++] = S->child[sn->first + i];
+    }
+    return 0;                                            /* the root is node 0 */
+}
+static McgTreeNode *mg_tree(const RTree *T, int *n_out) {
+    int n = T->n;
+    McgTreeNode *out = calloc((size_t)n, sizeof *out);
+    int *order = malloc(sizeof(int) * (size_t)n);
+    int head = 0, tail = 1; order[0] = T->root;
+    while (head < tail) {
+        const RNode *nd = &T->node[order[head]];
+        McgTreeNode *m = &out[head];
+        for (int d = 0; d < 7; d++) { m->lo[d] = (int32_t)nd->lo[d]; m->hi[d] = (int32_t)nd->hi[d]; }
+        m->biome = nd->count == 0 ? nd->biome : -1;
+        if (nd->count == 0) { m->count = 0; m->first = -1; }
+        else {
+            m->count = nd->count; m->first = tail;
+            for (int k = 0; k < nd->count; k++) order[tail++] = T->kid[nd->first + k];
+        }
+        head++;
+    }
+    free(order);
+    *n_out = tail;
+    return out;
+}
+
+McgBiomeWorld *or_mcg_world(const OrWorld *w, int64_t seed) {
+    McgOwned *o = calloc(1, sizeof *o);
+    McgProg *P = &o->P;
+    P->old = !w->fmt_new;
+    P->nodes = calloc(64, sizeof(McgNode)); P->noise = calloc(8, sizeof(McgNoise));
+    P->layers = calloc(8 * 2 * MAXOCT, sizeof(McgLayer)); P->octs = calloc(8 * 2 * MAXOCT + 1, sizeof(McgOct));
+    P->sp = calloc(SPL_NODES, sizeof(McgSpline)); P->splf = calloc(2 * SPL_PTS, sizeof(float));
+    P->spc = calloc(SPL_PTS, sizeof(int)); P->arr = calloc(8, sizeof(int)); P->thr = calloc(1, sizeof(double));
+    SeedState *st = malloc(sizeof *st);
+    seed_init(w, st, seed);
+    McgBiomeWorld *W = &o->W;
+    W->newf = w->fmt_new;
+    W->zoom_seed = or_hashed_seed(seed);
+    for (int i = 0; i < 5; i++) W->end_ids[i] = w->end_ids[i];
+    W->prog = P;
+    const int N = w->fmt_new;
+    int root[6] = { -1, -1, -1, -1, -1, -1 };
+    if (w->dim == OR_OVERWORLD) {
+        W->source = MCG_BS_MULTI; W->min_qy = -16; W->qh = 96;
+        int ns[6];
+        for (int i = 0; i < 6; i++) ns[i] = mg_noise(P, &w->sp[i], &st->n[i], w->fmt_new);
+        int sx, sz, nodes[5];
+        if (N) {
+            McgNode a = mg_node(MCGK_NOISE); a.ns = ns[N_OFFSET]; a.d0 = 0.25; a.d1 = 0.0;
+            sx = mg_unf(P, MCGK_CMUL, mg_add(P, a), 4.0f, 0.0);
+            McgNode b = mg_node(MCGK_SHIFT_B); b.ns = ns[N_OFFSET]; b.d0 = 0.25; b.d1 = 0.25;
+            sz = mg_add(P, b);
+            sx = mg_un(P, MCGK_CACHE, sx); sz = mg_un(P, MCGK_CACHE, sz);
+            for (int i = 0; i < 5; i++) {
+                McgNode c = mg_node(MCGK_NOISE_XZ); c.ns = ns[N_TEMP + i]; c.d0 = 0.25; c.d1 = 0.0; c.a = sx; c.c = sz;
+                nodes[i] = mg_add(P, c);
+            }
+        } else {
+            McgNode a = mg_node(MCGO_SHIFT_A); a.ns = ns[N_OFFSET]; a.d0 = 0.25;
+            McgNode b = mg_node(MCGO_SHIFT_B); b.ns = ns[N_OFFSET]; b.d0 = 0.25;
+            sx = mg_un(P, MCGO_PASS, mg_add(P, a)); sz = mg_un(P, MCGO_PASS, mg_add(P, b));
+            McgNode z = mg_node(MCGO_CONST); z.d0 = 0.0;
+            int zero = mg_add(P, z);
+            for (int i = 0; i < 5; i++) {
+                McgNode c = mg_node(MCGO_SHIFTED_NOISE); c.ns = ns[N_TEMP + i]; c.d0 = 0.25; c.d1 = 0.0; c.a = sx; c.b = zero; c.c = sz;
+                nodes[i] = mg_add(P, c);
+            }
+        }
+        const int T = nodes[0], H = nodes[1], C = nodes[2], E = nodes[3], Wr = nodes[4];
+        /* ridges_folded = (|(|W| + r0)| + r1) * r2 */
+        int rf;
+        if (N) rf = mg_unf(P, MCGK_CMUL, mg_unf(P, MCGK_CADD, mg_un(P, MCGK_ABS, mg_unf(P, MCGK_CADD, mg_un(P, MCGK_ABS, Wr), w->rf_f[0], 0.0)), w->rf_f[1], 0.0), w->rf_f[2], 0.0);
+        else rf = mg_unf(P, MCGO_CMUL, mg_unf(P, MCGO_CADD, mg_un(P, MCGO_ABS, mg_unf(P, MCGO_CADD, mg_un(P, MCGO_ABS, Wr), 0.0f, w->rf_d[0])), 0.0f, w->rf_d[1]), 0.0f, w->rf_d[2]);
+        mg_spline(P, &w->spl);
+        McgNode s = mg_node(N ? MCGK_SPLINE : MCGO_SPLINE); s.sp = 0; s.arr0 = P->narr; s.narr = 4;
+        P->arr[P->narr++] = C; P->arr[P->narr++] = E; P->arr[P->narr++] = Wr; P->arr[P->narr++] = rf;
+        const int spl = mg_add(P, s);
+        int D;
+        if (N) {
+            /* offset = lerp(blend_alpha, blend_offset, offc + spline); depth = y gradient + offset */
+            const int off = mg_unf(P, MCGK_CADD, spl, w->offc_f, 0.0);
+            McgNode al = mg_node(MCGK_CTX_ALPHA); al.f0 = 1.0f;
+            McgNode bo = mg_node(MCGK_CTX_OFFSET); bo.f0 = 0.0f;
+            McgNode le = mg_node(MCGK_LERP); le.a = mg_add(P, al); le.b = mg_add(P, bo); le.c = off;
+            const int blended = mg_un(P, MCGK_CACHE, mg_add(P, le));
+            McgNode g = mg_node(MCGK_GRAD_CLAMP);
+            g.i0 = 1; g.i1 = w->gy0; g.i2 = w->gy0 < w->gy1 ? w->gy0 : w->gy1; g.i3 = w->gy0 < w->gy1 ? w->gy1 : w->gy0;
+            g.f0 = w->gv0_f; g.f1 = (w->gv1_f - w->gv0_f) / (float)(w->gy1 - w->gy0);
+            McgNode ad = mg_node(MCGK_ADD); ad.a = mg_add(P, g); ad.b = blended;
+            D = mg_add(P, ad);
+        } else {
+            /* 0*(1 - alpha) + (offc + spline)*alpha with alpha = 1: engine/mc_climate.h blend order */
+            const int off = mg_unf(P, MCGO_CADD, mg_unf(P, MCGO_CMUL, mg_unf(P, MCGO_CADD, spl, 0.0f, w->offc_d), 0.0f, 1.0), 0.0f, 0.0);
+            McgNode g = mg_node(MCGO_Y_GRADIENT); g.i0 = w->gy0; g.i1 = w->gy1; g.d0 = w->gv0_d; g.d1 = w->gv1_d;
+            McgNode ad = mg_node(MCGO_ADD); ad.a = mg_add(P, g); ad.b = off;
+            D = mg_add(P, ad);
+        }
+        root[0] = T; root[1] = H; root[2] = C; root[3] = E; root[4] = D; root[5] = Wr;
+        o->tree = mg_tree(&w->tree, &W->ntree);
+    } else if (w->dim == OR_NETHER) {
+        W->source = MCG_BS_MULTI; W->min_qy = 0; W->qh = 64;
+        int nt = mg_noise(P, &w->sp[0], &st->n[0], w->fmt_new), nv = mg_noise(P, &w->sp[1], &st->n[1], w->fmt_new);
+        int zero;
+        if (N) {
+            McgNode t = mg_node(MCGK_NOISE); t.ns = nt; t.d0 = 0.25; t.d1 = 0.0; root[0] = mg_add(P, t);
+            McgNode v = mg_node(MCGK_NOISE); v.ns = nv; v.d0 = 0.25; v.d1 = 0.0; root[1] = mg_add(P, v);
+            McgNode z = mg_node(MCGK_CONST); z.f0 = 0.0f; zero = mg_add(P, z);
+        } else {
+            McgNode z = mg_node(MCGO_CONST); z.d0 = 0.0; zero = mg_add(P, z);
+            McgNode t = mg_node(MCGO_SHIFTED_NOISE); t.ns = nt; t.d0 = 0.25; t.d1 = 0.0; t.a = t.b = t.c = zero; root[0] = mg_add(P, t);
+            McgNode v = mg_node(MCGO_SHIFTED_NOISE); v.ns = nv; v.d0 = 0.25; v.d1 = 0.0; v.a = v.b = v.c = zero; root[1] = mg_add(P, v);
+        }
+        for (int k = 2; k < 6; k++) root[k] = zero;
+        o->tree = mg_tree(&w->tree, &W->ntree);
+    } else {
+        W->source = MCG_BS_END; W->min_qy = 0; W->qh = 64;
+        if (N) {
+            /* erosion = max((clamp(100 - |(x, z)|, -100, 80) - 8) / 128, (height - 8) / 128) */
+            McgNode d = mg_node(MCGK_DIST); d.i0 = 0; d.i1 = 0; d.i2 = 0; d.i3 = 0; d.cm = 2; d.cy = 0;
+            int len = mg_add(P, d);
+            int c = mg_unf(P, MCGK_CSUB, len, 100.0f, 0.0);
+            McgNode cl = mg_node(MCGK_CLAMP); cl.a = c; cl.f0 = -100.0f; cl.f1 = 80.0f;
+            c = mg_unf(P, MCGK_CMUL, mg_unf(P, MCGK_CADD, mg_add(P, cl), -8.0f, 0.0), 0.0078125f, 0.0);
+            McgNode e = mg_node(MCGK_END); e.oct = mg_oct(P, &st->end);
+            McgNode mx = mg_node(MCGK_MAX); mx.a = c; mx.b = mg_add(P, e); mx.f0 = INFINITY;
+            root[3] = mg_add(P, mx);
+        } else {
+            McgNode e = mg_node(MCGO_END_ISLANDS); e.oct = mg_oct(P, &st->end);
+            root[3] = mg_add(P, e);
+        }
+    }
+    P->nroots = 6;
+    for (int k = 0; k < 6; k++) P->root[k] = root[k];
+    W->tree = o->tree;
+    W->tree_root = 0;
+    free(st);
+    return W;
+}
+void or_mcg_world_free(McgBiomeWorld *W) {
+    if (!W) return;
+    McgOwned *o = (McgOwned *)W;                     /* W is the first member */
+    McgProg *P = &o->P;
+    free(P->nodes); free(P->noise); free(P->layers); free(P->octs); free(P->sp); free(P->splf); free(P->spc);
+    free(P->arr); free(P->thr); free(o->tree); free(o);
+}

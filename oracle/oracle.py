@@ -68,6 +68,15 @@ def lib():
         L.or_splitmix64_next.argtypes = [C.POINTER(C.c_uint64)]
         L.or_fnv1a_names.restype = C.c_uint64
         L.or_fnv1a_names.argtypes = [u8p, C.c_size_t]
+        L.or_mcg_world.restype = P
+        L.or_mcg_world.argtypes = [P, C.c_int64]
+        L.or_mcg_world_free.argtypes = [P]
+        L.or_mcg_climate.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.or_mcg_cell.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_mcg_chunk_seq.argtypes = [P, C.c_int, C.c_int, i32p, i32p]
+        L.or_mcg_points.argtypes = [P, C.c_int, C.POINTER(C.c_int), u8p, C.POINTER(C.c_int)]
+        L.or_mcg_grid.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p, C.POINTER(C.c_int)]
+        L.or_mcg_quart_history.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p]
         _lib = L
     return _lib
 
@@ -258,3 +267,66 @@ class RefEngine:
         if self.buf is not None:
             RefEngine._L.mcg_host_free(self.buf)
             self.buf = None
+
+
+class McgWorldRef:
+    """libmcgen's biome lookup on a McgBiomeWorld (oracle/mcgplugin.c): the checker of the mcgpu_* entry
+    points.  ``world`` is an address of a McgBiomeWorld; ``McgWorldRef.vanilla`` builds the vanilla world of
+    an Oracle for one seed (its program evaluates to the Oracle's climate bit for bit)."""
+
+    def __init__(self, addr, owned=False, keep=None):
+        self.addr = addr
+        self._owned = owned
+        self._keep = keep
+
+    @classmethod
+    def vanilla(cls, oracle, seed):
+        p = lib().or_mcg_world(oracle.h, int(seed))
+        if not p:
+            raise RuntimeError("or_mcg_world failed")
+        return cls(p, owned=True, keep=oracle)
+
+    def climate(self, bx, by, bz):
+        out = (C.c_double * 6)()
+        lib().or_mcg_climate(self.addr, bx, by, bz, out)
+        return list(out)
+
+    def cell(self, qx, qy, qz):
+        u, ch = C.c_int(0), C.c_int(0)
+        b = lib().or_mcg_cell(self.addr, qx, qy, qz, C.byref(u), C.byref(ch))
+        return b, u.value, ch.value
+
+    def chunk_seq(self, cx, cz, qh):
+        out = np.empty(qh * 16, np.int32)
+        valid = np.empty(qh * 16, np.int32)
+        lib().or_mcg_chunk_seq(self.addr, cx, cz, _p(out, C.c_int32), _p(valid, C.c_int32))
+        return out.reshape(qh, 4, 4), valid.reshape(qh, 4, 4)
+
+    def quart_history(self, qx0, qz0, nx, nz, qy):
+        out = np.empty(nx * nz, np.uint8)
+        rc = lib().or_mcg_quart_history(self.addr, qx0, qz0, nx, nz, qy, _p(out, C.c_uint8))
+        return out, rc == 0
+
+    def points(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
+        out = np.empty(len(xyz), np.uint8)
+        bad = C.c_int(0)
+        lib().or_mcg_points(self.addr, len(xyz), xyz.ctypes.data_as(C.POINTER(C.c_int)), _p(out, C.c_uint8), C.byref(bad))
+        return out, bad.value
+
+    def grid(self, x0, z0, nx, nz, step, y):
+        out = np.empty(nx * nz, np.uint8)
+        bad = C.c_int(0)
+        lib().or_mcg_grid(self.addr, x0, z0, nx, nz, step, y, _p(out, C.c_uint8), C.byref(bad))
+        return out.reshape(nz, nx), bad.value
+
+    def close(self):
+        if self._owned and self.addr:
+            lib().or_mcg_world_free(self.addr)
+        self.addr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,120 @@
+"""The libmcgen GPU plugin ABI on the host (no compute): every symbol include/mcgpu.h declares is exported,
+the ABI version is libmcgen's, the structures have the reference's sizes, and malformed world
+descriptions are refused by mcgpu_biome_new with a reason before anything touches the device
+(the reference plugin would read out of range on the device instead)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import mcgprog
+import mcgpu
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mcgpu.h")
+
+
+def test_exports_every_declared_symbol():
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(mcgpu_[a-z_]+)\s*\(", txt)))
+    assert len(names) == 11
+    L = mcgpu.lib()
+    assert not [n for n in names if not hasattr(L, n)]
+    assert L.mcgpu_abi() == mcgpu.MCGPU_ABI == 2
+
+
+def test_struct_sizes_match_reference_layout():
+    # libmcgen/gpu/mcgen_gpu_abi.h on LP64: field order and types fix these sizes
+    assert C.sizeof(mcgpu.McgOct) == 280
+    assert C.sizeof(mcgpu.McgLayer) == 40
+    assert C.sizeof(mcgpu.McgNoise) == 24
+    assert C.sizeof(mcgpu.McgSpline) == 24
+    assert C.sizeof(mcgpu.McgNode) == 112
+    assert C.sizeof(mcgpu.McgTreeNode) == 72
+    assert C.sizeof(mcgpu.McgBiomeWorld) == 80
+    assert mcgpu.K["FTS"] == 56 and mcgpu.O["PASS"] == 28
+
+
+def _refused(W, text):
+    L = mcgpu.lib()
+    err = C.create_string_buffer(512)
+    h = L.mcgpu_biome_new(C.c_void_p(C.addressof(W)), err, len(err))
+    assert not h
+    assert text in err.value.decode(), err.value.decode()
+
+
+@pytest.fixture(scope="module")
+def van():
+    return mcgprog.vanilla("26.3", "overworld", 1)
+
+
+def _const_prog(P, roots_from=None):
+    c = P.node(mcgpu.K["CONST"], f0=0.25)
+    return [c] * 6
+
+
+def test_refuses_unsupported_kind(van):
+    P = mcgprog.Prog(False, van.prog)
+    c = P.node(mcgpu.K["CONST"], f0=2.0)
+    p = P.node(mcgpu.K["POW"], a=c, b=c)
+    _refused(mcgprog.world(P.build([p] * 6), van.world), "not supported")
+
+
+def test_refuses_forward_reference(van):
+    P = mcgprog.Prog(False, van.prog)
+    a = P.node(mcgpu.K["ABS"], a=1)
+    P.node(mcgpu.K["CONST"], f0=1.0)
+    _refused(mcgprog.world(P.build([a] * 6), van.world), "must be an earlier node")
+
+
+def test_refuses_missing_field(van):
+    P = mcgprog.Prog(False, van.prog)
+    c = P.node(mcgpu.K["CONST"], f0=1.0)
+    _refused(mcgprog.world(P.build([c, c, c, -1, c, c]), van.world), "no climate field 3")
+
+
+def test_refuses_branch_mismatch(van):
+    P = mcgprog.Prog(True, van.prog)
+    c = P.node(mcgpu.O["CONST"], d0=1.0)
+    W = mcgprog.world(P.build([c] * 6), van.world)
+    W.newf = 1
+    _refused(W, "does not match")
+
+
+def test_refuses_deep_spline(van):
+    P = mcgprog.Prog(False, van.prog)
+    x = P.node(mcgpu.K["CONST"], f0=0.5)
+    t = 1.0
+    for _ in range(17):
+        t = (0, [(0.0, 0.0, t), (1.0, 0.0, 2.0)])
+    sp = P.spline(t)
+    s = P.node(mcgpu.K["SPLINE"], sp=sp, arr0=P.list_([x]), narr=1)
+    _refused(mcgprog.world(P.build([s] * 6), van.world), "deeper than 16")
+
+
+def test_refuses_isel_thresholds(van):
+    P = mcgprog.Prog(False, van.prog)
+    x = P.node(mcgpu.K["CONST"], f0=0.5)
+    P.thr += [0.0, 1.0]
+    s = P.node(mcgpu.K["ISEL"], a=x, arr0=P.list_([x, x]), narr=2, thr0=0, nthr=2)
+    _refused(mcgprog.world(P.build([s] * 6), van.world), "more thresholds than choices")
+
+
+def test_refuses_noise_out_of_range(van):
+    P = mcgprog.Prog(False, van.prog)
+    n = P.node(mcgpu.K["NOISE"], ns=3, d0=1.0, d1=1.0)
+    _refused(mcgprog.world(P.build([n] * 6), van.world), "noise index 3 out of range")
+
+
+def test_refuses_tree_cycle(van):
+    P = mcgprog.Prog(False, van.prog)
+    c = P.node(mcgpu.K["CONST"], f0=0.5)
+    T = mcgprog.flat_tree([([0] * 6, [0] * 6, 1), ([1] * 6, [1] * 6, 2)])
+    T[1].first, T[1].count = 0, 1            # a leaf turned into a node whose child is the root
+    _refused(mcgprog.world(P.build([c] * 6), van.world, tree=T), "reached twice")
+
+
+def test_refuses_zero_gradient_period(van):
+    P = mcgprog.Prog(False, van.prog)
+    g = P.node(mcgpu.K["GRAD_REPEAT"], i0=1, i1=0, i2=0, f0=0.0, f1=1.0)
+    _refused(mcgprog.world(P.build([g] * 6), van.world), "gradient period 0")
