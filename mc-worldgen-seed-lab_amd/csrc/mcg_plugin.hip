@@ -661,13 +661,13 @@ __global__ __launch_bounds__(kWG) void k_zoom_lattice(PWorld W, PJob L, int x0, 
             dev::zoom_quart(W.zoom_seed, x, y, z, q);
             const int qy = clamp_qy(W, q[1]);
             const int layer = qy == L.qys[0] ? 0 : 1;
-            int b;
-            if (W.source == MCG_BS_END) {
-                const int cx = q[0] >> 2, cz = q[2] >> 2;
-                if ((int64_t)cx * cx + (int64_t)cz * cz <= 4096LL) b = W.end_ids[0];
-                else b = L.out[((int64_t)layer * L.nqz + (cz - L.q0z)) * L.nqx + (cx - L.q0x)];
-            } else {
-                b = L.out[((int64_t)layer * L.nqz + (q[2] - L.q0z)) * L.nqx + (q[0] - L.q0x)];
+            // the lattice covers every cell (chunk) the zoom can pick; the range check only guards it
+            const int ax = W.source == MCG_BS_END ? q[0] >> 2 : q[0], az = W.source == MCG_BS_END ? q[2] >> 2 : q[2];
+            int b = 255;
+            if (W.source == MCG_BS_END && (int64_t)ax * ax + (int64_t)az * az <= 4096LL) {
+                b = W.end_ids[0];
+            } else if (ax >= L.q0x && ax - L.q0x < L.nqx && az >= L.q0z && az - L.q0z < L.nqz) {
+                b = L.out[((int64_t)layer * L.nqz + (az - L.q0z)) * L.nqx + (ax - L.q0x)];
             }
             bad = b == 255;
             out[it] = (uint8_t)b;
