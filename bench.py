@@ -380,6 +380,43 @@ def bench_endfield(args, torch, mcbiome):
             "checksum": float(np.float64(out.sum())), "cpu_baseline": cpu}
 
 
+def bench_plugin(args, torch, mcbiome):
+    """libmcgen GPU plugin (mcgpu_biome_grid) on the engine's vanilla world for one seed: a 4096x4096
+    block grid at y 64 (step 1) per call, origin from splitmix64(5); the call is synchronous and returns
+    the grid in host memory, as libmcgen's bridge uses it."""
+    import mcgpu
+    d = splitmix64_stream(5, 3)
+    seed, x0, z0 = signed64(d[0]), uniform(d[1], 1 << 22), uniform(d[2], 1 << 22)
+    n, y = 4096, 64
+    eng = mcbiome.Engine("26.3", "overworld")
+    pw = eng.plugin_world(seed)
+    w = mcgpu.BiomeWorld(pw.addr)
+    for _ in range(args.warmup):
+        w.grid(x0, z0, n, n, 1, y)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, bad = w.grid(x0, z0, n, n, 1, y)
+    dt = (time.perf_counter() - t0) / args.steps
+    cpu = None
+    if not args.no_cpu:
+        import ctypes as C
+        import oracle
+        ref = oracle.McgWorldRef.vanilla(oracle.Oracle("26.3", "overworld"), seed)
+        m = 256
+        t1 = time.perf_counter()
+        ref.grid(x0, z0, m, m, 1, y)
+        ct = time.perf_counter() - t1
+        cpu = {"value": round(m * m / ct, 1), "unit": "samples/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": "256x256 blocks of the same grid, the oracle's restatement of libmcgen's biome_at "
+                         "(lazy density functions, per block, OpenMP over blocks)"}
+    return {"metric": "libmcgen plugin block biomes/s (mcgpu_biome_grid, host round trip included)",
+            "value": round(n * n / dt, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "strong",
+            "vs_baseline": None, "dtype": "f32+f64 (26.3 density-function program)", "data": "synthetic seed",
+            "config": {"workload": "plugin: 26.3 Overworld, 4096x4096 blocks at y 64, step 1, one call per step"},
+            "unresolved": int(bad), "cells_per_call": int(2 * ((n + 3) // 4 + 2) ** 2), "cpu_baseline": cpu}
+
+
 def bench_plain(args, torch, mcbiome):
     """One seed (first output of splitmix64(0)), 256x256 quarts at qy 16, origin from the next two draws."""
     d = splitmix64_stream(0, 3)
@@ -470,7 +507,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield", "plugin"])
     ap.add_argument("--search-candidates", type=int, default=1 << 28)
     ap.add_argument("--tile-w", type=int, default=0, help="fixed tile (width | cols_per_lane<<16); 0 = autotune")
     ap.add_argument("--graph", action="store_true", help="replay each plan as a hipGraph")
@@ -498,7 +535,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve,
-          "search": bench_search, "endfield": bench_endfield}[args.workload]
+          "search": bench_search, "endfield": bench_endfield, "plugin": bench_plugin}[args.workload]
     print(json.dumps(fn(args, torch, mcbiome)))
 
 

@@ -208,6 +208,17 @@ int mcb_server_stats(McbServer *s, double *p50_ms, double *p99_ms, double *p999_
 int mcb_server_reset_stats(McbServer *s);
 void mcb_server_destroy(McbServer *s);
 
+/* ================================================================================================
+ * The engine's vanilla world for one seed in libmcgen GPU-plugin form (McgBiomeWorld of
+ * include/mcgpu.h): the climate sampler written as a density-function program in libmcgen's lowering,
+ * the seed's octave tables (computed by the device seed-state kernel), the biome tree breadth-first.
+ * Feeds mcgpu_biome_new without libmcgen.  No counterpart in the reference: libmcgen builds these
+ * descriptions from datapacks (libmcgen/src/gpu_bridge.c:246-315).  Free with mcb_plugin_world_free.
+ * ================================================================================================ */
+struct McgBiomeWorld;
+int mcb_plugin_world(McbEngine *e, int64_t seed, struct McgBiomeWorld **out);
+void mcb_plugin_world_free(struct McgBiomeWorld *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,24 +64,24 @@ int mcb::ensure_ident(McbEngine *e, int n) {
     return 0;
 }
 
-namespace {
-
-// Cell tables depend only on the tree (not on the seed or preset): one build per distinct tree per
-// process, shared by every engine and context world that loads it.
-std::shared_ptr<const CellTable> cells_for_tree(const std::vector<McbNode> &tree, std::string &err) {
+// Cell tables depend only on the tree (not on the seed or preset): one build per distinct tree (and
+// dominance rule) per process, shared by every engine, context world and plugin world that loads it.
+std::shared_ptr<const CellTable> mcb::cells_for_tree(const std::vector<McbNode> &tree, std::string &err, bool strict) {
     static std::mutex mu;
     static std::map<uint64_t, std::shared_ptr<const CellTable>> cache;
-    uint64_t h = 1469598103934665603ULL;
+    uint64_t h = 1469598103934665603ULL ^ (strict ? 0x5bd1e995ULL : 0);
     const uint8_t *p = reinterpret_cast<const uint8_t *>(tree.data());
     for (size_t i = 0; i < tree.size() * sizeof(McbNode); i++) h = (h ^ p[i]) * 1099511628211ULL;
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(h);
     if (it != cache.end()) return it->second;
     auto ct = std::make_shared<CellTable>();
-    if (build_cells(tree, *ct, err) != 0) return nullptr;
+    if (build_cells(tree, *ct, err, strict) != 0) return nullptr;
     cache[h] = ct;
     return ct;
 }
+
+namespace {
 
 // Fills the world's cell-table fields and uploads the entries and lists.
 int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {

@@ -17,6 +17,8 @@
 // (distance, index) minimum of any target in B, so the list keeps the exact result; the same test on
 // a subtree (its node box bounds every leaf below it from below) prunes whole subtrees while the list
 // is collected.  Cells are bounded by +-kCellTMax; a target outside that range walks the tree.
+// The strict variant drops a leaf only when another is nearer everywhere in the cell (tot > 0), so its
+// lists also hold every leaf that can tie at the least distance: ties are decided from the list alone.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -52,12 +54,13 @@ inline int64_t gmin(Iv c, int64_t al, int64_t ah, int64_t sl, int64_t sh) {
     return m;
 }
 
-// Does leaf S dominate node `ni` (and so every leaf of its subtree [ni, esc)) over the cell?
-inline bool dominates(const McbNode &S, int si, const McbNode &N, int ni, int esc, const Iv *B) {
+// Does leaf S dominate node `ni` (and so every leaf of its subtree [ni, esc)) over the cell?  Strict:
+// only when S is nearer everywhere (an equally near leaf may tie, so it stays listed).
+inline bool dominates(const McbNode &S, int si, const McbNode &N, int ni, int esc, const Iv *B, bool strict) {
     if (si >= ni && si < esc) return false;
     int64_t tot = (int64_t)N.d6 - (int64_t)S.d6;
     for (int k = 0; k < 6; k++) tot += gmin(B[k], N.lo[k], N.hi[k], S.lo[k], S.hi[k]);
-    return tot > 0 || (tot == 0 && si < ni);
+    return tot > 0 || (!strict && tot == 0 && si < ni);
 }
 
 // exact (distance, pre-order index) minimum for one target, by walking the pre-order array
@@ -84,6 +87,7 @@ int best_leaf(const std::vector<McbNode> &T, const int64_t *t) {
 struct Builder {
     const std::vector<McbNode> &T;
     const CellTable &ct;
+    bool strict = false;
     int ncell[6];
     std::vector<std::vector<uint16_t>> lists;     // per cell
     std::atomic<int> next{0};
@@ -129,7 +133,7 @@ struct Builder {
             const int esc = nd.escape;
             bool dom = false;
             for (size_t q = 0; q < D.size() && !dom; q++)
-                if (dominates(T[D[q]], D[q], nd, cur, esc, B)) {
+                if (dominates(T[D[q]], D[q], nd, cur, esc, B, strict)) {
                     dom = true;
                     if (q) std::swap(D[q], D[0]);      // the last successful dominator is tried first
                 }
@@ -153,7 +157,7 @@ struct Builder {
             for (int l : C) {
                 bool dom = false;
                 for (size_t q = 0; q < D.size() && !dom; q++)
-                    if (D[q] != l) dom = dominates(T[D[q]], D[q], T[l], l, l + 1, Bd);
+                    if (D[q] != l) dom = dominates(T[D[q]], D[q], T[l], l, l + 1, Bd, strict);
                 if (!dom) C1.push_back(l);
             }
             std::vector<uint16_t> &out = lists[(size_t)c5 * ncell[4] + dc];
@@ -161,7 +165,7 @@ struct Builder {
                 const McbNode &L = T[C1[a]];
                 bool dom = false;
                 for (size_t b = 0; b < C1.size() && !dom; b++)
-                    if (b != a) dom = dominates(T[C1[b]], C1[b], L, C1[a], C1[a] + 1, Bd);
+                    if (b != a) dom = dominates(T[C1[b]], C1[b], L, C1[a], C1[a] + 1, Bd, strict);
                 if (!dom) out.push_back((uint16_t)C1[a]);
             }
         }
@@ -170,7 +174,7 @@ struct Builder {
 
 }  // namespace
 
-int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &err) {
+int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &err, bool strict) {
     const auto t0 = std::chrono::steady_clock::now();
     ct = CellTable();
     if (tree.size() >= 65535) {
@@ -199,6 +203,7 @@ int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &er
         ct.bp.insert(ct.bp.end(), bps[k].begin(), bps[k].end());
     }
     Builder b(tree, ct);
+    b.strict = strict;
     int64_t n5 = 1;
     for (int k : {0, 1, 2, 3, 5}) n5 *= b.ncell[k];
     b.lists.resize((size_t)n5 * b.ncell[4]);

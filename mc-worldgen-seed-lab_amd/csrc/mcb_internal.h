@@ -36,7 +36,10 @@ struct CellTable {
     std::vector<uint16_t> list;
     double build_ms = 0.0;
 };
-int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &err);
+// strict: a leaf is left out of a cell only when another leaf is strictly nearer everywhere in it, so the
+// list also holds every leaf that can tie at the least distance (the plugin's tie test needs them)
+int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &err, bool strict = false);
+std::shared_ptr<const CellTable> cells_for_tree(const std::vector<McbNode> &tree, std::string &err, bool strict = false);
 
 // BiomeManager.obfuscateSeed (SHA-256 of the seed's little-endian bytes, first 8 digest bytes LE)
 int64_t hashed_seed(int64_t seed);

@@ -19,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "mcb_internal.h"
 #include "mcg_plugin.h"
 
 using namespace mcb::plug;
@@ -321,8 +322,41 @@ bool grow(T **p, size_t *cap, size_t need) {
 struct HostWorld {
     Compiled C;
     std::vector<PNode> tree;
+    std::shared_ptr<const mcb::CellTable> cells;   // strict cell table, when the tree admits one
     const McgProg *p = nullptr;
 };
+
+// The cell table is built over the engine's compact node form, so it is used only when that form holds
+// the tree exactly (16-bit bounds, 32-bit offset distance, < 65535 nodes) and the table stays small;
+// otherwise every search walks the tree.  Built once per distinct tree in the process (~1 s for a
+// vanilla Overworld tree).
+std::shared_ptr<const mcb::CellTable> plugin_cells(const std::vector<PNode> &T) {
+    if (T.size() >= 65535) return nullptr;
+    std::vector<McbNode> t16(T.size());
+    std::vector<int32_t> ends[6];
+    for (size_t i = 0; i < T.size(); i++) {
+        const PNode &p = T[i];
+        McbNode &m = t16[i];
+        for (int k = 0; k < 6; k++) {
+            if (p.lo[k] < -32768 || p.lo[k] > 32767 || p.hi[k] < -32768 || p.hi[k] > 32767) return nullptr;
+            m.lo[k] = (int16_t)p.lo[k];
+            m.hi[k] = (int16_t)p.hi[k];
+            if (p.leaf) { ends[k].push_back(p.lo[k]); ends[k].push_back(p.hi[k]); }
+        }
+        if (p.d6 > 0xFFFFFFFFull) return nullptr;
+        m.d6 = (uint32_t)p.d6;
+        m.escape = (uint16_t)p.escape;
+        m.leaf = p.leaf ? (uint16_t)(0x8000 | (p.biome & 0xFF)) : 0;
+    }
+    double cells = 1.0;
+    for (int k = 0; k < 6; k++) {
+        std::sort(ends[k].begin(), ends[k].end());
+        cells *= (double)(std::unique(ends[k].begin(), ends[k].end()) - ends[k].begin()) + 1.0;
+    }
+    if (cells > (double)(1 << 22)) return nullptr;
+    std::string err;
+    return mcb::cells_for_tree(t16, err, true);
+}
 bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
     if (W->source != MCG_BS_MULTI && W->source != MCG_BS_END && W->source != MCG_BS_FIXED) {
         set_error("unknown biome source %d", W->source);
@@ -338,7 +372,10 @@ bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
     if (W->source == MCG_BS_MULTI)
         for (int k = 0; k < 6; k++)
             if (H.C.root_slot[k] < 0) { set_error("no climate field %d", k); return false; }
-    if (W->source == MCG_BS_MULTI && !convert_tree(W->tree, W->ntree, W->tree_root, H.tree)) return false;
+    if (W->source == MCG_BS_MULTI) {
+        if (!convert_tree(W->tree, W->ntree, W->tree_root, H.tree)) return false;
+        H.cells = plugin_cells(H.tree);
+    }
     const int vbytes = H.C.nslot * kWG * (W->newf ? 4 : 8);
     if (vbytes > kMaxLdsBytes) {
         set_error("program keeps %d values alive at once; at most %d fit", H.C.nslot, kMaxLdsBytes / (kWG * (W->newf ? 4 : 8)));
@@ -378,7 +415,10 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
         {p->layers, (size_t)p->nlayers * sizeof(McgLayer), 0},   {p->octs, (size_t)p->noct * sizeof(McgOct), 0},
         {p->sp, (size_t)p->nsp * sizeof(McgSpline), 0},          {p->splf, (size_t)p->nsplf * sizeof(float), 0},
         {p->spc, (size_t)p->nspc * sizeof(int32_t), 0},          {C.arr.data(), C.arr.size() * sizeof(int32_t), 0},
-        {p->thr, (size_t)p->nthr * sizeof(double), 0},           {H.tree.data(), H.tree.size() * sizeof(PNode), 0}};
+        {p->thr, (size_t)p->nthr * sizeof(double), 0},           {H.tree.data(), H.tree.size() * sizeof(PNode), 0},
+        {H.cells ? H.cells->bp.data() : nullptr, H.cells ? H.cells->bp.size() * sizeof(int32_t) : 0, 0},
+        {H.cells ? H.cells->entry.data() : nullptr, H.cells ? H.cells->entry.size() * sizeof(uint32_t) : 0, 0},
+        {H.cells ? H.cells->list.data() : nullptr, H.cells ? H.cells->list.size() * sizeof(uint16_t) : 0, 0}};
     size_t total = 0;
     for (auto &q : parts) { q.off = total; total += (q.bytes + 255) & ~(size_t)255; }
     HCHK(hipMalloc((void **)&g->d_data, std::max<size_t>(total, 256)));
@@ -395,6 +435,12 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     P.arr = (const int32_t *)at(7);
     P.thr = (const double *)at(8);
     P.tree = (const PNode *)at(9);
+    if (H.cells) {
+        P.cbp = (const int32_t *)at(10);
+        P.centry = (const uint32_t *)at(11);
+        P.clist = (const uint16_t *)at(12);
+        for (int k = 0; k < 6; k++) { P.cnb[k] = H.cells->nb[k]; P.coff[k] = H.cells->off[k]; }
+    }
     P.nins = (int32_t)C.ins.size();
     P.noct = p->noct;
     P.ntree = (int32_t)H.tree.size();

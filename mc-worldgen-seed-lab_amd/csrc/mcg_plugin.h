@@ -55,6 +55,12 @@ struct PWorld {
     const int32_t *arr;            // slot numbers (ISEL choices, spline coordinates)
     const double *thr;
     const PNode *tree;
+    // climate cell table over the tree with strict lists (every leaf that can be nearest or tie in the
+    // cell, in pre-order); centry null: every search walks the tree
+    const int32_t *cbp;            // breakpoints of dimension k at cbp + coff[k], cnb[k] of them
+    const uint32_t *centry;        // (first << 8) | count; count 0: walk the tree
+    const uint16_t *clist;
+    int32_t cnb[6], coff[6];
     int32_t nins, noct, ntree, nslot, spl_depth;
     int32_t root_slot[6];          // -1: field absent
     int32_t source, fixed_biome, end_ids[5], min_qy, qh;

@@ -502,6 +502,51 @@ __device__ bool rt_tie(const PNode *__restrict__ T, int n, const int64_t *tg, ui
     return false;
 }
 
+// least-distance search through the cell table: the cell's list holds (in pre-order) every leaf that can
+// be nearest or tie at the least distance there, so the first minimum of the list is the walk's result
+// and the number of leaves at that distance is the tie count.  Within the table's +-2^17 range the
+// distances stay far below 2^63, so no sum wraps.  Returns false when the target is outside the table
+// or its cell is not listed (the caller walks the tree).
+__device__ __forceinline__ bool cell_search(const PWorld &W, const int64_t *tg, int &leaf, uint64_t &d, bool &tied) {
+    if (!W.centry) return false;
+    int c[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int64_t t = tg[k];
+        if (t < -(int64_t)kCellTMax || t > (int64_t)kCellTMax) return false;
+        const int32_t *bp = W.cbp + W.coff[k];
+        int lo = 0, hi = W.cnb[k];                     // cell = number of breakpoints <= t
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((int64_t)bp[mid] <= t) lo = mid + 1; else hi = mid;
+        }
+        c[k] = lo;
+    }
+    // index = (((((cT*nH + cH)*nC + cC)*nE + cE)*nW + cW)*nD + cD)
+    int64_t idx = c[0];
+    idx = idx * (W.cnb[1] + 1) + c[1];
+    idx = idx * (W.cnb[2] + 1) + c[2];
+    idx = idx * (W.cnb[3] + 1) + c[3];
+    idx = idx * (W.cnb[5] + 1) + c[5];
+    idx = idx * (W.cnb[4] + 1) + c[4];
+    const uint32_t e = W.centry[idx];
+    const int cnt = (int)(e & 0xFF);
+    if (!cnt) return false;
+    const uint16_t *L = W.clist + (e >> 8);
+    uint64_t best = ~0ULL;
+    int bl = -1, nt = 0;
+    for (int q = 0; q < cnt; q++) {
+        const int l = L[q];
+        const uint64_t dq = pdist(W.tree[l], tg);
+        if (dq < best) { best = dq; bl = l; nt = 1; }
+        else if (dq == best) nt++;
+    }
+    leaf = bl;
+    d = best;
+    tied = nt >= 2;
+    return true;
+}
+
 // ---------------------------------------------------------------------------------------------------
 // biome of one quart cell (world_biome_cell, biome.c:87-120; mcgpu_biome.cu:54-96) as a per-lane state
 // machine around one program evaluation, so the kernel has a single evaluation site whatever the lanes
@@ -565,12 +610,18 @@ __device__ __forceinline__ void cell_step(const PWorld &W, const VT *vals, Cell 
         tg[q] = ok ? (int64_t)f : 0;
     }
     if (!ok) { c.st = ST_DONE; c.result = -1; return; }
-    const int l0 = rt_find(W.tree, W.ntree, tg);
-    if (l0 < 0) { c.st = ST_DONE; c.result = -1; return; }
-    const uint64_t d = pdist(W.tree[l0], tg);
+    int l0;
+    uint64_t d;
+    bool tied = false;
+    const bool listed = cell_search(W, tg, l0, d, tied);
+    if (!listed) {
+        l0 = rt_find(W.tree, W.ntree, tg);
+        if (l0 < 0) { c.st = ST_DONE; c.result = -1; return; }
+        d = pdist(W.tree[l0], tg);
+    }
     if (c.st == ST_BACK) {
         const int idx = c.idx0 - c.k;
-        if (idx == 0 || !rt_tie(W.tree, W.ntree, tg, d)) {
+        if (idx == 0 || !(listed ? tied : rt_tie(W.tree, W.ntree, tg, d))) {
             c.leaf = l0;
             c.n = c.k;
             if (c.n == 0) { c.st = ST_DONE; c.result = W.tree[l0].biome; return; }

@@ -20,6 +20,27 @@ VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2}
 DIMS = {"overworld": 0, "the_nether": 1, "nether": 1, "the_end": 2, "end": 2}
 
 
+class PluginWorld:
+    """Owner of a McgBiomeWorld made by mcb_plugin_world."""
+
+    def __init__(self, engine, seed):
+        p = C.c_void_p()
+        _check(lib().mcb_plugin_world(engine.h, int(seed), C.byref(p)))
+        self.addr = p.value
+        self._engine = engine
+
+    def close(self):
+        if getattr(self, "addr", None):
+            lib().mcb_plugin_world_free(self.addr)
+            self.addr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class EngineError(RuntimeError):
     pass
 
@@ -81,6 +102,8 @@ def lib():
         L.mcb_search.argtypes = [P, C.POINTER(SeedSource), C.POINTER(Obs), C.c_int, C.c_int, i64p, C.c_uint32,
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_double)]
         L.mcb_end_obs_radius.argtypes = [P, C.POINTER(C.c_uint64)]
+        L.mcb_plugin_world.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
+        L.mcb_plugin_world_free.argtypes = [C.c_void_p]
         L.mcb_end_field.argtypes = [P, i64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.POINTER(C.c_double)]
         L.mcb_points.argtypes = [P, i64p, C.c_int, i32p, C.c_int, u8p, i32p]
@@ -278,6 +301,11 @@ class Engine:
         if r < 0:
             raise EngineError(lib().mcb_last_error().decode())
         return r
+
+    def plugin_world(self, seed):
+        """This world for `seed` in libmcgen GPU-plugin form (a McgBiomeWorld, include/mcgpu.h): pass
+        ``.addr`` to mcgpu_biome_new (mcgpu.BiomeWorld)."""
+        return PluginWorld(self, seed)
 
     def end_field(self, seeds, bx0, bz0, nx, nz, step=1, kind=1):
         """End island field: kind 0 height value at section (bx/8, bz/8), kind 1 erosion at the block;

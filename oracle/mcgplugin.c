@@ -396,7 +396,12 @@ static int end_cell(const McgBiomeWorld *W, int qx, int qy, int qz) {           
     return h < -0.21875 ? W->end_ids[3] : W->end_ids[4];
 }
 
+/* cap >= 0: stop once the chain is longer than cap (the plugin then reports the cell unresolved) */
+static int cell_impl(const McgBiomeWorld *W, int qx, int qy, int qz, int *unres, int *chain, int cap);
 int or_mcg_cell(const McgBiomeWorld *W, int qx, int qy, int qz, int *unres, int *chain) {  /* biome.c:87-120 */
+    return cell_impl(W, qx, qy, qz, unres, chain, -1);
+}
+static int cell_impl(const McgBiomeWorld *W, int qx, int qy, int qz, int *unres, int *chain, int cap) {
     qy = clampq(W, qy);
     if (unres) *unres = 0;
     if (chain) *chain = 0;
@@ -405,10 +410,10 @@ int or_mcg_cell(const McgBiomeWorld *W, int qx, int qy, int qz, int *unres, int 
     const McgTreeNode *T = W->tree;
     int bx = qx & ~3, bz = qz & ~3, ly = qy - W->min_qy;
     int idx = (ly >> 2) * 64 + (qx & 3) * 16 + (ly & 3) * 4 + (qz & 3);
-    const int cap = idx + 1;                       /* the chain stops at fill index 0 at the latest */
-    i64 (*st)[7] = malloc(sizeof(i64[7]) * (size_t)cap);
-    i64 *sd = malloc(sizeof(i64) * (size_t)cap);
-    int *sl = malloc(sizeof(int) * (size_t)cap);
+    const int scap = idx + 1;                      /* the chain stops at fill index 0 at the latest */
+    i64 (*st)[7] = malloc(sizeof(i64[7]) * (size_t)scap);
+    i64 *sd = malloc(sizeof(i64) * (size_t)scap);
+    int *sl = malloc(sizeof(int) * (size_t)scap);
     int n = 0, base = -1, bad = 0;
     for (;;) {
         int s = idx >> 6, r = idx & 63;
@@ -419,6 +424,7 @@ int or_mcg_cell(const McgBiomeWorld *W, int qx, int qy, int qz, int *unres, int 
         i64 d = ndist(&T[l0], t);
         if (idx == 0 || rt_count_min(T, W->tree_root, t, d) < 2) { base = l0; break; }
         memcpy(st[n], t, sizeof t); sd[n] = d; sl[n] = l0; n++; idx--;
+        if (cap >= 0 && n > cap) { bad = 1; break; }
     }
     int leaf = bad ? -1 : base;
     for (int i = n - 1; i >= 0 && leaf >= 0; i--) leaf = ndist(&T[leaf], st[i]) == sd[i] ? leaf : sl[i];
@@ -452,7 +458,7 @@ int or_mcg_points(const McgBiomeWorld *W, int n, const int *xyz, uint8_t *out, i
         int32_t q[3];
         or_zoom(W->zoom_seed, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], q);
         int u = 0;
-        int b = or_mcg_cell(W, q[0], q[1], q[2], &u, NULL);
+        int b = cell_impl(W, q[0], q[1], q[2], &u, NULL, 12);
         if (u || b < 0) { out[i] = 255; bad++; } else out[i] = (uint8_t)b;
     }
     if (nunres) *nunres = bad;
@@ -466,7 +472,7 @@ int or_mcg_grid(const McgBiomeWorld *W, int x0, int z0, int nx, int nz, int step
         int32_t q[3];
         or_zoom(W->zoom_seed, x0 + (i % nx) * step, y, z0 + (i / nx) * step, q);
         int u = 0;
-        int b = or_mcg_cell(W, q[0], q[1], q[2], &u, NULL);
+        int b = cell_impl(W, q[0], q[1], q[2], &u, NULL, 12);
         if (u || b < 0) { out[i] = 255; bad++; } else out[i] = (uint8_t)b;
     }
     if (nunres) *nunres = bad;

@@ -277,10 +277,12 @@ def test_node_kinds_exact(builder, version, old):
 # ---------------------------------------------------------------------------------------------------
 # ties, NaN, fixed source
 # ---------------------------------------------------------------------------------------------------
-@pytest.mark.parametrize("shift", [1500, 6000])
+@pytest.mark.parametrize("shift", [1500, 6000, "snapped"])
 def test_overlapping_leaves_chain(shift):
-    from test_oracle_plugin import overlap_world
-    V, W = overlap_world("26.3", 8, shift)
+    """Walk path (copies add breakpoints) and cell-table path (snapped copies reuse them)."""
+    from test_oracle_plugin import overlap_world, overlap_world_snapped
+    V, W = overlap_world_snapped("26.1", 8) if shift == "snapped" else overlap_world("26.3", 8, shift)
+    shift = 7 if shift == "snapped" else shift
     ref = oracle.McgWorldRef(C.addressof(W), keep=W)
     rng = np.random.default_rng(shift)
     xyz = _points(rng, 800, 200_000, -64, 320)
@@ -300,3 +302,61 @@ def test_fixed_source():
     W.fixed_biome = 41
     g, bad = mcgpu.BiomeWorld(W).grid(-50, 70, 33, 17, 5, 12)
     assert bad == 0 and (g == 41).all()
+
+
+# ---------------------------------------------------------------------------------------------------
+# the engine's exporter (mcb_plugin_world) writes the oracle's plugin-form world, array for array
+# ---------------------------------------------------------------------------------------------------
+def _fields(s):
+    return [(n, getattr(s, n)) for n, _ in s._fields_]
+
+
+def _eq_struct(a, b, skip=()):
+    for (n, x), (_, y) in zip(_fields(a), _fields(b)):
+        if n in skip:
+            continue
+        if hasattr(x, "_length_"):
+            assert list(x) == list(y), n
+        else:
+            assert x == y or (x != x and y != y), n
+
+
+@pytest.mark.parametrize("version,dim", CASES)
+def test_export_equals_oracle_world(version, dim):
+    import mcbiome
+    for seed in (0, -4242424242):
+        V = mcgprog.vanilla(version, dim, seed)
+        e = mcbiome.Engine(version, dim)
+        pw = e.plugin_world(seed)
+        W = C.cast(C.c_void_p(pw.addr), C.POINTER(mcgpu.McgBiomeWorld)).contents
+        R = V.world
+        for n in ("source", "newf", "min_qy", "qh", "zoom_seed", "ntree", "tree_root"):
+            assert getattr(W, n) == getattr(R, n), n
+        assert list(W.end_ids) == list(R.end_ids)
+        P, Q = W.prog.contents, R.prog.contents
+        for n in ("old", "nnodes", "nnoise", "nlayers", "noct", "nsp", "nsplf", "nspc", "narr", "nroots"):
+            assert getattr(P, n) == getattr(Q, n), n
+        assert list(P.root)[:6] == list(Q.root)[:6]
+        for arr, cnt in (("nodes", "nnodes"), ("noise", "nnoise"), ("layers", "nlayers"), ("sp", "nsp")):
+            for i in range(getattr(P, cnt)):
+                _eq_struct(getattr(P, arr)[i], getattr(Q, arr)[i])
+        for i in range(P.noct):
+            a, b = P.octs[i], Q.octs[i]
+            assert (a.xo, a.yo, a.zo) == (b.xo, b.yo, b.zo) and bytes(a.p) == bytes(b.p), i
+        assert [P.splf[i] for i in range(P.nsplf)] == [Q.splf[i] for i in range(Q.nsplf)]
+        assert [P.spc[i] for i in range(P.nspc)] == [Q.spc[i] for i in range(Q.nspc)]
+        assert [P.arr[i] for i in range(P.narr)] == [Q.arr[i] for i in range(Q.narr)]
+        for i in range(W.ntree):
+            a, b = W.tree[i], R.tree[i]
+            _eq_struct(a, b, skip=("lo", "hi"))
+            assert list(a.lo)[:6] == list(b.lo)[:6] and list(a.hi)[:6] == list(b.hi)[:6], i
+            # offset dimension: the same squared distance to a target offset of 0
+            d = max(-b.hi[6], b.lo[6], 0) if -b.hi[6] <= 0 else -b.hi[6]
+            assert a.lo[6] == a.hi[6] == d, i
+        # and the plugin gives the same biomes from either description
+        rng = np.random.default_rng(seed & 0xFFFF)
+        lo, hi = HEIGHT[dim]
+        xyz = _points(rng, 500, 1_000_000, lo, hi)
+        g1, _ = mcgpu.BiomeWorld(pw.addr).points(xyz)
+        g2, _ = mcgpu.BiomeWorld(V.world).points(xyz)
+        assert np.array_equal(g1, g2)
