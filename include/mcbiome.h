@@ -219,6 +219,13 @@ struct McgBiomeWorld;
 int mcb_plugin_world(McbEngine *e, int64_t seed, struct McgBiomeWorld **out);
 void mcb_plugin_world_free(struct McgBiomeWorld *w);
 
+/* Whole chunks as libmcgen stores them (world_chunk_biomes, libmcgen/src/biome.c:125-157): for each of n
+ * chunks (cx, cz) in cxz, qh*16 bytes out[c*qh*16 + (qy*4 + qz)*4 + qx], qy counted from the world's lowest
+ * quart (Overworld: 96 layers from quart -16; Nether, End: 64 from 0).  Each chunk's fitness ties are
+ * resolved along its fill order (sections bottom-up, then x, y, z) with the search history starting at
+ * the chunk's first cell; the End per cell. */
+int mcb_chunk_biomes(McbEngine *e, int64_t seed, int n, const int32_t *cxz, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

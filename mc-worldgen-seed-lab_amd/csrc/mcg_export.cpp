@@ -16,6 +16,7 @@
 
 #include "../../include/mcgpu.h"
 #include "mcb_internal.h"
+#include "mcg_plugin.h"
 
 using namespace mcb;
 
@@ -296,5 +297,24 @@ int mcb_plugin_world(McbEngine *e, int64_t seed, McgBiomeWorld **out) {
 }
 
 void mcb_plugin_world_free(McgBiomeWorld *W) { delete reinterpret_cast<Owned *>(W); }
+
+// Whole chunks as libmcgen stores them (world_chunk_biomes, libmcgen/src/biome.c:125-157), through the
+// plugin kernels on the engine's vanilla world: per cell climate + search, ties resolved along the chunk's
+// fill order from its first cell.  The 26.3 chunk path samples the unshifted noises by volume, which sums
+// each octave's gradient terms as (x + z) + y instead of (x + y) + z and scales coordinates as
+// block * (0.25 * frequency); every gradient vector has a zero component and 0.25 is a power of two, so
+// both give the point path's values bit for bit and the point evaluation serves here.
+int mcb_chunk_biomes(McbEngine *e, int64_t seed, int n, const int32_t *cxz, uint8_t *out) {
+    if (!e || n < 0 || (n && (!cxz || !out))) return fail(MCB_E_ARG, "bad arguments");
+    McgBiomeWorld *W = nullptr;
+    if (int rc = mcb_plugin_world(e, seed, &W)) return rc;
+    char err[512] = {0};
+    int bad = 0;
+    const int r = mcb::plug::chunk_biomes(W, n, cxz, out, &bad, err, sizeof err);
+    mcb_plugin_world_free(W);
+    if (r) return fail(MCB_E_DEVICE, "%s", err);
+    if (bad) return fail(MCB_E_DATA, "%d cells have a non-finite climate", bad);
+    return MCB_OK;
+}
 
 }  // extern "C"

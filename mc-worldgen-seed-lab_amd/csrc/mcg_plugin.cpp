@@ -395,6 +395,7 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     P.min_qy = W->min_qy;
     P.qh = W->qh;
     P.zoom_seed = W->zoom_seed;
+    P.chain_cap = kChainCap;
     for (int k = 0; k < 6; k++) P.root_slot[k] = -1;
     g->source = W->source;
     g->newf = W->newf != 0;
@@ -446,6 +447,7 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     P.ntree = (int32_t)H.tree.size();
     P.nslot = C.nslot;
     P.spl_depth = C.spl_depth;
+    P.chain_cap = kChainCap;
     for (int k = 0; k < 6; k++) P.root_slot[k] = C.root_slot[k];
     return true;
 }
@@ -518,7 +520,47 @@ bool run_points(GBiome *g, int n, const int *xyz, unsigned char *out, int *nunre
     return true;
 }
 
+bool run_chunks(GBiome *g, int n, const int32_t *cxz, uint8_t *out, int *nunres) {
+    const int cells = g->W.qh * 16;
+    const int64_t items = (int64_t)n * cells;
+    if (!grow(&g->d_out, &g->out_cap, (size_t)items) || !grow(&g->d_xyz, &g->xyz_cap, (size_t)n * 2)) return false;
+    HCHK(hipMemsetAsync(g->d_bad, 0, sizeof(int32_t), g->st));
+    HCHK(hipMemcpyAsync(g->d_xyz, cxz, (size_t)n * 2 * sizeof(int32_t), hipMemcpyHostToDevice, g->st));
+    PJob J;
+    memset(&J, 0, sizeof J);
+    J.gen = GEN_CHUNKS;
+    J.n = items;
+    J.xyz = g->d_xyz;
+    J.cells = cells;
+    J.out = g->d_out;
+    J.nbad = g->d_bad;
+    HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    int32_t bad = 0;
+    HCHK(hipMemcpyAsync(out, g->d_out, (size_t)items, hipMemcpyDeviceToHost, g->st));
+    HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
+    HCHK(hipStreamSynchronize(g->st));
+    if (nunres) *nunres = bad;
+    return true;
+}
+
 }  // namespace
+
+int mcb::plug::chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, uint8_t *out, int *nunres, char *err,
+                            size_t errlen) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    bool ok = false;
+    HostWorld H;
+    std::unique_ptr<GBiome> g(new GBiome());
+    if (n < 0 || (n && (!cxz || !out))) set_error("bad arguments");
+    else if (prepare_world(W, H) && ensure_init() && build_world(g.get(), W, H)) {
+        g->W.chain_cap = 1 << 30;            // a chunk's fill order is finite: the walk ends at its first cell
+        ok = n == 0 || run_chunks(g.get(), n, cxz, out, nunres);
+        if (n == 0 && nunres) *nunres = 0;
+    }
+    g->release();
+    if (!ok && err && errlen) snprintf(err, errlen, "%s", g_err);
+    return ok ? 0 : 1;
+}
 
 extern "C" {
 

@@ -62,13 +62,14 @@ struct PWorld {
     const uint16_t *clist;
     int32_t cnb[6], coff[6];
     int32_t nins, noct, ntree, nslot, spl_depth;
+    int32_t chain_cap;             // tied cells followed back before a cell is reported unresolved
     int32_t root_slot[6];          // -1: field absent
     int32_t source, fixed_biome, end_ids[5], min_qy, qh;
     int64_t zoom_seed;
 };
 
 // work description of one launch
-enum { GEN_LATTICE = 0, GEN_POINTS = 1, GEN_GRIDPTS = 2 };
+enum { GEN_LATTICE = 0, GEN_POINTS = 1, GEN_GRIDPTS = 2, GEN_CHUNKS = 3 };
 struct PJob {
     int32_t gen;
     int64_t n;                     // items
@@ -77,10 +78,16 @@ struct PJob {
     int32_t q0x, q0z, nqx, nqz, qys[2];
     // grid points: block (x0 + ix * step, y, z0 + iz * step)
     int32_t x0, z0, nx, step, y;
-    const int32_t *xyz;            // points: 3 ints per item
+    const int32_t *xyz;            // points: 3 ints per item; chunks: (cx, cz) per chunk
+    int32_t cells;                 // chunks: quart cells per chunk (16 per quart layer), item = chunk * cells + cell
     uint8_t *out;
     int32_t *nbad;
 };
+
+// Whole chunks of a world description in libmcgen's stored form (world_chunk_biomes,
+// libmcgen/src/biome.c:125-157): out[c*cells + (qy*4 + qz)*4 + qx], each chunk's ties resolved along its
+// fill order from the chunk's first cell with no length cap.  0 = ok, else err is set.
+int chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, uint8_t *out, int *nunres, char *err, size_t errlen);
 
 // launchers (mcg_plugin.hip); lds = bytes of dynamic LDS, perm_lds = permutation tables staged in LDS
 hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st);

@@ -629,7 +629,7 @@ __device__ __forceinline__ void cell_step(const PWorld &W, const VT *vals, Cell 
             c.k = c.n - 1;                         // re-evaluate the tied cells forward
             return;
         }
-        if (c.k == kChainCap) { c.st = ST_DONE; c.result = -1; return; }
+        if (c.k == W.chain_cap) { c.st = ST_DONE; c.result = -1; return; }
         c.k++;
         return;
     }
@@ -674,7 +674,10 @@ __global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
             }
             have = it < J.n;
             if (!have) break;
-            if (J.gen == GEN_LATTICE) {
+            if (J.gen == GEN_CHUNKS) {
+                const int ch = (int)(it / J.cells), r = (int)(it % J.cells);
+                cell_start(W, J.xyz[2 * ch] * 4 + (r & 3), W.min_qy + (r >> 4), J.xyz[2 * ch + 1] * 4 + ((r >> 2) & 3), c);
+            } else if (J.gen == GEN_LATTICE) {
                 const int64_t per = (int64_t)J.nqx * J.nqz;
                 const int layer = (int)(it / per), r = (int)(it % per);
                 const int ax = J.q0x + r % J.nqx, az = J.q0z + r / J.nqx, qy = J.qys[layer];

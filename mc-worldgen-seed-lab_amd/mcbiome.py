@@ -104,6 +104,7 @@ def lib():
         L.mcb_end_obs_radius.argtypes = [P, C.POINTER(C.c_uint64)]
         L.mcb_plugin_world.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
         L.mcb_plugin_world_free.argtypes = [C.c_void_p]
+        L.mcb_chunk_biomes.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
         L.mcb_end_field.argtypes = [P, i64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.POINTER(C.c_double)]
         L.mcb_points.argtypes = [P, i64p, C.c_int, i32p, C.c_int, u8p, i32p]
@@ -301,6 +302,14 @@ class Engine:
         if r < 0:
             raise EngineError(lib().mcb_last_error().decode())
         return r
+
+    def chunk_biomes(self, seed, chunks):
+        """Whole chunks (cx, cz) as libmcgen stores them: (n, qh, 4, 4) uint8 indexed [chunk, qy, qz, qx]."""
+        cxz = np.ascontiguousarray(chunks, dtype=np.int32).reshape(-1, 2)
+        qh = 96 if DIMS[self.dim] == 0 else 64
+        out = np.empty(len(cxz) * qh * 16, np.uint8)
+        _check(lib().mcb_chunk_biomes(self.h, int(seed), len(cxz), _p(cxz, C.c_int32), _p(out, C.c_uint8)))
+        return out.reshape(len(cxz), qh, 4, 4)
 
     def plugin_world(self, seed):
         """This world for `seed` in libmcgen GPU-plugin form (a McgBiomeWorld, include/mcgpu.h): pass

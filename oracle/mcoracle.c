@@ -488,6 +488,42 @@ static float normal_f(const Normal *nn, const Spec *s, double x, double y, doubl
     return acc;
 }
 
+/* Volume order of a 26.3 octave (libmcgen/src/noise.c:142-190, PerlinNoise.addToVolume at one point):
+ * the x and z gradient terms are summed first and the y term added to that, where the point path sums
+ * x, y, z in order; coordinates are block * (scale * frequency).  Non-smeared octaves only. */
+static float oct_eval_f_vol(const Oct *o, double X, double Y, double Z) {
+    double x = wrap_new(X) + o->xo, y = wrap_new(Y) + o->yo, z = wrap_new(Z) + o->zo;
+    i32 xi = ifloor(x), yi = ifloor(y), zi = ifloor(z);
+    float fx = (float)(x - xi), fy = (float)(y - yi), fz = (float)(z - zi);
+    int a = PERM(o, xi), b = PERM(o, xi + 1);
+    int aa = PERM(o, a + yi), ab = PERM(o, a + yi + 1), ba = PERM(o, b + yi), bb = PERM(o, b + yi + 1);
+    const int hs[8] = { PERM(o, aa + zi), PERM(o, ba + zi), PERM(o, ab + zi), PERM(o, bb + zi),
+                        PERM(o, aa + zi + 1), PERM(o, ba + zi + 1), PERM(o, ab + zi + 1), PERM(o, bb + zi + 1) };
+    float c[8];
+    for (int k = 0; k < 8; k++) {
+        const signed char *g = GRAD3[hs[k] & 15];
+        float xr = (k & 1) ? fx - 1.0f : fx, yr = (k & 2) ? fy - 1.0f : fy, zr = (k & 4) ? fz - 1.0f : fz;
+        float dxz = (float)g[0] * xr + (float)g[2] * zr;
+        c[k] = dxz + (float)g[1] * yr;
+    }
+    float u = fade_f(fx), v = fade_f(fy), w = fade_f(fz);
+    float lo = lerp_f(v, lerp_f(u, c[0], c[1]), lerp_f(u, c[2], c[3]));
+    float hi = lerp_f(v, lerp_f(u, c[4], c[5]), lerp_f(u, c[6], c[7]));
+    return lerp_f(w, lo, hi);
+}
+/* NoiseStack.addToVolume at one block point (noise.c:359-366): layer by layer, amplitude amp * layer
+ * amplitude, coordinates bx * (sxz * f), by * (sy * f), bz * (sxz * f) */
+static float normal_f_vol(const Normal *nn, const Spec *s, int bx, int by, int bz, double sxz, double sy, float amp) {
+    int n = s->nlev;
+    float acc = 0.0f;
+    for (int i = 0; i < 2 * n; i++) {
+        const int stack = s->legacy ? i / n : (i & 1), k = s->legacy ? i % n : (i >> 1);
+        const double f = stack ? s->freq[k] * INPUT_FACTOR : s->freq[k];
+        acc += (amp * s->amp_f[k]) * oct_eval_f_vol(&nn->o[stack * n + k], bx * (sxz * f), by * (sy * f), bz * (sxz * f));
+    }
+    return acc;
+}
+
 /* ================================================================================================
  * Cubic spline (float).  engine/mc_climate.h:21-66, parser engine/mc_data.h:50-68
  * ================================================================================================ */
@@ -962,7 +998,13 @@ static void seed_init(const OrWorld *w, SeedState *st, i64 seed) {
 
 /* Climate.Sampler at a block position: engine/mc_climate.h:113-164.  out: T,H,C,E,D,W (float) and
  * df7: C,E,W,offset,depth,T,H as the density functions' own type widened to double. */
+/* vol = 1: the chunk (volume) path of 26.3 (libmcgen/src/df_new.c:916-958 via biome.c:125-141): the shift
+ * noises (shift_a's NOISE, SHIFT_B) go through the volume order; every other node is evaluated per point */
+static void climate_eval_m(const OrWorld *w, const SeedState *st, int bx, int by, int bz, float out[6], double df7[7], int vol);
 static void climate_eval(const OrWorld *w, const SeedState *st, int bx, int by, int bz, float out[6], double df7[7]) {
+    climate_eval_m(w, st, bx, by, bz, out, df7, 0);
+}
+static void climate_eval_m(const OrWorld *w, const SeedState *st, int bx, int by, int bz, float out[6], double df7[7], int vol) {
     if (!w->fmt_new) {
         double sx = normal_d(&st->n[N_OFFSET], &w->sp[N_OFFSET], bx * 0.25, 0.0 * 0.25, bz * 0.25) * 4.0;
         double sz = normal_d(&st->n[N_OFFSET], &w->sp[N_OFFSET], bz * 0.25, bx * 0.25, 0.0 * 0.25) * 4.0;
@@ -982,8 +1024,10 @@ static void climate_eval(const OrWorld *w, const SeedState *st, int bx, int by, 
         if (out) { out[0] = (float)T; out[1] = (float)H; out[2] = (float)C; out[3] = (float)E; out[4] = (float)D; out[5] = (float)W; }
         if (df7) { df7[0] = C; df7[1] = E; df7[2] = W; df7[3] = off; df7[4] = D; df7[5] = T; df7[6] = H; }
     } else {
-        float sx = normal_f(&st->n[N_OFFSET], &w->sp[N_OFFSET], bx * 0.25, 0.0 * 0.25, bz * 0.25) * 4.0f;
-        float sz = normal_f(&st->n[N_OFFSET], &w->sp[N_OFFSET], bz * 0.25, bx * 0.25, 0.0 * 0.25) * 4.0f;
+        float sx = vol ? normal_f_vol(&st->n[N_OFFSET], &w->sp[N_OFFSET], bx, by, bz, 0.25, 0.0, 1.0f) * 4.0f
+                       : normal_f(&st->n[N_OFFSET], &w->sp[N_OFFSET], bx * 0.25, 0.0 * 0.25, bz * 0.25) * 4.0f;
+        float sz = vol ? normal_f_vol(&st->n[N_OFFSET], &w->sp[N_OFFSET], bz, bx, 0, 0.25, 0.25, 4.0f)
+                       : normal_f(&st->n[N_OFFSET], &w->sp[N_OFFSET], bz * 0.25, bx * 0.25, 0.0 * 0.25) * 4.0f;
         double x = bx * 0.25 + (double)sx, z = bz * 0.25 + (double)sz, y = (double)by * 0.0;
         float T = normal_f(&st->n[N_TEMP], &w->sp[N_TEMP], x, y, z);
         float H = normal_f(&st->n[N_VEG], &w->sp[N_VEG], x, y, z);
@@ -1006,17 +1050,24 @@ static void climate_eval(const OrWorld *w, const SeedState *st, int bx, int by, 
 static inline i64 quantize(float v) { return (i64)(v * 10000.0f); }   /* engine/mc_climate.h:93 */
 
 /* target point of a quart sample; returns 0 for End (no tree) */
+static int sample_target_m(const OrWorld *w, const SeedState *st, int qx, int qy, int qz, i64 t[RT_DIM], int vol);
 static int sample_target(const OrWorld *w, const SeedState *st, int qx, int qy, int qz, i64 t[RT_DIM]) {
+    return sample_target_m(w, st, qx, qy, qz, t, 0);
+}
+static int sample_target_m(const OrWorld *w, const SeedState *st, int qx, int qy, int qz, i64 t[RT_DIM], int vol) {
     int bx = qx * 4, by = qy * 4, bz = qz * 4;
     if (w->dim == OR_OVERWORLD) {
-        float v[6]; climate_eval(w, st, bx, by, bz, v, NULL);
+        float v[6]; climate_eval_m(w, st, bx, by, bz, v, NULL, vol && w->fmt_new);
         for (int k = 0; k < 6; k++) t[k] = quantize(v[k]);
         t[6] = 0;
         return 1;
     }
     if (w->dim == OR_NETHER) {                                /* engine/cuda/mcg_host.c:63-74 */
         float T, H;
-        if (w->fmt_new) {
+        if (w->fmt_new && vol) {                             /* unshifted NOISE nodes: volume order */
+            T = normal_f_vol(&st->n[0], &w->sp[0], bx, by, bz, 0.25, 0.0, 1.0f);
+            H = normal_f_vol(&st->n[1], &w->sp[1], bx, by, bz, 0.25, 0.0, 1.0f);
+        } else if (w->fmt_new) {
             T = normal_f(&st->n[0], &w->sp[0], bx * 0.25, by * 0.0, bz * 0.25);
             H = normal_f(&st->n[1], &w->sp[1], bx * 0.25, by * 0.0, bz * 0.25);
         } else {
@@ -1063,7 +1114,44 @@ void or_points(const OrWorld *w, const int64_t *seeds, int ns, const int32_t *pt
     }
 }
 
+/* semantics 2: the Java oracle's chunk grids (oracle/src/v263/oracle/compat/Compat.java:169-224): climate
+ * through the chunk (volume) path, searched chunk by chunk (chunk rows, then chunk columns; inside a chunk
+ * quart rows, then columns) with the search history carried across the whole grid */
+static void grid_chunk(const OrWorld *w, int64_t seed, int qx0, int qz0, int nx, int nz, int qy, uint8_t *out) {
+    const size_t n = (size_t)nx * nz;
+    i64 (*tg)[RT_DIM] = malloc(sizeof(i64) * RT_DIM * n);
+    int *leaf0 = malloc(sizeof(int) * n);
+#pragma omp parallel
+    {
+        SeedState *st = malloc(sizeof *st);
+        seed_init(w, st, seed);
+#pragma omp for schedule(dynamic, 1)
+        for (int iz = 0; iz < nz; iz++)
+            for (int ix = 0; ix < nx; ix++) {
+                size_t i = (size_t)iz * nx + ix;
+                sample_target_m(w, st, qx0 + ix, qy, qz0 + iz, tg[i], 1);
+                leaf0[i] = rt_search(&w->tree, w->tree.root, tg[i], -1);
+            }
+        free(st);
+    }
+    const int tx0 = qx0 >> 2, tx1 = (qx0 + nx - 1) >> 2, tz0 = qz0 >> 2, tz1 = (qz0 + nz - 1) >> 2;
+    int last = -1;
+    for (int tz = tz0; tz <= tz1; tz++)
+        for (int tx = tx0; tx <= tx1; tx++)
+            for (int qz = tz * 4; qz <= tz * 4 + 3; qz++)
+                for (int qx = tx * 4; qx <= tx * 4 + 3; qx++) {
+                    if (qz < qz0 || qz >= qz0 + nz || qx < qx0 || qx >= qx0 + nx) continue;
+                    size_t i = (size_t)(qz - qz0) * nx + (qx - qx0);
+                    int l0 = leaf0[i];
+                    if (last >= 0 && node_dist(&w->tree.node[last], tg[i]) == node_dist(&w->tree.node[l0], tg[i])) l0 = last;
+                    last = l0;
+                    out[i] = (u8)w->tree.node[l0].biome;
+                }
+    free(tg); free(leaf0);
+}
+
 void or_grid(const OrWorld *w, int64_t seed, int qx0, int qz0, int nx, int nz, int qy, int semantics, uint8_t *out) {
+    if (semantics == 2 && w->dim != OR_END) { grid_chunk(w, seed, qx0, qz0, nx, nz, qy, out); return; }
     int hist = semantics == 1 && w->dim != OR_END;
     int *leaf0 = hist ? malloc(sizeof(int) * (size_t)nx * nz) : NULL;
     i64 (*tg)[RT_DIM] = hist ? malloc(sizeof(i64) * RT_DIM * (size_t)nx * nz) : NULL;
@@ -1488,4 +1576,40 @@ void or_mcg_world_free(McgBiomeWorld *W) {
     McgProg *P = &o->P;
     free(P->nodes); free(P->noise); free(P->layers); free(P->octs); free(P->sp); free(P->splf); free(P->spc);
     free(P->arr); free(P->thr); free(o->tree); free(o);
+}
+
+
+/* Biomes of whole chunks as libmcgen stores them (world_chunk_biomes, libmcgen/src/biome.c:125-157):
+ * 26.3 multi-noise through the chunk (volume) path, 26.1/26.2 per point, each chunk searched in its fill
+ * order (sections bottom-up, then x, y, z) with the history starting fresh; the End per cell.  out per chunk:
+ * qh*16 bytes, [(qy*4 + qz)*4 + qx], qy from the world's lowest quart (Overworld -16, 96 high; 0 and 64). */
+void or_chunk_biomes(const OrWorld *w, int64_t seed, int n, const int32_t *cxz, uint8_t *out) {
+    const int qmin = w->dim == OR_OVERWORLD ? -16 : 0, qh = w->dim == OR_OVERWORLD ? 96 : 64;
+#pragma omp parallel
+    {
+        SeedState *st = malloc(sizeof *st);
+        seed_init(w, st, seed);
+#pragma omp for schedule(dynamic, 1)
+        for (int c = 0; c < n; c++) {
+            const int qx0 = cxz[2 * c] * 4, qz0 = cxz[2 * c + 1] * 4;
+            uint8_t *o = out + (size_t)c * qh * 16;
+            if (w->dim == OR_END) {
+                for (int qy = 0; qy < qh; qy++)
+                    for (int qz = 0; qz < 4; qz++)
+                        for (int qx = 0; qx < 4; qx++) o[(qy * 4 + qz) * 4 + qx] = (u8)end_biome(w, st, qx0 + qx, qz0 + qz);
+                continue;
+            }
+            int last = -1;
+            for (int sy = 0; sy < qh; sy += 4)
+                for (int qx = 0; qx < 4; qx++)
+                    for (int qy = sy; qy < sy + 4 && qy < qh; qy++)
+                        for (int qz = 0; qz < 4; qz++) {
+                            i64 t[RT_DIM];
+                            sample_target_m(w, st, qx0 + qx, qmin + qy, qz0 + qz, t, 1);
+                            last = rt_search(&w->tree, w->tree.root, t, last);
+                            o[(qy * 4 + qz) * 4 + qx] = (u8)w->tree.node[last].biome;
+                        }
+        }
+        free(st);
+    }
 }

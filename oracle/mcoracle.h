@@ -42,7 +42,9 @@ void or_points(const OrWorld *w, const int64_t *seeds, int ns, const int32_t *pt
 /* Quart grid out[iz*nx+ix] at (qx0+ix, qy, qz0+iz).
  * semantics 0: per-sample search with no candidate (engine).
  * semantics 1: row-major search history (each search starts from the previous
- *              sample's leaf, first sample from none).  */
+ *              sample's leaf, first sample from none).
+ * semantics 2: the Java oracle's chunk grids: 26.3 climate through the chunk (volume) path, search
+ *              history carried chunk by chunk (quart rows then columns inside a chunk).  */
 void or_grid(const OrWorld *w, int64_t seed, int qx0, int qz0, int nx, int nz, int qy, int semantics,
              uint8_t *out);
 
@@ -94,6 +96,8 @@ double or_tree_visits(const OrWorld *w, const int32_t *tg6, int n);
 uint64_t or_splitmix64_next(uint64_t *state);
 
 uint64_t or_fnv1a_names(const uint8_t *ids, size_t n);
+/* whole chunks as libmcgen stores them (biome.c:125-157): per chunk qh*16 bytes [(qy*4+qz)*4+qx] */
+void or_chunk_biomes(const OrWorld *w, int64_t seed, int n, const int32_t *cxz, uint8_t *out);
 
 #ifdef __cplusplus
 }

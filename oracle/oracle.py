@@ -68,6 +68,7 @@ def lib():
         L.or_splitmix64_next.argtypes = [C.POINTER(C.c_uint64)]
         L.or_fnv1a_names.restype = C.c_uint64
         L.or_fnv1a_names.argtypes = [u8p, C.c_size_t]
+        L.or_chunk_biomes.argtypes = [P, C.c_int64, C.c_int, i32p, u8p]
         L.or_mcg_world.restype = P
         L.or_mcg_world.argtypes = [P, C.c_int64]
         L.or_mcg_world_free.argtypes = [P]
@@ -151,6 +152,14 @@ class Oracle:
         out = np.zeros(nx * nz, np.uint8)
         lib().or_grid(self.h, seed, qx0, qz0, nx, nz, qy, semantics, _p(out, C.c_uint8))
         return out.reshape(nz, nx)
+
+    def chunk_biomes(self, seed, chunks):
+        """Whole chunks (cx, cz) as libmcgen stores them: (n, qh, 4, 4) [chunk, qy, qz, qx]."""
+        cxz = np.ascontiguousarray(chunks, dtype=np.int32).reshape(-1, 2)
+        qh = 96 if DIMS[self.dim] == 0 else 64
+        out = np.empty(len(cxz) * qh * 16, np.uint8)
+        lib().or_chunk_biomes(self.h, int(seed), len(cxz), _p(cxz, C.c_int32), _p(out, C.c_uint8))
+        return out.reshape(len(cxz), qh, 4, 4)
 
     def block_grid(self, seed, x0, z0, nx, nz, step, y):
         out = np.zeros(nx * nz, np.uint8)
