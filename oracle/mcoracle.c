@@ -773,13 +773,21 @@ static double simplex2(const Oct *o, double xin, double yin) {        /* mc_end.
 }
 static inline float clampf(float v, float lo, float hi) { float m = v < hi ? v : hi; return v < lo ? lo : m; }
 
-static float end_height(const Oct *o, int fmt_new, i32 sx, i32 sz) {   /* mc_end.h:66-93 */
+/* Java's Mth.clamp and Math.max (libmcgen jm_clampf / jm_maxf): a NaN stays NaN */
+static inline float clampf_java(float v, float lo, float hi) { return v < lo ? lo : (v != v ? v : (v <= hi ? v : hi)); }
+static inline float maxf_java(float a, float b) { return a != a ? a : (a >= b ? a : b); }
+static float end_height_m(const Oct *o, int fmt_new, i32 sx, i32 sz, int java);
+static float end_height(const Oct *o, int fmt_new, i32 sx, i32 sz) { return end_height_m(o, fmt_new, sx, sz, 0); }
+/* java = 1: libmcgen's and Java's clamp and max (df_old.c:449-465), where the 26.1/26.2 centre term's
+ * wrapped square (|section| beyond ~32768) gives a NaN that stays; java = 0: the engine's clamp, which
+ * turns it into 80 */
+static float end_height_m(const Oct *o, int fmt_new, i32 sx, i32 sz, int java) {   /* mc_end.h:66-93 */
     i32 cx = sx / 2, cz = sz / 2, ox = sx % 2, oz = sz % 2;
     float best;
     if (!fmt_new) {
         u32 r2 = (u32)sx * (u32)sx + (u32)sz * (u32)sz;            /* int arithmetic with wrap */
         float len = (float)sqrt((double)(float)(i32)r2);
-        best = clampf(100.0f - len * 8.0f, -100.0f, 80.0f);
+        best = java ? clampf_java(100.0f - len * 8.0f, -100.0f, 80.0f) : clampf(100.0f - len * 8.0f, -100.0f, 80.0f);
     } else best = -100.0f;
     for (i32 dx = -12; dx <= 12; dx++)
         for (i32 dz = -12; dz <= 12; dz++) {
@@ -791,6 +799,10 @@ static float end_height(const Oct *o, int fmt_new, i32 sx, i32 sz) {   /* mc_end
             float size = fmodf(fabsf((float)tx) * 3439.0f + fabsf((float)tz) * 147.0f, 13.0f) + 9.0f;
             float xd = (float)(ox - dx * 2), zd = (float)(oz - dz * 2);
             float h = 100.0f - (float)sqrt((double)(xd * xd + zd * zd)) * size;
+            if (java) {
+                best = maxf_java(best, clampf_java(h, -100.0f, 80.0f));
+                continue;
+            }
             h = clampf(h, -100.0f, 80.0f);
             best = best > h ? best : h;
         }
@@ -1593,10 +1605,24 @@ void or_chunk_biomes(const OrWorld *w, int64_t seed, int n, const int32_t *cxz, 
         for (int c = 0; c < n; c++) {
             const int qx0 = cxz[2 * c] * 4, qz0 = cxz[2 * c + 1] * 4;
             uint8_t *o = out + (size_t)c * qh * 16;
-            if (w->dim == OR_END) {
-                for (int qy = 0; qy < qh; qy++)
-                    for (int qz = 0; qz < 4; qz++)
-                        for (int qx = 0; qx < 4; qx++) o[(qy * 4 + qz) * 4 + qx] = (u8)end_biome(w, st, qx0 + qx, qz0 + qz);
+            if (w->dim == OR_END) {                     /* biome.c:26-45 with libmcgen's (Java) End height */
+                const i32 cx = cxz[2 * c], cz = cxz[2 * c + 1];
+                int b = w->end_ids[0];
+                if ((i64)cx * cx + (i64)cz * cz > 4096) {
+                    const i32 bx = (cx * 2 + 1) * 8, bz = (cz * 2 + 1) * 8;
+                    float hgt = end_height_m(&st->end, w->fmt_new, bx / 8, bz / 8, 1);
+                    double h;
+                    if (!w->fmt_new) h = ((double)hgt - 8.0) / 128.0;
+                    else {
+                        float dx = (float)(0 - bx), dz = (float)(0 - bz);
+                        float len = (float)sqrt((double)(dx * dx + 0.0f * 0.0f + dz * dz));
+                        float cc = (clampf_java(100.0f - len, -100.0f, 80.0f) - 8.0f) * 0.0078125f;
+                        float e = (hgt - 8.0f) / 128.0f;
+                        h = (double)(cc != cc ? cc : (cc >= e ? cc : e));
+                    }
+                    b = h > 0.25 ? w->end_ids[1] : (h >= -0.0625 ? w->end_ids[2] : (h < -0.21875 ? w->end_ids[3] : w->end_ids[4]));
+                }
+                memset(o, b, (size_t)qh * 16);
                 continue;
             }
             int last = -1;

@@ -30,7 +30,7 @@ def test_chunk_grid_matches_java():
 
 
 @pytest.mark.parametrize("version,dim", [("26.3", "overworld"), ("26.3", "nether"), ("26.1", "overworld"),
-                                         ("26.3", "end")])
+                                         ("26.3", "end"), ("26.1", "end")])
 def test_chunks_equal_chain_formulation(version, dim):
     seed = 77
     V = mcgprog.vanilla(version, dim, seed)
