@@ -417,6 +417,38 @@ def bench_plugin(args, torch, mcbiome):
             "unresolved": int(bad), "cells_per_call": int(2 * ((n + 3) // 4 + 2) ** 2), "cpu_baseline": cpu}
 
 
+def bench_chunks(args, torch, mcbiome):
+    """Whole chunks as libmcgen stores them (mcb_chunk_biomes): 26.3 Overworld, a 64 x 64 block of chunks
+    (4096 chunks x 96 quart layers x 16 = 6,291,456 cells) per call, origin from splitmix64(6)."""
+    d = splitmix64_stream(6, 3)
+    seed, cx0, cz0 = signed64(d[0]), uniform(d[1], 1 << 18), uniform(d[2], 1 << 18)
+    xs, zs = np.meshgrid(np.arange(64), np.arange(64))
+    chunks = np.stack([cx0 + xs.ravel(), cz0 + zs.ravel()], 1).astype(np.int32)
+    eng = mcbiome.Engine("26.3", "overworld")
+    for _ in range(args.warmup):
+        eng.chunk_biomes(seed, chunks)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = eng.chunk_biomes(seed, chunks)
+    dt = (time.perf_counter() - t0) / args.steps
+    cells = chunks.shape[0] * 96 * 16
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        orc = oracle.Oracle("26.3", "overworld")
+        t1 = time.perf_counter()
+        orc.chunk_biomes(seed, chunks[:128])
+        ct = time.perf_counter() - t1
+        cpu = {"value": round(128 * 96 * 16 / ct, 1), "unit": "cells/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": "128 of the chunks, the oracle's restatement of world_chunk_biomes (OpenMP over chunks)"}
+    return {"metric": "chunk-mode biome cells/s (mcb_chunk_biomes, world export and host copy included)",
+            "value": round(cells / dt, 1), "unit": "cells/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f32+f64 (26.3)", "data": "synthetic seed and origin",
+            "config": {"workload": "chunks: 26.3 Overworld, 64x64 chunks x 96 quart layers per call"},
+            "checksum": int(out.astype(np.int64).sum()), "cpu_baseline": cpu}
+
+
 def bench_plain(args, torch, mcbiome):
     """One seed (first output of splitmix64(0)), 256x256 quarts at qy 16, origin from the next two draws."""
     d = splitmix64_stream(0, 3)
@@ -507,7 +539,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield", "plugin"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield", "plugin", "chunks"])
     ap.add_argument("--search-candidates", type=int, default=1 << 28)
     ap.add_argument("--tile-w", type=int, default=0, help="fixed tile (width | cols_per_lane<<16); 0 = autotune")
     ap.add_argument("--graph", action="store_true", help="replay each plan as a hipGraph")
@@ -535,7 +567,8 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve,
-          "search": bench_search, "endfield": bench_endfield, "plugin": bench_plugin}[args.workload]
+          "search": bench_search, "endfield": bench_endfield, "plugin": bench_plugin,
+          "chunks": bench_chunks}[args.workload]
     print(json.dumps(fn(args, torch, mcbiome)))
 
 
