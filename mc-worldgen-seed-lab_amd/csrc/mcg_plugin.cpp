@@ -545,6 +545,31 @@ bool run_chunks(GBiome *g, int n, const int32_t *cxz, uint8_t *out, int *nunres)
 
 }  // namespace
 
+void *mcb::plug::world_new(const McgBiomeWorld *W, int chain_cap, char *err, size_t errlen) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    HostWorld H;
+    std::unique_ptr<GBiome> g(new GBiome());
+    if (W && prepare_world(W, H) && ensure_init() && build_world(g.get(), W, H)) {
+        g->W.chain_cap = chain_cap;
+        return g.release();
+    }
+    if (!W) set_error("null world");
+    g->release();
+    if (err && errlen) snprintf(err, errlen, "%s", g_err);
+    return nullptr;
+}
+
+int mcb::plug::world_chunks(void *h, int n, const int32_t *cxz, uint8_t *out, int *nunres) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!h || n < 0 || (n && (!cxz || !out))) { set_error("bad arguments"); return 1; }
+    if (n == 0) {
+        if (nunres) *nunres = 0;
+        return 0;
+    }
+    if (!ensure_init()) return 1;
+    return run_chunks((GBiome *)h, n, cxz, out, nunres) ? 0 : 1;
+}
+
 int mcb::plug::chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, uint8_t *out, int *nunres, char *err,
                             size_t errlen) {
     std::lock_guard<std::mutex> lk(g_mu);

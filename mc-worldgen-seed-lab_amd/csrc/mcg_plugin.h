@@ -89,6 +89,12 @@ struct PJob {
 // fill order from the chunk's first cell with no length cap.  0 = ok, else err is set.
 int chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, uint8_t *out, int *nunres, char *err, size_t errlen);
 
+// A compiled world kept between calls (the handle mcgpu_biome_new returns), with a chosen tie-chain cap:
+// kChainCap for the plugin ABI, unbounded (1 << 30) for libmcgen's CPU semantics (biome.c:87-120 has none).
+// The grid and points calls of the handle are mcgpu_biome_grid / mcgpu_biome_points.
+void *world_new(const McgBiomeWorld *W, int chain_cap, char *err, size_t errlen);
+int world_chunks(void *h, int n, const int32_t *cxz, uint8_t *out, int *nunres);
+
 // launchers (mcg_plugin.hip); lds = bytes of dynamic LDS, perm_lds = permutation tables staged in LDS
 hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st);
 // block grid from a lattice of cells (MULTI) or of chunks (END): out[iz*nx+ix]

@@ -451,28 +451,28 @@ int or_mcg_chunk_seq(const McgBiomeWorld *W, int cx, int cz, int32_t *out, int32
     return 0;
 }
 
-int or_mcg_points(const McgBiomeWorld *W, int n, const int *xyz, uint8_t *out, int *nunres) {
+int or_mcg_points(const McgBiomeWorld *W, int n, const int *xyz, uint8_t *out, int *nunres, int cap) {
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad)
     for (int i = 0; i < n; i++) {
         int32_t q[3];
         or_zoom(W->zoom_seed, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], q);
         int u = 0;
-        int b = cell_impl(W, q[0], q[1], q[2], &u, NULL, 12);
+        int b = cell_impl(W, q[0], q[1], q[2], &u, NULL, cap);
         if (u || b < 0) { out[i] = 255; bad++; } else out[i] = (uint8_t)b;
     }
     if (nunres) *nunres = bad;
     return 0;
 }
 
-int or_mcg_grid(const McgBiomeWorld *W, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out, int *nunres) {
+int or_mcg_grid(const McgBiomeWorld *W, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out, int *nunres, int cap) {
     int bad = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : bad)
     for (int i = 0; i < nx * nz; i++) {
         int32_t q[3];
         or_zoom(W->zoom_seed, x0 + (i % nx) * step, y, z0 + (i / nx) * step, q);
         int u = 0;
-        int b = cell_impl(W, q[0], q[1], q[2], &u, NULL, 12);
+        int b = cell_impl(W, q[0], q[1], q[2], &u, NULL, cap);
         if (u || b < 0) { out[i] = 255; bad++; } else out[i] = (uint8_t)b;
     }
     if (nunres) *nunres = bad;

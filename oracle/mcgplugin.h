@@ -25,9 +25,10 @@ int or_mcg_cell(const McgBiomeWorld *W, int qx, int qy, int qz, int *unres, int 
 int or_mcg_chunk_seq(const McgBiomeWorld *W, int cx, int cz, int32_t *out, int32_t *valid);
 /* quart grid (qx0+ix, qy, qz0+iz) in row-major order with search history (Java oracle grids) */
 int or_mcg_quart_history(const McgBiomeWorld *W, int qx0, int qz0, int nx, int nz, int qy, uint8_t *out);
-/* mcgpu_biome_points / mcgpu_biome_grid semantics (255 = unresolved, counted in *nunres) */
-int or_mcg_points(const McgBiomeWorld *W, int n, const int *xyz, uint8_t *out, int *nunres);
-int or_mcg_grid(const McgBiomeWorld *W, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out, int *nunres);
+/* mcgpu_biome_points / mcgpu_biome_grid semantics (255 = unresolved, counted in *nunres); cap = longest tie
+ * chain followed (12: the plugin ABI; -1: none, libmcgen's CPU mcgen_biome_at) */
+int or_mcg_points(const McgBiomeWorld *W, int n, const int *xyz, uint8_t *out, int *nunres, int cap);
+int or_mcg_grid(const McgBiomeWorld *W, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out, int *nunres, int cap);
 
 #ifdef __cplusplus
 }

@@ -75,8 +75,8 @@ def lib():
         L.or_mcg_climate.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
         L.or_mcg_cell.argtypes = [P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_mcg_chunk_seq.argtypes = [P, C.c_int, C.c_int, i32p, i32p]
-        L.or_mcg_points.argtypes = [P, C.c_int, C.POINTER(C.c_int), u8p, C.POINTER(C.c_int)]
-        L.or_mcg_grid.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p, C.POINTER(C.c_int)]
+        L.or_mcg_points.argtypes = [P, C.c_int, C.POINTER(C.c_int), u8p, C.POINTER(C.c_int), C.c_int]
+        L.or_mcg_grid.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p, C.POINTER(C.c_int), C.c_int]
         L.or_mcg_quart_history.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p]
         _lib = L
     return _lib
@@ -316,17 +316,18 @@ class McgWorldRef:
         rc = lib().or_mcg_quart_history(self.addr, qx0, qz0, nx, nz, qy, _p(out, C.c_uint8))
         return out, rc == 0
 
-    def points(self, xyz):
+    def points(self, xyz, cap=12):
+        """cap: longest tie chain followed (12 = the plugin ABI, -1 = none, libmcgen's mcgen_biome_at)"""
         xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)
         out = np.empty(len(xyz), np.uint8)
         bad = C.c_int(0)
-        lib().or_mcg_points(self.addr, len(xyz), xyz.ctypes.data_as(C.POINTER(C.c_int)), _p(out, C.c_uint8), C.byref(bad))
+        lib().or_mcg_points(self.addr, len(xyz), xyz.ctypes.data_as(C.POINTER(C.c_int)), _p(out, C.c_uint8), C.byref(bad), cap)
         return out, bad.value
 
-    def grid(self, x0, z0, nx, nz, step, y):
+    def grid(self, x0, z0, nx, nz, step, y, cap=12):
         out = np.empty(nx * nz, np.uint8)
         bad = C.c_int(0)
-        lib().or_mcg_grid(self.addr, x0, z0, nx, nz, step, y, _p(out, C.c_uint8), C.byref(bad))
+        lib().or_mcg_grid(self.addr, x0, z0, nx, nz, step, y, _p(out, C.c_uint8), C.byref(bad), cap)
         return out.reshape(nz, nx), bad.value
 
     def close(self):

@@ -1,0 +1,58 @@
+"""The libmcgen-compatible biome API (include/mcgen.h) on the host: every declared symbol is exported, and
+the calls that need no device follow libmcgen's contract (versions, enumerations, names, refusals of what
+this library does not compute)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import mcgen
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mcgen.h")
+
+
+def test_exports_every_declared_symbol():
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(mcgen_[a-z_]+)\s*\(", txt)) - {"mcgen_seeds_unified"})
+    assert len(names) >= 40
+    L = mcgen.lib()
+    assert not [n for n in names if not hasattr(L, n)]
+    assert L.mcgen_abi_version() == 1 and b"abi 1" in L.mcgen_version()
+
+
+def test_open_versions_and_enumerations():
+    for v in ("26.1", "26.2", "26.3"):
+        g = mcgen.Gen(v, pack_dir="/nonexistent/pack")
+        L = mcgen.lib()
+        assert L.mcgen_dimension_count(g.h) == 3
+        assert [L.mcgen_dimension_name(g.h, i) for i in range(3)] == [b"minecraft:overworld", b"minecraft:the_nether",
+                                                                     b"minecraft:the_end"]
+        assert [L.mcgen_preset_name(g.h, b"minecraft:overworld", i) for i in range(3)] == [b"normal", b"large_biomes", b"amplified"]
+        assert L.mcgen_preset_count(g.h, b"minecraft:the_end") == 1
+        assert L.mcgen_biome_count(g.h) == 67
+        assert L.mcgen_biome_name(g.h, 41) == b"minecraft:plains" and L.mcgen_biome_name(g.h, 67) is None
+        assert L.mcgen_block_state_count(g.h) == 0 and L.mcgen_tweak_count(g.h) == 0
+        g.close()
+    with pytest.raises(mcgen.McgenError) as e:
+        mcgen.Gen("26.4-snapshot-2")
+    assert e.value.code == mcgen.MCGEN_E_VERSION
+
+
+def test_refusals_without_device():
+    g = mcgen.Gen("26.3")
+    with pytest.raises(mcgen.McgenError) as e:
+        g.world("minecraft:overworld", "normal", 1, tweaks=[("minecraft:sea_level", 70.0)])
+    assert e.value.code == mcgen.MCGEN_E_UNSUPPORTED
+    with pytest.raises(mcgen.McgenError) as e:
+        g.world("minecraft:the_aether", "normal", 1)
+    assert e.value.code == mcgen.MCGEN_E_ARG
+    with pytest.raises(mcgen.McgenError) as e:
+        g.world("minecraft:overworld", "floating_islands", 1)
+    assert e.value.code == mcgen.MCGEN_E_ARG
+    L = mcgen.lib()
+    assert L.mcgen_biome_at(None, 0, 0, 0) == -1
+    out = (C.c_uint8 * 4)()
+    assert L.mcgen_biome_grid_cpu(None, 0, 0, 2, 2, 1, 64, out) == mcgen.MCGEN_E_UNSUPPORTED
+    assert L.mcgen_gpu_set_compute(mcgen.MCGEN_COMPUTE_CPU, -1) == mcgen.MCGEN_E_UNSUPPORTED
+    assert L.mcgen_gpu_set_compute(7, -1) == mcgen.MCGEN_E_ARG
