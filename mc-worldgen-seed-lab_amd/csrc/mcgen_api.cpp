@@ -185,7 +185,8 @@ int mcgen_generate_region(McWorld *w, int cx0, int cz0, int nx, int nz, uint32_t
     (void)threads;
     if (!out) return MCGEN_E_ARG;
     *out = nullptr;
-    if (!w || !w->h || nx <= 0 || nz <= 0 || (int64_t)nx * nz > (1 << 24)) { put_err(err, errlen, "bad arguments"); return MCGEN_E_ARG; }
+    // at most 65,536 chunks per call (100 MB of Overworld biomes on the host)
+    if (!w || !w->h || nx <= 0 || nz <= 0 || (int64_t)nx * nz > (1 << 16)) { put_err(err, errlen, "bad arguments"); return MCGEN_E_ARG; }
     if (stages & ~(uint32_t)MC_STAGE_BIOMES) {
         put_err(err, errlen, "only MC_STAGE_BIOMES is computed by this library");
         return MCGEN_E_UNSUPPORTED;
