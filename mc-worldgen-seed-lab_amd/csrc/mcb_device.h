@@ -168,9 +168,27 @@ __device__ __forceinline__ float gdot_f(GradEnt g, float x, float y, float z) {
     const half2v yz = __builtin_bit_cast(half2v, (uint32_t)(g >> 32));
     return fmaf((float)yz.y, z, fmaf((float)yz.x, y, gx * x));
 }
-__device__ __forceinline__ double gdot_d(int h, double x, double y, double z) {
-    h &= 15;
-    return fma((double)gcomp(kGZ, h), z, fma((double)gcomp(kGY, h), y, (double)gcomp(kGX, h) * x));
+// Double-precision entry (26.1/26.2), 16 bytes read with one ds_read_b128: gx as a double, gy and gz as
+// f32 (one conversion each).  Bit-field extraction plus integer conversion of all three components took
+// eight VALU instructions per corner; this takes three (the index shift and two conversions).
+struct __attribute__((aligned(16))) GradD {
+    double gx;
+    float gy, gz;
+};
+__device__ __forceinline__ void init_grad(GradD *G) {
+    for (int h = threadIdx.x; h < 256; h += blockDim.x) {
+        const int k = h & 15;
+        G[h] = GradD{(double)gcomp(kGX, k), (float)gcomp(kGY, k), (float)gcomp(kGZ, k)};
+    }
+}
+// gradient table type of a precision mode: 1 = float arithmetic (26.3), 0 = double (26.1/26.2)
+template <int MODE> struct GT { typedef GradEnt T; };
+template <> struct GT<0> { typedef GradD T; };
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double gdot_d(const GradD &g, double x, double y, double z) {
+    const u32x4 v = *reinterpret_cast<const u32x4 *>(&g);          // one 16-byte LDS read
+    const double gx = __builtin_bit_cast(double, (uint64_t)v.y << 32 | v.x);
+    return fma((double)__uint_as_float(v.w), z, fma((double)__uint_as_float(v.z), y, gx * x));
 }
 __device__ __forceinline__ float lerp_f(float t, float a, float b) { return a + t * (b - a); }
 __device__ __forceinline__ double lerp_d(double t, double a, double b) { return a + t * (b - a); }
@@ -188,7 +206,7 @@ template <> struct R<0> { typedef double T; };
 // Trilinear blend of the 8 lattice corners (the reference's nesting: x, then y, then z).  hXYZ = hash
 // of corner (X, Y, Z); f* = fractions; u, v, w = fades along x, y, z.
 template <int MODE>
-__device__ __forceinline__ typename R<MODE>::T corners(const GradEnt *G, int h000, int h100, int h010, int h110, int h001,
+__device__ __forceinline__ typename R<MODE>::T corners(const typename GT<MODE>::T *G, int h000, int h100, int h010, int h110, int h001,
                                                        int h101, int h011, int h111, typename R<MODE>::T fx,
                                                        typename R<MODE>::T fy, typename R<MODE>::T fz,
                                                        typename R<MODE>::T u, typename R<MODE>::T v, typename R<MODE>::T w) {
@@ -202,10 +220,10 @@ __device__ __forceinline__ typename R<MODE>::T corners(const GradEnt *G, int h00
                       lerp_f(v, lerp_f(u, c001, c101), lerp_f(u, c011, c111)));
     } else {
         const double gx = fx - 1.0, gy = fy - 1.0, gz = fz - 1.0;
-        const double c000 = gdot_d(h000, fx, fy, fz), c100 = gdot_d(h100, gx, fy, fz);
-        const double c010 = gdot_d(h010, fx, gy, fz), c110 = gdot_d(h110, gx, gy, fz);
-        const double c001 = gdot_d(h001, fx, fy, gz), c101 = gdot_d(h101, gx, fy, gz);
-        const double c011 = gdot_d(h011, fx, gy, gz), c111 = gdot_d(h111, gx, gy, gz);
+        const double c000 = gdot_d(G[h000], fx, fy, fz), c100 = gdot_d(G[h100], gx, fy, fz);
+        const double c010 = gdot_d(G[h010], fx, gy, fz), c110 = gdot_d(G[h110], gx, gy, fz);
+        const double c001 = gdot_d(G[h001], fx, fy, gz), c101 = gdot_d(G[h101], gx, fy, gz);
+        const double c011 = gdot_d(G[h011], fx, gy, gz), c111 = gdot_d(G[h111], gx, gy, gz);
         return lerp_d(w, lerp_d(v, lerp_d(u, c000, c100), lerp_d(u, c010, c110)),
                       lerp_d(v, lerp_d(u, c001, c101), lerp_d(u, c011, c111)));
     }
@@ -213,7 +231,7 @@ __device__ __forceinline__ typename R<MODE>::T corners(const GradEnt *G, int h00
 
 // Octave whose y coordinate is the octave offset (climate noises, ShiftA): x, z vary.
 template <int MODE, class PA>
-__device__ __forceinline__ typename R<MODE>::T oct_yc(PA P2, const GradEnt *G, const McbOct &o, double X, double Z) {
+__device__ __forceinline__ typename R<MODE>::T oct_yc(PA P2, const typename GT<MODE>::T *G, const McbOct &o, double X, double Z) {
     typedef typename R<MODE>::T T;
     const double x = wrapc(X) + o.xo, z = wrapc(Z) + o.zo;
     const double xf = floor(x), zf = floor(z);
@@ -230,7 +248,7 @@ __device__ __forceinline__ typename R<MODE>::T oct_yc(PA P2, const GradEnt *G, c
 
 // Octave whose z coordinate is the octave offset (ShiftB): x, y vary.
 template <int MODE, class PA>
-__device__ __forceinline__ typename R<MODE>::T oct_zc(PA P2, const GradEnt *G, const McbOct &o, double X, double Y) {
+__device__ __forceinline__ typename R<MODE>::T oct_zc(PA P2, const typename GT<MODE>::T *G, const McbOct &o, double X, double Y) {
     typedef typename R<MODE>::T T;
     const double x = wrapc(X) + o.xo, y = wrapc(Y) + o.yo;
     const double xf = floor(x), yf = floor(y);
@@ -252,7 +270,7 @@ __device__ __forceinline__ typename R<MODE>::T oct_zc(PA P2, const GradEnt *G, c
 // second stack (NoiseStack.get, mc_noise.h:303-321); 26.1/26.2: amp_d[k] * octave(wrap(X'*f), wrap(Y'*f))
 // * vf_d[k] with X' = X, or X*1.0181... for the second stack (mc_noise.h:286-301).
 template <int MODE, bool ZC, class PA>
-__device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const GradEnt *G, const McbOct &o, const McbNoise &N, int k,
+__device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const typename GT<MODE>::T *G, const McbOct &o, const McbNoise &N, int k,
                                                            int st, double X, double Y) {
     if constexpr (MODE == 1) {
         const double f = st ? N.freq[k] * INPUT_FACTOR : N.freq[k];
@@ -271,7 +289,7 @@ __device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const GradEnt *
 // mc_noise.h:303-312; legacy-nether order :313-321); 26.1/26.2 sums each stack in double, then
 // (v1+v2)*valueFactor (mc_noise.h:286-301).
 template <int MODE, bool ZC, int NL, int BASE, bool LEGACY>
-__device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm2, const GradEnt *G, const McbOct *oct,
+__device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm2, const typename GT<MODE>::T *G, const McbOct *oct,
                                                        const McbNoise &N, double X, double Y) {
     auto term = [&](int k, int st) {
         const int o = BASE + st * NL + k;
@@ -590,7 +608,7 @@ __device__ __forceinline__ void shift_coords(typename R<MODE>::T a, typename R<M
 }
 
 template <int MODE>
-__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const uint8_t *perm, const GradEnt *G,
+__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const uint8_t *perm, const typename GT<MODE>::T *G,
                                           const SplLds *SP, const McbOct *oct, int qx, int qz, Column &c) {
     typedef typename R<MODE>::T T_;
     const McbNoise *N = W->noise;
@@ -625,7 +643,7 @@ __device__ __forceinline__ int ow_depth(const McbWorld *__restrict__ W, const Co
 
 // Nether climate: two legacy NormalNoises at (bx*0.25, by*0.0, bz*0.25) (engine/cuda/mcg_host.c:63-74)
 template <int MODE>
-__device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, const uint8_t *perm, const GradEnt *G,
+__device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, const uint8_t *perm, const typename GT<MODE>::T *G,
                                           const McbOct *oct, int qx, int qz, int *t) {
     const double X = (double)qx, Z = (double)qz;
     // layout MCB_NE_LEVELS {2, 2}, legacy order: bases 0 and 4

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                                               const McbTile *__restrict__ tiles, int ntiles, int tile_w, int cpl,
                                               uint8_t *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
-    __shared__ __attribute__((aligned(16))) GradEnt G[256];
+    __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
     __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
     __shared__ SplLds spl[1];
     SplLds *SP = spl;
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
                                                 const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
                                                 int32_t *__restrict__ tg) {
     __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
-    __shared__ __attribute__((aligned(16))) GradEnt G[256];
+    __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
     const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
     const int s = blockIdx.y;
     const int slot = slot_of[s];

@@ -82,7 +82,7 @@ template <int MODE>
 __global__ __launch_bounds__(128) void k_search_ow(SearchArgs A) {
     typedef typename R<MODE>::T T_;
     __shared__ uint32_t tiles[2][64 * 64];
-    __shared__ __attribute__((aligned(16))) GradEnt G[256];
+    __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
     __shared__ SplLds spl[1];
     __shared__ uint32_t clut[MCB_CELL_LUT];
     __shared__ uint32_t hist[MCB_MAX_OBS + 1];
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(128) void k_search_ow(SearchArgs A) {
 template <int MODE, int DIM>
 __global__ __launch_bounds__(64) void k_search_slots(SearchArgs A, const McbSlot *__restrict__ slots, int ncand) {
     __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
-    __shared__ __attribute__((aligned(16))) GradEnt G[256];
+    __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
     __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
     __shared__ SplLds spl[1];
     const McbWorld *__restrict__ W = A.W;
