@@ -377,8 +377,9 @@ bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
         H.cells = plugin_cells(H.tree);
     }
     const int vbytes = H.C.nslot * kWG * (W->newf ? 4 : 8);
-    if (vbytes > kMaxLdsBytes) {
-        set_error("program keeps %d values alive at once; at most %d fit", H.C.nslot, kMaxLdsBytes / (kWG * (W->newf ? 4 : 8)));
+    if (grad_lds_bytes(W->newf != 0) + vbytes > kMaxLdsBytes) {
+        set_error("program keeps %d values alive at once; at most %d fit", H.C.nslot,
+                  (kMaxLdsBytes - grad_lds_bytes(W->newf != 0)) / (kWG * (W->newf ? 4 : 8)));
         return false;
     }
     H.p = p;
@@ -404,11 +405,11 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     if (W->source == MCG_BS_FIXED) return true;
     const McgProg *p = H.p;
     const Compiled &C = H.C;
-    // LDS: value slots always, permutation tables when they fit beside them
-    const int vbytes = C.nslot * kWG * (g->newf ? 4 : 8);
-    const int pbytes = (p->noct * 256 + 15) & ~15;
-    g->perm_lds = (int64_t)vbytes + pbytes <= kMaxLdsBytes;
-    g->lds = vbytes + (g->perm_lds ? pbytes : 0);
+    // LDS: gradient table and value slots always, the doubled permutation tables when they fit beside them
+    const int gbytes = grad_lds_bytes(g->newf), vbytes = C.nslot * kWG * (g->newf ? 4 : 8);
+    const int64_t pbytes = (int64_t)p->noct * 512;
+    g->perm_lds = gbytes + vbytes + pbytes <= kMaxLdsBytes;
+    g->lds = gbytes + vbytes + (g->perm_lds ? (int)pbytes : 0);
     // one device allocation: ins, noise, layers, octs, sp, splf, spc, arr, thr, tree
     struct Part { const void *src; size_t bytes; size_t off; };
     std::vector<Part> parts = {

@@ -21,6 +21,8 @@ constexpr int kWG = 256;                 // lanes per workgroup of the plugin ke
 constexpr int kChainCap = 12;            // tied cells followed back along the fill order (reference: 12)
 constexpr int kSplineDepth = 16;         // spline nesting accepted (the reference's frame stack holds 16)
 constexpr int kMaxLdsBytes = 64 * 1024;  // LDS per workgroup the kernels may use
+// LDS gradient table of k_cells: 256 entries of 8 bytes (float noise) or 16 bytes (double noise)
+constexpr int grad_lds_bytes(bool newf) { return newf ? 256 * 8 : 256 * 16; }
 
 // one program node, compiled (128 bytes; operand fields are value slots)
 struct PIns {

@@ -10,17 +10,27 @@
 //
 // MI355X layout: the compiled program (PIns) is read with scalar loads, every lane running the same
 // instruction; live values sit in LDS slots, column per lane (slot s of lane l at s*256 + l, no bank
-// conflicts); the octave permutation tables are staged in LDS once per workgroup when they fit.
+// conflicts); the octave permutation tables are staged in LDS once per workgroup when they fit, each
+// twice in a row (512 bytes) so the lattice hash needs no index masks, and a gradient table per
+// permutation value (the engine's: f32/f16 entries for float noise, 16-byte double entries for the
+// legacy path) replaces the bit-field extraction of every corner's gradient.
 #include "mcg_plugin.h"
 #include "mcb_device.h"
 
 namespace mcb {
 namespace plug {
 
-using dev::gcomp;
-using dev::kGX;
-using dev::kGY;
-using dev::kGZ;
+using dev::GradD;
+using dev::GradEnt;
+using dev::gdot_d;
+using dev::gdot_f;
+
+// dynamic LDS layout of k_cells: gradient table at 0, then the permutation tables (doubled, when staged),
+// then the value slots
+template <bool NEWF> struct GradOf { typedef GradEnt T; };
+template <> struct GradOf<false> { typedef GradD T; };
+template <bool NEWF> constexpr int grad_bytes() { return 256 * (int)sizeof(typename GradOf<NEWF>::T); }
+static_assert(grad_bytes<true>() == grad_lds_bytes(true) && grad_bytes<false>() == grad_lds_bytes(false), "gradient table size");
 
 // ---------------------------------------------------------------------------------------------------
 // Java arithmetic (mcgpu_dev.cuh:24-58)
@@ -80,14 +90,10 @@ __device__ __forceinline__ float jsignf(float v) {
 // ---------------------------------------------------------------------------------------------------
 // octaves (mcgpu_dev.cuh:64-155; CPU noise.c)
 // ---------------------------------------------------------------------------------------------------
-__device__ __forceinline__ float pdot_f(int h, float x, float y, float z) {
-    h &= 15;
-    return fmaf((float)gcomp(kGZ, h), z, fmaf((float)gcomp(kGY, h), y, (float)gcomp(kGX, h) * x));
-}
-__device__ __forceinline__ double pdot_d(int h, double x, double y, double z) {
-    h &= 15;
-    return fma((double)gcomp(kGZ, h), z, fma((double)gcomp(kGY, h), y, (double)gcomp(kGX, h) * x));
-}
+// Permutation lookup: D2 = the doubled 512-byte table (the coordinates are reduced mod 256 once, every
+// later index stays below 512, so perm[i & 255] = P[i] without masks); else the 256-byte table, masked.
+template <bool D2> __device__ __forceinline__ int pperm(const uint8_t *P, int i) { return D2 ? (int)P[i] : (int)P[i & 255]; }
+template <bool D2> __device__ __forceinline__ int pcoord(int v) { return D2 ? (v & 255) : v; }
 __device__ __forceinline__ float smooth_f(float x) { return x * x * x * (x * (x * 6.0f - 15.0f) + 10.0f); }
 __device__ __forceinline__ double smooth_d(double x) { return x * x * x * (x * (x * 6.0 - 15.0) + 10.0); }
 __device__ __forceinline__ double wrap_new(double x) {
@@ -96,23 +102,26 @@ __device__ __forceinline__ double wrap_new(double x) {
 }
 __device__ __forceinline__ double wrap_old(double x) { return x - (double)jd2l(floor(x / 3.3554432E7 + 0.5)) * 3.3554432E7; }
 
-#define PP(i) ((int)P[(i) & 0xFF])
-__device__ __forceinline__ float sample_lerp_f(const uint8_t *P, int x, int y, int z, float rx, float ry, float rz,
-                                               float ry_orig) {
+#define PP(i) pperm<D2>(P, i)
+template <bool D2>
+__device__ __forceinline__ float sample_lerp_f(const uint8_t *P, const GradEnt *G, int x, int y, int z, float rx, float ry,
+                                               float rz, float ry_orig) {
+    x = pcoord<D2>(x); y = pcoord<D2>(y); z = pcoord<D2>(z);
     const int x0 = PP(x), x1 = PP(x + 1);
     const int xy00 = PP(x0 + y), xy01 = PP(x0 + y + 1), xy10 = PP(x1 + y), xy11 = PP(x1 + y + 1);
     const float gx = rx - 1.0f, gy = ry - 1.0f, gz = rz - 1.0f;
-    const float d000 = pdot_f(PP(xy00 + z), rx, ry, rz), d100 = pdot_f(PP(xy10 + z), gx, ry, rz);
-    const float d010 = pdot_f(PP(xy01 + z), rx, gy, rz), d110 = pdot_f(PP(xy11 + z), gx, gy, rz);
-    const float d001 = pdot_f(PP(xy00 + z + 1), rx, ry, gz), d101 = pdot_f(PP(xy10 + z + 1), gx, ry, gz);
-    const float d011 = pdot_f(PP(xy01 + z + 1), rx, gy, gz), d111 = pdot_f(PP(xy11 + z + 1), gx, gy, gz);
+    const float d000 = gdot_f(G[PP(xy00 + z)], rx, ry, rz), d100 = gdot_f(G[PP(xy10 + z)], gx, ry, rz);
+    const float d010 = gdot_f(G[PP(xy01 + z)], rx, gy, rz), d110 = gdot_f(G[PP(xy11 + z)], gx, gy, rz);
+    const float d001 = gdot_f(G[PP(xy00 + z + 1)], rx, ry, gz), d101 = gdot_f(G[PP(xy10 + z + 1)], gx, ry, gz);
+    const float d011 = gdot_f(G[PP(xy01 + z + 1)], rx, gy, gz), d111 = gdot_f(G[PP(xy11 + z + 1)], gx, gy, gz);
     const float a1 = smooth_f(rx), a2 = smooth_f(ry_orig), a3 = smooth_f(rz);
     return jlerpf(a3, jlerpf(a2, jlerpf(a1, d000, d100), jlerpf(a1, d010, d110)),
                   jlerpf(a2, jlerpf(a1, d001, d101), jlerpf(a1, d011, d111)));
 }
 // GradientNoise (26.3+): perlin (kind 0) or smeared (kind 1, y fudge fs)
-__device__ __forceinline__ float octave_f(const uint8_t *P, const McgOct *o, int kind, double fs, double _x, double _y,
-                                          double _z) {
+template <bool D2>
+__device__ __forceinline__ float octave_f(const uint8_t *P, const GradEnt *G, const McgOct *o, int kind, double fs, double _x,
+                                          double _y, double _z) {
     const double x = wrap_new(_x) + o->xo, y = wrap_new(_y) + o->yo, z = wrap_new(_z) + o->zo;
     const int32_t fx = jfloor(x), fy = jfloor(y), fz = jfloor(z);
     const float rx = (float)(x - fx), rz = (float)(z - fz);
@@ -122,10 +131,11 @@ __device__ __forceinline__ float octave_f(const uint8_t *P, const McgOct *o, int
         const double lim = (_y >= 0.0 && _y < ry) ? _y : ry;
         fry = (float)(ry - (double)jfloor(lim / fs + (double)1.0E-7f) * fs);
     }
-    return sample_lerp_f(P, fx, fy, fz, rx, fry, rz, (float)ry);
+    return sample_lerp_f<D2>(P, G, fx, fy, fz, rx, fry, rz, (float)ry);
 }
 // ImprovedNoise.noise of the legacy path, coordinates already wrapped (yScale = yFudge = 0*factor)
-__device__ __forceinline__ double octave_d(const uint8_t *P, const McgOct *o, double _x, double _y, double _z,
+template <bool D2>
+__device__ __forceinline__ double octave_d(const uint8_t *P, const GradD *G, const McgOct *o, double _x, double _y, double _z,
                                            double y_scale, double y_fudge) {
     const double x = _x + o->xo, y = _y + o->yo, z = _z + o->zo;
     const int32_t xf = jfloor(x), yf = jfloor(y), zf = jfloor(z);
@@ -136,22 +146,24 @@ __device__ __forceinline__ double octave_d(const uint8_t *P, const McgOct *o, do
         yrf = (double)jfloor(lim / y_scale + (double)1.0E-7f) * y_scale;
     }
     const double yy = yr - yrf;
-    const int x0 = PP(xf), x1 = PP(xf + 1);
-    const int xy00 = PP(x0 + yf), xy01 = PP(x0 + yf + 1), xy10 = PP(x1 + yf), xy11 = PP(x1 + yf + 1);
-    const double d000 = pdot_d(PP(xy00 + zf), xr, yy, zr), d100 = pdot_d(PP(xy10 + zf), xr - 1.0, yy, zr);
-    const double d010 = pdot_d(PP(xy01 + zf), xr, yy - 1.0, zr), d110 = pdot_d(PP(xy11 + zf), xr - 1.0, yy - 1.0, zr);
-    const double d001 = pdot_d(PP(xy00 + zf + 1), xr, yy, zr - 1.0), d101 = pdot_d(PP(xy10 + zf + 1), xr - 1.0, yy, zr - 1.0);
-    const double d011 = pdot_d(PP(xy01 + zf + 1), xr, yy - 1.0, zr - 1.0);
-    const double d111 = pdot_d(PP(xy11 + zf + 1), xr - 1.0, yy - 1.0, zr - 1.0);
+    const int xm = pcoord<D2>(xf), ym = pcoord<D2>(yf), zm = pcoord<D2>(zf);
+    const int x0 = PP(xm), x1 = PP(xm + 1);
+    const int xy00 = PP(x0 + ym), xy01 = PP(x0 + ym + 1), xy10 = PP(x1 + ym), xy11 = PP(x1 + ym + 1);
+    const double d000 = gdot_d(G[PP(xy00 + zm)], xr, yy, zr), d100 = gdot_d(G[PP(xy10 + zm)], xr - 1.0, yy, zr);
+    const double d010 = gdot_d(G[PP(xy01 + zm)], xr, yy - 1.0, zr), d110 = gdot_d(G[PP(xy11 + zm)], xr - 1.0, yy - 1.0, zr);
+    const double d001 = gdot_d(G[PP(xy00 + zm + 1)], xr, yy, zr - 1.0), d101 = gdot_d(G[PP(xy10 + zm + 1)], xr - 1.0, yy, zr - 1.0);
+    const double d011 = gdot_d(G[PP(xy01 + zm + 1)], xr, yy - 1.0, zr - 1.0);
+    const double d111 = gdot_d(G[PP(xy11 + zm + 1)], xr - 1.0, yy - 1.0, zr - 1.0);
     const double a1 = smooth_d(xr), a2 = smooth_d(yr), a3 = smooth_d(zr);
     return jlerp(a3, jlerp(a2, jlerp(a1, d000, d100), jlerp(a1, d010, d110)),
                  jlerp(a2, jlerp(a1, d001, d101), jlerp(a1, d011, d111)));
 }
 #undef PP
 
-template <bool PL>
-__device__ __forceinline__ const uint8_t *perm_of(const PWorld &W, const uint8_t *lds_perm, int oct) {
-    return PL ? lds_perm + oct * 256 : W.octs[oct].p;
+// lds = base of the dynamic LDS (gradient table first, then the doubled permutation tables)
+template <bool NEWF, bool PL>
+__device__ __forceinline__ const uint8_t *perm_of(const PWorld &W, const uint8_t *lds, int oct) {
+    return PL ? lds + grad_bytes<NEWF>() + oct * 512 : W.octs[oct].p;
 }
 // NoiseStack value (26.3+; mcgpu_dev.cuh:157-168, noise.c:339-347)
 template <bool PL>
@@ -162,7 +174,8 @@ __device__ float ns_get(const PWorld &W, const uint8_t *lp, int nsi, double x, d
     for (int i = 0; i < mn.n0; i++) {
         const McgLayer L = W.layers[mn.l0 + i];
         const double f = L.a;
-        const float g = octave_f(perm_of<PL>(W, lp, L.oct), W.octs + L.oct, L.kind, L.b, x * f, y * f, z * f);
+        const float g = octave_f<PL>(perm_of<true, PL>(W, lp, L.oct), (const GradEnt *)lp, W.octs + L.oct, L.kind, L.b, x * f,
+                                     y * f, z * f);
         v += L.amp_f * g;
     }
     return v;
@@ -176,7 +189,8 @@ __device__ double old_perlin(const PWorld &W, const uint8_t *lp, int l0, int n, 
         const McgLayer L = W.layers[l0 + i];
         if (L.oct >= 0) {
             const double factor = L.a;
-            const double nv = octave_d(perm_of<PL>(W, lp, L.oct), W.octs + L.oct, wrap_old(x * factor), wrap_old(y * factor),
+            const double nv = octave_d<PL>(perm_of<false, PL>(W, lp, L.oct), (const GradD *)lp, W.octs + L.oct,
+                                           wrap_old(x * factor), wrap_old(y * factor),
                                        wrap_old(z * factor), 0.0 * factor, 0.0 * factor);
             value += L.amp_d * nv * L.b;
         }
@@ -310,7 +324,7 @@ __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox
             break;
         }
         case MCGK_SHIFT_B: r = ns_get<PL>(W, lp, s.ns, bz * s.d0, bx * s.d0, 0.0) * 4.0f; break;
-        case MCGK_END: r = (end_height_j<false>(perm_of<PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0f) / 128.0f; break;
+        case MCGK_END: r = (end_height_j<false>(perm_of<true, PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0f) / 128.0f; break;
         case MCGK_DIST: r = dist_value(s.i3, (float)(s.i0 - bx), (float)(s.i1 - by), (float)(s.i2 - bz)); break;
         case MCGK_GRAD_CLAMP: case MCGK_GRAD_REPEAT: case MCGK_GRAD_MIRROR:
             r = grad_value(s, s.i0 == 0 ? bx : (s.i0 == 1 ? by : bz));
@@ -401,7 +415,7 @@ __device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int b
         case MCGO_SHIFT_B: r = old_normal<PL>(W, lp, s.ns, (double)bz * s.d0, (double)bx * s.d0, 0.0 * s.d0) * 4.0; break;
         case MCGO_SHIFT: r = old_normal<PL>(W, lp, s.ns, (double)bx * s.d0, (double)by * s.d1, (double)bz * s.d0) * 4.0; break;
         case MCGO_END_ISLANDS:
-            r = ((double)end_height_j<true>(perm_of<PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0) / 128.0;
+            r = ((double)end_height_j<true>(perm_of<false, PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0) / 128.0;
             break;
         case MCGO_Y_GRADIENT: r = clamped_map((double)by, (double)s.i0, (double)s.i1, s.d0, s.d1); break;
         case MCGO_WEIRD_SCALED: {
@@ -650,13 +664,15 @@ template <bool NEWF, bool PL, int SPD>
 __global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
     typedef typename VT_of<NEWF>::T VT;
     extern __shared__ __align__(16) uint8_t lds[];
-    const int pbytes = PL ? ((W.noct * 256 + 15) & ~15) : 0;
+    constexpr int gbytes = grad_bytes<NEWF>();
+    const int pbytes = PL ? W.noct * 512 : 0;
+    dev::init_grad((typename GradOf<NEWF>::T *)lds);
     if (PL) {
-        uint32_t *dst = (uint32_t *)lds;
-        for (int i = threadIdx.x; i < W.noct * 64; i += kWG) dst[i] = ((const uint32_t *)W.octs[i >> 6].p)[i & 63];
-        __syncthreads();
+        uint32_t *dst = (uint32_t *)(lds + gbytes);     // word w of octave o: table bytes 4(w & 63).., twice
+        for (int i = threadIdx.x; i < W.noct * 128; i += kWG) dst[i] = ((const uint32_t *)W.octs[i >> 7].p)[i & 63];
     }
-    VT *vals = (VT *)(lds + pbytes) + threadIdx.x;
+    __syncthreads();
+    VT *vals = (VT *)(lds + gbytes + pbytes) + threadIdx.x;
     const uint8_t *lp = lds;
     const int64_t stride = (int64_t)gridDim.x * kWG;
     int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x;
