@@ -12,7 +12,9 @@ stream, and the reference CPU engine on this host (cpu_baseline).  Prints ONE JS
 Other workloads (--workload): plain (one seed, 256x256 quarts at qy 16), mixed (10,000 uneven requests of
 every version and dimension, 10% zoomed), serve (Poisson arrivals at 200/1000/5000 req/s), search (seed
 search over a candidate range against 8 reference observations), endfield (End island height field of
-256 seeds over 2048x2048 blocks).
+256 seeds over 2048x2048 blocks), plugin (mcgpu_biome_grid 4096x4096 blocks), chunks (mcb_chunk_biomes of
+64x64 chunks), mcgen (the libmcgen API on the reference's own 2048x2048 step-4 biome-map benchmark, with
+vs_baseline against its published time).
 """
 import argparse
 import json
@@ -449,6 +451,67 @@ def bench_chunks(args, torch, mcbiome):
             "checksum": int(out.astype(np.int64).sum()), "cpu_baseline": cpu}
 
 
+# libmcgen's published biome-map timings (docs/blender/gpu.md:114-120; libmcgen/tests/g9_bench.c:46-54):
+# mcgen_gpu_biome_grid of 2048x2048 samples, step 4, y 64, origin (-4096, -4096), seed 12345, normal preset,
+# best of 3 wall-clock calls including the result copy, on an RTX 4080 SUPER.  Seconds per call.
+MCGEN_PUBLISHED = {("26.3", "overworld"): 0.053, ("26.1", "overworld"): 0.120, ("26.2", "overworld"): 0.119,
+                   ("26.3", "the_nether"): 0.009, ("26.3", "the_end"): 0.074}
+
+
+def bench_mcgen(args, torch, mcbiome):
+    """The libmcgen API (include/mcgen.h) on the reference's own biome-map benchmark: mcgen_gpu_biome_grid
+    of 2048x2048 samples at step 4, y 64, origin (-4096, -4096), seed 12345, Overworld 26.3 normal, one
+    synchronous call per step (the grid returns in host memory).  The other published version/dimension
+    rows are timed the same way and reported beside it."""
+    import mcgen
+    n, step, y, seed = 2048, 4, 64, 12345
+    x0 = z0 = -n * step // 2
+
+    def timed(version, dim, steps, warmup):
+        g = mcgen.Gen(version)
+        w = g.world("minecraft:" + dim, "normal", seed)
+        out = np.empty(n * n, np.uint8)
+        for _ in range(warmup):
+            w.gpu_biome_grid(x0, z0, n, n, step, y, out)
+        ts = []
+        for _ in range(steps):
+            t = time.perf_counter()
+            w.gpu_biome_grid(x0, z0, n, n, step, y, out)
+            ts.append(time.perf_counter() - t)
+        w.close()
+        g.close()
+        return sum(ts) / len(ts), min(ts), out
+
+    dt, best, out = timed("26.3", "overworld", args.steps, max(args.warmup, 1))
+    rows = []
+    for (v, d), pub in MCGEN_PUBLISHED.items():
+        a, b, _ = (dt, best, None) if (v, d) == ("26.3", "overworld") else timed(v, d, 3, 1)
+        rows.append({"version": v, "dimension": d, "ms_per_call": round(a * 1e3, 3), "best_ms": round(b * 1e3, 3),
+                     "published_ms": pub * 1e3, "speedup_vs_published": round(pub / a, 2)})
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        ref = oracle.McgWorldRef.vanilla(oracle.Oracle("26.3", "overworld"), seed)
+        m = 512
+        t1 = time.perf_counter()
+        ref.grid(x0, z0, m, m, step, y)
+        ct = time.perf_counter() - t1
+        cpu = {"value": round(m * m / ct, 1), "unit": "samples/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": "the first 512x512 samples of the same grid, the oracle's restatement of libmcgen's "
+                         "biome_at (lazy density functions, per sample, OpenMP over samples)"}
+    value = n * n / dt
+    return {"metric": "libmcgen mcgen_gpu_biome_grid samples/s (2048x2048, step 4, host copy included)",
+            "value": round(value, 1), "unit": "samples/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+            "scaling": "strong", "vs_baseline": round(value / (n * n / MCGEN_PUBLISHED[("26.3", "overworld")]), 2),
+            "baseline_note": "published 0.053 s per call (7.9e7 samples/s) on an RTX 4080 SUPER, best of 3; "
+                             "value here is the mean over the timed calls",
+            "dtype": "f32+f64 (26.3 density-function program)", "data": "the reference benchmark's seed and grid",
+            "config": {"workload": "mcgen: 26.3 Overworld normal, seed 12345, 2048x2048 samples at step 4 from "
+                                   "(-4096, -4096), y 64, one mcgen_gpu_biome_grid call per step"},
+            "checksum": int(out.astype(np.int64).sum()), "table": rows, "cpu_baseline": cpu}
+
+
 def bench_plain(args, torch, mcbiome):
     """One seed (first output of splitmix64(0)), 256x256 quarts at qy 16, origin from the next two draws."""
     d = splitmix64_stream(0, 3)
@@ -539,7 +602,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield", "plugin", "chunks"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield", "plugin", "chunks",
+                                                              "mcgen"])
     ap.add_argument("--search-candidates", type=int, default=1 << 28)
     ap.add_argument("--tile-w", type=int, default=0, help="fixed tile (width | cols_per_lane<<16); 0 = autotune")
     ap.add_argument("--graph", action="store_true", help="replay each plan as a hipGraph")
@@ -568,7 +632,7 @@ def main():
         del src, dst
     fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve,
           "search": bench_search, "endfield": bench_endfield, "plugin": bench_plugin,
-          "chunks": bench_chunks}[args.workload]
+          "chunks": bench_chunks, "mcgen": bench_mcgen}[args.workload]
     print(json.dumps(fn(args, torch, mcbiome)))
 
 

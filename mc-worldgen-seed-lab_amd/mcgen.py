@@ -53,6 +53,7 @@ def lib():
             "mcgen_biome_at": (C.c_int, [P, C.c_int, C.c_int, C.c_int]),
             "mcgen_biome_grid": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p]),
             "mcgen_biome_grid_cpu": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p]),
+            "mcgen_gpu_biome_grid": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p]),
             "mcgen_gpu_biome_points": (C.c_int, [P, C.c_int, ip, u8p, ip]),
             "mcgen_generate_region": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_int, PROGRESS_FN,
                                                 P, C.POINTER(P), C.c_char_p, C.c_size_t]),
@@ -112,6 +113,15 @@ class World:
         rc = lib().mcgen_biome_grid(self.h, x0, z0, nx, nz, step, y, out.ctypes.data_as(C.POINTER(C.c_uint8)))
         if rc:
             raise McgenError(rc, "mcgen_biome_grid")
+        return out.reshape(nz, nx)
+
+    def gpu_biome_grid(self, x0, z0, nx, nz, step, y, out=None):
+        """mcgen_gpu_biome_grid into `out` (a uint8 array of nx*nz, allocated when None)."""
+        if out is None:
+            out = np.empty(nx * nz, np.uint8)
+        rc = lib().mcgen_gpu_biome_grid(self.h, x0, z0, nx, nz, step, y, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc:
+            raise McgenError(rc, "mcgen_gpu_biome_grid")
         return out.reshape(nz, nx)
 
     def region_biomes(self, cx0, cz0, nx, nz):
