@@ -14,7 +14,8 @@ every version and dimension, 10% zoomed), serve (Poisson arrivals at 200/1000/50
 search over a candidate range against 8 reference observations), endfield (End island height field of
 256 seeds over 2048x2048 blocks), plugin (mcgpu_biome_grid 4096x4096 blocks), chunks (mcb_chunk_biomes of
 64x64 chunks), mcgen (the libmcgen API on the reference's own 2048x2048 step-4 biome-map benchmark, with
-vs_baseline against its published time).
+vs_baseline against its published time), points (mcb_points_dev on the reference's own 16,384-seed x
+64-point benchmark, vs_baseline against its published rate).
 """
 import argparse
 import json
@@ -512,6 +513,119 @@ def bench_mcgen(args, torch, mcbiome):
             "checksum": int(out.astype(np.int64).sum()), "table": rows, "cpu_baseline": cpu}
 
 
+# The reference's published engine point rates (crack/src/gpu_biomes/results/bench_gpu_26.{1,2,3}.txt:4-8,
+# bench_gpu_biomes.cu:83-100): 16,384 seeds (End 4,096) x 64 shared quart points per seed, kernel time only,
+# best of the repetitions, RTX 4080 SUPER.  Points/s.
+POINTS_PUBLISHED = {("26.3", "overworld"): 4.67e7, ("26.1", "overworld"): 2.94e7, ("26.2", "overworld"): 2.91e7,
+                    ("26.3", "the_nether"): 3.63e8, ("26.3", "the_end"): 7.72e6}
+SEEDS_1PT_PUBLISHED = 1.97e7     # 26.3 Overworld seeds/s at one point per seed (bench_gpu_26.3.txt:4)
+
+
+def points_case(dim, k):
+    """Seeds and the 64 shared quart points of the reference's point benchmark (bench_gpu_biomes.cu:84-92):
+    x, z uniform in +-5000 (End +-50000, at least 400 from the origin), y in [-16, 80] (Nether [0, 32],
+    End 16); synthetic draws from splitmix64(k)."""
+    ns = 4096 if dim == "the_end" else 16384
+    d = splitmix64_stream(k, ns + 3 * 64 * 4)
+    seeds = np.array([signed64(v) for v in d[:ns]], np.int64)
+    r = 50000 if dim == "the_end" else 5000
+    pts, j = [], ns
+    while len(pts) < 64:
+        x, z, y = (int(d[j] % (2 * r + 1)) - r, int(d[j + 1] % (2 * r + 1)) - r, d[j + 2])
+        j += 3
+        if dim == "the_end" and x * x + z * z < 400 * 400:
+            continue
+        y = int(y % 97) - 16 if dim == "overworld" else (int(y % 33) if dim == "the_nether" else 16)
+        pts.append((x, y, z))
+    return seeds, np.array(pts, np.int32)
+
+
+def bench_points(args, torch, mcbiome):
+    """The engine's seeds x points path (mcb_points_dev, the mcg_ref_points contract) on the reference's own
+    point benchmark: 16,384 seeds x 64 shared quart points, 26.3 Overworld normal; device-resident inputs,
+    HIP events on the call's stream around seed-state construction and evaluation (the reference times its
+    points kernel, which builds the tables inside).  The other published version/dimension rows and the
+    one-point-per-seed rate are timed the same way and reported beside it."""
+    stream = torch.cuda.current_stream()
+
+    def timed(version, dim, steps, warmup, seeds, pts):
+        eng = mcbiome.Engine(version, dim)
+        d_seeds = torch.from_numpy(seeds).cuda()
+        d_pts = torch.from_numpy(pts.ravel()).cuda()
+        d_out = torch.empty(seeds.size * pts.shape[0], dtype=torch.uint8, device="cuda")
+        call = lambda: eng.points_dev(d_seeds.data_ptr(), seeds.size, d_pts.data_ptr(), pts.shape[0],
+                                      d_out.data_ptr(), 0, stream.cuda_stream)
+        for _ in range(warmup):
+            call()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out = d_out.cpu().numpy()
+        eng.close()
+        return e0.elapsed_time(e1) / steps / 1e3, out
+
+    seeds, pts = points_case("overworld", 7)
+    dt, out = timed("26.3", "overworld", args.steps, max(args.warmup, 1), seeds, pts)
+    value = seeds.size * pts.shape[0] / dt
+    rows = []
+    for (v, d), pub in POINTS_PUBLISHED.items():
+        if (v, d) == ("26.3", "overworld"):
+            a = dt
+        else:
+            s2, p2 = points_case(d, 7)
+            a, _ = timed(v, d, 5, 1, s2, p2)
+            s2n = s2.size
+        n = seeds.size * 64 if (v, d) == ("26.3", "overworld") else s2n * 64
+        rows.append({"version": v, "dimension": d, "points_per_s": round(n / a, 1), "published_points_per_s": pub,
+                     "ratio": round(n / a / pub, 2)})
+    many = np.array([signed64(x) for x in splitmix64_stream(8, 131072)], np.int64)
+    a1, _ = timed("26.3", "overworld", 5, 1, many, pts[:1])
+    rows.append({"version": "26.3", "dimension": "overworld", "seeds_per_s_one_point": round(many.size / a1, 1),
+                 "published_seeds_per_s": SEEDS_1PT_PUBLISHED, "ratio": round(many.size / a1 / SEEDS_1PT_PUBLISHED, 2)})
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        kind = "reference" if oracle.RefEngine.available() else "port"
+        eng = oracle.RefEngine("26.3", "overworld") if kind == "reference" else oracle.Oracle("26.3", "overworld")
+        ncores = min(16, os.cpu_count() or 1)
+        chunks = [seeds[i:i + 64] for i in range(0, 4096, 64)]
+        lock, done = threading.Lock(), [0]
+        t0 = time.perf_counter()
+
+        def worker():
+            while True:
+                with lock:
+                    if not chunks or time.perf_counter() - t0 > 10.0:
+                        return
+                    c = chunks.pop()
+                eng.points(c, pts)
+                with lock:
+                    done[0] += c.size
+        th = [threading.Thread(target=worker) for _ in range(ncores)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        ct = time.perf_counter() - t0
+        cpu = {"value": round(done[0] * 64 / ct, 1), "unit": "points/s", "cores": ncores, "kind": kind,
+               "sample": "%d of the seeds x the same 64 points, 64 seeds per mcg_ref_points call over %d host "
+                         "threads" % (done[0], ncores)}
+    return {"metric": "engine biome points/s (mcb_points_dev, 16,384 seeds x 64 shared quart points)",
+            "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": max(args.warmup, 1), "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+            "scaling": "strong", "vs_baseline": round(value / POINTS_PUBLISHED[("26.3", "overworld")], 2),
+            "baseline_note": "published 4.67e7 points/s on an RTX 4080 SUPER (kernel time, best of the repetitions); "
+                             "value here is the mean over the timed calls, seed-state construction included",
+            "dtype": "f32+f64 (26.3)", "data": "synthetic seeds and points of the reference benchmark's shape",
+            "config": {"workload": "points: 26.3 Overworld normal, 16,384 seeds x 64 quart points (x, z in +-5000, "
+                                   "y in [-16, 80]), one mcb_points_dev call per step"},
+            "checksum": int(out.astype(np.int64).sum()), "table": rows, "cpu_baseline": cpu}
+
+
 def bench_plain(args, torch, mcbiome):
     """One seed (first output of splitmix64(0)), 256x256 quarts at qy 16, origin from the next two draws."""
     d = splitmix64_stream(0, 3)
@@ -603,7 +717,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield", "plugin", "chunks",
-                                                              "mcgen"])
+                                                              "mcgen", "points"])
     ap.add_argument("--search-candidates", type=int, default=1 << 28)
     ap.add_argument("--tile-w", type=int, default=0, help="fixed tile (width | cols_per_lane<<16); 0 = autotune")
     ap.add_argument("--graph", action="store_true", help="replay each plan as a hipGraph")
@@ -632,7 +746,7 @@ def main():
         del src, dst
     fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve,
           "search": bench_search, "endfield": bench_endfield, "plugin": bench_plugin,
-          "chunks": bench_chunks, "mcgen": bench_mcgen}[args.workload]
+          "chunks": bench_chunks, "mcgen": bench_mcgen, "points": bench_points}[args.workload]
     print(json.dumps(fn(args, torch, mcbiome)))
 
 

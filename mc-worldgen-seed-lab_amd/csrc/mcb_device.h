@@ -136,6 +136,18 @@ struct TileP {
     const uint8_t *p;                                   // tile bytes + 4 * lane
     __device__ __forceinline__ int operator[](int i) const { return p[((i & 255) >> 2) * 256 + (i & 3)]; }
 };
+// GP = a 256-byte table in global memory (a seed slot's), masked.
+struct GP {
+    const uint8_t *p;
+    __device__ __forceinline__ int operator[](int i) const { return p[i & 255]; }
+};
+// Permutation sources of the climate columns: the doubled LDS tables (a byte pointer, OCT_BYTES per
+// octave) or a seed slot's own tables in global memory (SlotPerm).
+struct SlotPerm {
+    const uint8_t (*t)[256];
+};
+__device__ __forceinline__ P2 perm_at(const uint8_t *perm2, int o) { return P2{perm2 + o * OCT_BYTES}; }
+__device__ __forceinline__ GP perm_at(SlotPerm s, int o) { return GP{s.t[o]}; }
 
 // gradient table of ImprovedNoise (16 x {gx,gy,gz}), each component a 2-bit two's complement field
 constexpr int kGrad[16][3] = {{1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
@@ -288,12 +300,12 @@ __device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const typename 
 // offset.  26.3 interleaves first/second layers and accumulates in float (NoiseStack.get,
 // mc_noise.h:303-312; legacy-nether order :313-321); 26.1/26.2 sums each stack in double, then
 // (v1+v2)*valueFactor (mc_noise.h:286-301).
-template <int MODE, bool ZC, int NL, int BASE, bool LEGACY>
-__device__ __forceinline__ typename R<MODE>::T normal2(const uint8_t *perm2, const typename GT<MODE>::T *G, const McbOct *oct,
+template <int MODE, bool ZC, int NL, int BASE, bool LEGACY, class PS>
+__device__ __forceinline__ typename R<MODE>::T normal2(PS perm2, const typename GT<MODE>::T *G, const McbOct *oct,
                                                        const McbNoise &N, double X, double Y) {
     auto term = [&](int k, int st) {
         const int o = BASE + st * NL + k;
-        return octave_term<MODE, ZC>(P2{perm2 + o * OCT_BYTES}, G, oct[o], N, k, st, X, Y);
+        return octave_term<MODE, ZC>(perm_at(perm2, o), G, oct[o], N, k, st, X, Y);
     };
     if constexpr (MODE == 1) {
         float v = 0.0f;
@@ -607,8 +619,8 @@ __device__ __forceinline__ void shift_coords(typename R<MODE>::T a, typename R<M
     }
 }
 
-template <int MODE>
-__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, const uint8_t *perm, const typename GT<MODE>::T *G,
+template <int MODE, class PS>
+__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, PS perm, const typename GT<MODE>::T *G,
                                           const SplLds *SP, const McbOct *oct, int qx, int qz, Column &c) {
     typedef typename R<MODE>::T T_;
     const McbNoise *N = W->noise;
@@ -642,8 +654,8 @@ __device__ __forceinline__ int ow_depth(const McbWorld *__restrict__ W, const Co
 }
 
 // Nether climate: two legacy NormalNoises at (bx*0.25, by*0.0, bz*0.25) (engine/cuda/mcg_host.c:63-74)
-template <int MODE>
-__device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, const uint8_t *perm, const typename GT<MODE>::T *G,
+template <int MODE, class PS>
+__device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, PS perm, const typename GT<MODE>::T *G,
                                           const McbOct *oct, int qx, int qz, int *t) {
     const double X = (double)qx, Z = (double)qz;
     // layout MCB_NE_LEVELS {2, 2}, legacy order: bases 0 and 4

@@ -314,38 +314,17 @@ __global__ __launch_bounds__(256) void k_end_field(const McbSlot *__restrict__ s
 }
 
 // ================================================================================================
-// points kernel: seeds x quart points (mcg_ref_points contract, engine/cuda/mcg_host.c:36-83)
+// points kernels: seeds x quart points (mcg_ref_points contract, engine/cuda/mcg_host.c:36-83)
 // ================================================================================================
-template <int MODE, int DIM>
-__global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
-                                                const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
-                                                const McbSlot *__restrict__ slots, const int32_t *__restrict__ slot_of,
-                                                const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
-                                                int32_t *__restrict__ tg) {
-    __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
-    __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
-    const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
-    const int s = blockIdx.y;
-    const int slot = slot_of[s];
-    __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
-    __shared__ SplLds spl[1];
-    SplLds *SP = spl;
-    if (DIM == MCB_END) stage_perm(perm, slots + slot, 1);
-    else stage_perm2(perm, slots + slot, n_oct);
-    init_grad(G);
-    if (DIM == MCB_OVERWORLD) {
-        stage_lut(clut, W);
-        stage_spline(SP, W->spline);
-    }
-    __syncthreads();
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= np) return;
-    const McbOct *oct = slots[slot].oct;
-    const int qx = pts[3 * p], qy = pts[3 * p + 1], qz = pts[3 * p + 2];
-    const size_t o = (size_t)s * np + p;
-    int t[6] = {0, 0, 0, 0, 0, 0};
+// Biome (and targets) of one quart point of one seed.  PS = where the permutation tables are (the
+// doubled LDS copy, or the seed slot itself); End: P = the seed's 256-byte island table.
+template <int MODE, int DIM, class PS>
+__device__ __forceinline__ int point_biome(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                           const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs, PS perm,
+                                           const uint8_t *P, const typename GT<MODE>::T *G, const SplLds *SP,
+                                           const uint32_t *clut, const McbOct *oct, int qx, int qy, int qz, int *t) {
     int b;
-    if (DIM == MCB_OVERWORLD) {
+    if constexpr (DIM == MCB_OVERWORLD) {
         Column c;
         ow_column<MODE>(W, perm, G, SP, oct, qx, qz, c);
         t[0] = c.t; t[1] = c.h; t[2] = c.c; t[3] = c.e; t[5] = c.w;
@@ -361,16 +340,78 @@ __global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, 
             int cand = -1;
             b = tree_search(tree, W->tree_n, t, cand);
         }
-    } else if (DIM == MCB_NETHER) {
+    } else if constexpr (DIM == MCB_NETHER) {
         ne_column<MODE>(W, perm, G, oct, qx, qz, t);
         int cand = -1;
         b = tree_search(tree, W->tree_n, t, cand);
     } else {
-        b = end_biome<MODE>(W, perm, qx, qz);
+        b = end_biome<MODE>(W, P, qx, qz);
     }
+    return b;
+}
+
+// Many points per seed: a workgroup per (seed, 256 points), the seed's tables staged in LDS.
+template <int MODE, int DIM>
+__global__ __launch_bounds__(256) void k_points(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                                const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
+                                                const McbSlot *__restrict__ slots, const int32_t *__restrict__ slot_of,
+                                                const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
+                                                int32_t *__restrict__ tg) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm[(DIM == MCB_OVERWORLD ? MCB_OW_OCT : MCB_NE_OCT) * OCT_BYTES];
+    __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
+    const int n_oct = DIM == MCB_OVERWORLD ? MCB_OW_OCT : (DIM == MCB_NETHER ? MCB_NE_OCT : 1);
+    const int s = blockIdx.y;
+    const int slot = slot_of[s];
+    __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
+    __shared__ SplLds spl[1];
+    if (DIM == MCB_END) stage_perm(perm, slots + slot, 1);
+    else stage_perm2(perm, slots + slot, n_oct);
+    init_grad(G);
+    if (DIM == MCB_OVERWORLD) {
+        stage_lut(clut, W);
+        stage_spline(spl, W->spline);
+    }
+    __syncthreads();
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= np) return;
+    const size_t o = (size_t)s * np + p;
+    int t[6] = {0, 0, 0, 0, 0, 0};
+    const int b = point_biome<MODE, DIM>(W, tree, cells, crecs, (const uint8_t *)perm, perm, G, spl, clut, slots[slot].oct,
+                                         pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], t);
     out[o] = (uint8_t)b;
     if (tg)
         for (int k = 0; k < 6; k++) tg[o * 6 + k] = t[k];
+}
+
+// Few points per seed: a lane per (seed, point), grid-stride; the permutations and octave records are
+// read from the seed slot in global memory (masked), and only the seed-independent tables (gradients,
+// cell locator, spline) are staged, once per workgroup.  Same arithmetic as k_points.
+template <int MODE, int DIM>
+__global__ __launch_bounds__(256) void k_points_g(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
+                                                  const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
+                                                  const McbSlot *__restrict__ slots, const int32_t *__restrict__ slot_of, int ns,
+                                                  const int32_t *__restrict__ pts, int np, uint8_t *__restrict__ out,
+                                                  int32_t *__restrict__ tg) {
+    __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
+    __shared__ uint32_t clut[DIM == MCB_OVERWORLD ? MCB_CELL_LUT : 1];
+    __shared__ SplLds spl[1];
+    init_grad(G);
+    if (DIM == MCB_OVERWORLD) {
+        stage_lut(clut, W);
+        stage_spline(spl, W->spline);
+    }
+    __syncthreads();
+    const int64_t n = (int64_t)ns * np;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const int s = (int)(i / np), p = (int)(i - (int64_t)s * np);
+        const McbSlot *sl = slots + slot_of[s];
+        int t[6] = {0, 0, 0, 0, 0, 0};
+        const int b = point_biome<MODE, DIM>(W, tree, cells, crecs, SlotPerm{sl->perm}, sl->perm[0], G, spl, clut, sl->oct,
+                                             pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], t);
+        out[i] = (uint8_t)b;
+        if (tg)
+            for (int k = 0; k < 6; k++) tg[i * 6 + k] = t[k];
+    }
 }
 
 // ================================================================================================
@@ -470,11 +511,22 @@ hipError_t grid_launch(const DevWorld &w, const McbSlot *slots, const McbRequest
         return grid_launch3<MODE, DIM, 8>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
     }
 }
+// Points per seed from which staging the seed in LDS (k_points) beats reading its slot (k_points_g),
+// measured on MI355X over 16..1024 points per seed: Overworld between 64 and 128 (26.3 and 26.1 alike),
+// Nether between 32 and 64, End between 128 and 256.
+constexpr int points_stage_min(int dim) { return dim == MCB_OVERWORLD ? 128 : (dim == MCB_NETHER ? 32 : 256); }
 template <int MODE, int DIM>
 hipError_t points_launch(const DevWorld &w, const McbSlot *slots, const int32_t *slot_of, int ns, const int32_t *pts, int np,
                          uint8_t *out, int32_t *tg, hipStream_t st) {
-    hipLaunchKernelGGL((k_points<MODE, DIM>), dim3((np + 255) / 256, ns), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
-                       w.d_cell_recs, slots, slot_of, pts, np, out, tg);
+    if (np < points_stage_min(DIM)) {
+        const int64_t n = (int64_t)ns * np;
+        const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL((k_points_g<MODE, DIM>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
+                           w.d_cell_recs, slots, slot_of, ns, pts, np, out, tg);
+    } else {
+        hipLaunchKernelGGL((k_points<MODE, DIM>), dim3((np + 255) / 256, ns), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
+                           w.d_cell_recs, slots, slot_of, pts, np, out, tg);
+    }
     return hipGetLastError();
 }
 

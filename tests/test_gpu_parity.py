@@ -40,6 +40,25 @@ def test_points_match_oracle(version, dim):
 
 
 @pytest.mark.parametrize("version", G.VERSIONS)
+@pytest.mark.parametrize("dim", ["overworld", "the_nether", "the_end"])
+def test_points_few_per_seed(version, dim):
+    """Few points per seed take the kernel that reads each seed's tables from its slot (one lane per seed
+    and point: below 128 points per seed in the Overworld, 32 in the Nether, 256 in the End); more stage
+    the seed in LDS.  Both equal the oracle on the same inputs, on each side of each threshold."""
+    rng = np.random.default_rng(300 + G.VERSIONS.index(version) * 3 + ["overworld", "the_nether", "the_end"].index(dim))
+    seeds = np.array(EDGE_SEEDS + [int(v) for v in rng.integers(-2**63, 2**63 - 1, 56, dtype=np.int64)], np.int64)
+    span = 2**12 if dim == "the_end" else 2**18
+    pts = np.concatenate([_rand_points(rng, 20, span), _rand_points(rng, 20, 2**8), _rand_points(rng, 217, span)])
+    print("few points", version, dim, seeds.size, flush=True)
+    want, want_tg = O.Oracle(version, dim).points(seeds, pts, want_tg=True)
+    eng = mcbiome.Engine(version, dim)
+    for k in (1, 7, 31, 32, 40, 127, 128, 255, 257):
+        got, got_tg = eng.points(seeds, pts[:k], want_tg=True)
+        assert np.array_equal(got_tg, want_tg[:, :k]), "targets differ at %d points per seed" % k
+        assert np.array_equal(got, want[:, :k]), "biomes differ at %d points per seed" % k
+
+
+@pytest.mark.parametrize("version", G.VERSIONS)
 def test_refpoints_dropin(version):
     """mcg_host_load + mcg_ref_points (the reference's CPU-engine entry points) served by the HIP engine."""
     rng = np.random.default_rng(7)
