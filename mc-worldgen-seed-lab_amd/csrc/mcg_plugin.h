@@ -64,6 +64,7 @@ struct PWorld {
     const uint16_t *clist;
     int32_t cnb[6], coff[6];
     int32_t nins, noct, ntree, nslot, spl_depth;
+    int32_t nnoise, nlayers;
     int32_t chain_cap;             // tied cells followed back before a cell is reported unresolved
     int32_t root_slot[6];          // -1: field absent
     int32_t source, fixed_biome, end_ids[5], min_qy, qh;

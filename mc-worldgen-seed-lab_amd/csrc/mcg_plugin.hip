@@ -90,6 +90,10 @@ __device__ __forceinline__ float jsignf(float v) {
 // ---------------------------------------------------------------------------------------------------
 // octaves (mcgpu_dev.cuh:64-155; CPU noise.c)
 // ---------------------------------------------------------------------------------------------------
+// an octave's offsets: the head of its McgOct record, or its copy in LDS
+struct Off3 {
+    double xo, yo, zo;
+};
 // Permutation lookup: D2 = the doubled 512-byte table (the coordinates are reduced mod 256 once, every
 // later index stays below 512, so perm[i & 255] = P[i] without masks); else the 256-byte table, masked.
 template <bool D2> __device__ __forceinline__ int pperm(const uint8_t *P, int i) { return D2 ? (int)P[i] : (int)P[i & 255]; }
@@ -120,7 +124,7 @@ __device__ __forceinline__ float sample_lerp_f(const uint8_t *P, const GradEnt *
 }
 // GradientNoise (26.3+): perlin (kind 0) or smeared (kind 1, y fudge fs)
 template <bool D2>
-__device__ __forceinline__ float octave_f(const uint8_t *P, const GradEnt *G, const McgOct *o, int kind, double fs, double _x,
+__device__ __forceinline__ float octave_f(const uint8_t *P, const GradEnt *G, const Off3 *o, int kind, double fs, double _x,
                                           double _y, double _z) {
     const double x = wrap_new(_x) + o->xo, y = wrap_new(_y) + o->yo, z = wrap_new(_z) + o->zo;
     const int32_t fx = jfloor(x), fy = jfloor(y), fz = jfloor(z);
@@ -135,7 +139,7 @@ __device__ __forceinline__ float octave_f(const uint8_t *P, const GradEnt *G, co
 }
 // ImprovedNoise.noise of the legacy path, coordinates already wrapped (yScale = yFudge = 0*factor)
 template <bool D2>
-__device__ __forceinline__ double octave_d(const uint8_t *P, const GradD *G, const McgOct *o, double _x, double _y, double _z,
+__device__ __forceinline__ double octave_d(const uint8_t *P, const GradD *G, const Off3 *o, double _x, double _y, double _z,
                                            double y_scale, double y_fudge) {
     const double x = _x + o->xo, y = _y + o->yo, z = _z + o->zo;
     const int32_t xf = jfloor(x), yf = jfloor(y), zf = jfloor(z);
@@ -160,22 +164,45 @@ __device__ __forceinline__ double octave_d(const uint8_t *P, const GradD *G, con
 }
 #undef PP
 
-// lds = base of the dynamic LDS (gradient table first, then the doubled permutation tables)
+// lds = base of the dynamic LDS: gradient table, then (PL) the doubled permutation tables and copies of
+// the noise records, layer records and octave offsets; the octave loop reads its uniform records there (a
+// broadcast LDS read) instead of through a chain of global loads.
 template <bool NEWF, bool PL>
 __device__ __forceinline__ const uint8_t *perm_of(const PWorld &W, const uint8_t *lds, int oct) {
     return PL ? lds + grad_bytes<NEWF>() + oct * 512 : W.octs[oct].p;
 }
+__device__ __forceinline__ int rec_base(const PWorld &W, int gbytes) { return gbytes + W.noct * 512; }
+__device__ __forceinline__ int al16(int v) { return (v + 15) & ~15; }
+template <bool NEWF, bool PL>
+__device__ __forceinline__ const McgNoise *noise_of(const PWorld &W, const uint8_t *lds, int i) {
+    return PL ? (const McgNoise *)(lds + rec_base(W, grad_bytes<NEWF>())) + i : W.noise + i;
+}
+template <bool NEWF, bool PL>
+__device__ __forceinline__ const McgLayer *layer_of(const PWorld &W, const uint8_t *lds, int i) {
+    return PL ? (const McgLayer *)(lds + rec_base(W, grad_bytes<NEWF>()) + al16(W.nnoise * (int)sizeof(McgNoise))) + i
+              : W.layers + i;
+}
+template <bool NEWF, bool PL>
+__device__ __forceinline__ const Off3 *off_of(const PWorld &W, const uint8_t *lds, int oct) {
+    return PL ? (const Off3 *)(lds + rec_base(W, grad_bytes<NEWF>()) + al16(W.nnoise * (int)sizeof(McgNoise)) +
+                               al16(W.nlayers * (int)sizeof(McgLayer))) + oct
+              : (const Off3 *)(W.octs + oct);
+}
+// bytes of the staged records after the permutation tables
+__device__ __forceinline__ int rec_bytes(const PWorld &W) {
+    return al16(W.nnoise * (int)sizeof(McgNoise)) + al16(W.nlayers * (int)sizeof(McgLayer)) + al16(W.noct * (int)sizeof(Off3));
+}
 // NoiseStack value (26.3+; mcgpu_dev.cuh:157-168, noise.c:339-347)
 template <bool PL>
 __device__ float ns_get(const PWorld &W, const uint8_t *lp, int nsi, double x, double y, double z) {
-    const McgNoise mn = W.noise[nsi];
+    const McgNoise mn = *noise_of<true, PL>(W, lp, nsi);
     float v = 0.0f;
 #pragma unroll 1
     for (int i = 0; i < mn.n0; i++) {
-        const McgLayer L = W.layers[mn.l0 + i];
+        const McgLayer L = *layer_of<true, PL>(W, lp, mn.l0 + i);
         const double f = L.a;
-        const float g = octave_f<PL>(perm_of<true, PL>(W, lp, L.oct), (const GradEnt *)lp, W.octs + L.oct, L.kind, L.b, x * f,
-                                     y * f, z * f);
+        const float g = octave_f<PL>(perm_of<true, PL>(W, lp, L.oct), (const GradEnt *)lp, off_of<true, PL>(W, lp, L.oct), L.kind,
+                                     L.b, x * f, y * f, z * f);
         v += L.amp_f * g;
     }
     return v;
@@ -186,10 +213,10 @@ __device__ double old_perlin(const PWorld &W, const uint8_t *lp, int l0, int n, 
     double value = 0.0;
 #pragma unroll 1
     for (int i = 0; i < n; i++) {
-        const McgLayer L = W.layers[l0 + i];
+        const McgLayer L = *layer_of<false, PL>(W, lp, l0 + i);
         if (L.oct >= 0) {
             const double factor = L.a;
-            const double nv = octave_d<PL>(perm_of<false, PL>(W, lp, L.oct), (const GradD *)lp, W.octs + L.oct,
+            const double nv = octave_d<PL>(perm_of<false, PL>(W, lp, L.oct), (const GradD *)lp, off_of<false, PL>(W, lp, L.oct),
                                            wrap_old(x * factor), wrap_old(y * factor),
                                        wrap_old(z * factor), 0.0 * factor, 0.0 * factor);
             value += L.amp_d * nv * L.b;
@@ -199,7 +226,7 @@ __device__ double old_perlin(const PWorld &W, const uint8_t *lp, int l0, int n, 
 }
 template <bool PL>
 __device__ double old_normal(const PWorld &W, const uint8_t *lp, int nsi, double x, double y, double z) {
-    const McgNoise mn = W.noise[nsi];
+    const McgNoise mn = *noise_of<false, PL>(W, lp, nsi);
     const double x2 = x * 1.0181268882175227, y2 = y * 1.0181268882175227, z2 = z * 1.0181268882175227;
     return (old_perlin<PL>(W, lp, mn.l0, mn.n0, x, y, z) + old_perlin<PL>(W, lp, mn.l1, mn.n1, x2, y2, z2)) *
            mn.value_factor;
@@ -665,11 +692,18 @@ __global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
     typedef typename VT_of<NEWF>::T VT;
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int gbytes = grad_bytes<NEWF>();
-    const int pbytes = PL ? W.noct * 512 : 0;
+    const int pbytes = PL ? W.noct * 512 + rec_bytes(W) : 0;
     dev::init_grad((typename GradOf<NEWF>::T *)lds);
     if (PL) {
         uint32_t *dst = (uint32_t *)(lds + gbytes);     // word w of octave o: table bytes 4(w & 63).., twice
         for (int i = threadIdx.x; i < W.noct * 128; i += kWG) dst[i] = ((const uint32_t *)W.octs[i >> 7].p)[i & 63];
+        // records: noise and layers copied word by word, octave offsets from the head of each McgOct
+        uint32_t *rn = (uint32_t *)noise_of<NEWF, true>(W, lds, 0), *rl = (uint32_t *)layer_of<NEWF, true>(W, lds, 0);
+        const int nn = W.nnoise * (int)sizeof(McgNoise) / 4, nl = W.nlayers * (int)sizeof(McgLayer) / 4;
+        for (int i = threadIdx.x; i < nn; i += kWG) rn[i] = ((const uint32_t *)W.noise)[i];
+        for (int i = threadIdx.x; i < nl; i += kWG) rl[i] = ((const uint32_t *)W.layers)[i];
+        Off3 *ro = (Off3 *)off_of<NEWF, true>(W, lds, 0);
+        for (int i = threadIdx.x; i < W.noct; i += kWG) ro[i] = Off3{W.octs[i].xo, W.octs[i].yo, W.octs[i].zo};
     }
     __syncthreads();
     VT *vals = (VT *)(lds + gbytes + pbytes) + threadIdx.x;
