@@ -849,7 +849,26 @@ __device__ __forceinline__ double zfiddle(int64_t r) { return ((double)((r >> 24
 
 // Quart cell BiomeManager.getBiome(x, y, z) reads for block (x, y, z) under hashed seed hs: the lattice
 // corner of least fiddled distance, first minimum in corner order (engine/cuda/mc_gpu.cuh:96-118).
-__device__ __forceinline__ void zoom_quart(int64_t hs, int x, int y, int z, int *q) {
+// The three offsets of lattice point (cx, cy, cz): they depend only on the point and the hashed seed, so
+// kernels over grids compute each once and share it between the blocks around the point.
+struct ZFid {
+    double x, y, z;
+};
+__device__ __forceinline__ ZFid zoom_fiddles(int64_t hs, int64_t cx, int64_t cy, int64_t cz) {
+    int64_t r = hs;
+    r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
+    r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
+    ZFid f;
+    f.x = zfiddle(r);
+    r = zstep(r, hs);
+    f.y = zfiddle(r);
+    r = zstep(r, hs);
+    f.z = zfiddle(r);
+    return f;
+}
+// The corner pick for block (x, y, z) with corner c's offsets from fid(c) (c = ux << 2 | uy << 1 | uz).
+template <class Fid>
+__device__ __forceinline__ void zoom_pick(int x, int y, int z, Fid fid, int *q) {
     const int ax = x - 2, ay = y - 2, az = z - 2;
     const int px = ax >> 2, py = ay >> 2, pz = az >> 2;
     const double fx = (double)(ax & 3) / 4.0, fy = (double)(ay & 3) / 4.0, fz = (double)(az & 3) / 4.0;
@@ -857,22 +876,17 @@ __device__ __forceinline__ void zoom_quart(int64_t hs, int x, int y, int z, int 
     double bd = __builtin_inf();
     for (int c = 0; c < 8; c++) {
         const int ux = (c >> 2) & 1, uy = (c >> 1) & 1, uz = c & 1;
-        const int64_t cx = px + ux, cy = py + uy, cz = pz + uz;
         const double dx = ux ? fx - 1.0 : fx, dy = uy ? fy - 1.0 : fy, dz = uz ? fz - 1.0 : fz;
-        int64_t r = hs;
-        r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
-        r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
-        const double ox = zfiddle(r);
-        r = zstep(r, hs);
-        const double oy = zfiddle(r);
-        r = zstep(r, hs);
-        const double oz = zfiddle(r);
-        const double d = (dz + oz) * (dz + oz) + (dy + oy) * (dy + oy) + (dx + ox) * (dx + ox);
+        const ZFid o = fid(c, px + ux, py + uy, pz + uz);
+        const double d = (dz + o.z) * (dz + o.z) + (dy + o.y) * (dy + o.y) + (dx + o.x) * (dx + o.x);
         if (bd > d) { best = c; bd = d; }
     }
     q[0] = px + ((best >> 2) & 1);
     q[1] = py + ((best >> 1) & 1);
     q[2] = pz + (best & 1);
+}
+__device__ __forceinline__ void zoom_quart(int64_t hs, int x, int y, int z, int *q) {
+    zoom_pick(x, y, z, [&](int, int cx, int cy, int cz) { return zoom_fiddles(hs, cx, cy, cz); }, q);
 }
 
 // Biomes of every leaf at the least distance (all ties), by a walk that enters a subtree when its

@@ -754,15 +754,44 @@ __global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
 }
 
 // block grid read from a lattice of cells (MULTI) or chunks (END)
-__global__ __launch_bounds__(kWG) void k_zoom_lattice(PWorld W, PJob L, int x0, int z0, int nx, int64_t n, int step, int y,
-                                                     uint8_t *out, int32_t *nbad) {
-    for (int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x; it - (int64_t)threadIdx.x < n; it += (int64_t)gridDim.x * kWG) {
-        const bool live = it < n;
+// Grid zoom by tiles of 64 x 4 samples: the workgroup first computes the offsets of every lattice point
+// its samples' corners touch (each once, into LDS), then each lane picks its corner from them (the same
+// operations in the same order as the per-block form).  A tile whose lattice exceeds kZoomPts (large
+// steps) computes each block's corners directly.
+constexpr int kZoomTX = 64, kZoomTZ = 4, kZoomPts = 1024;
+__global__ __launch_bounds__(kWG) void k_zoom_tiles(PWorld W, PJob L, int x0, int z0, int nx, int nz, int step, int y,
+                                                    uint8_t *out, int32_t *nbad) {
+    __shared__ dev::ZFid F[kZoomPts];
+    const int ntx = (nx + kZoomTX - 1) / kZoomTX;
+    const int64_t ntiles = (int64_t)ntx * ((nz + kZoomTZ - 1) / kZoomTZ);
+    for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int ix0 = (int)(t % ntx) * kZoomTX, iz0 = (int)(t / ntx) * kZoomTZ;
+        const int ix1 = min(ix0 + kZoomTX, nx) - 1, iz1 = min(iz0 + kZoomTZ, nz) - 1;
+        // lattice box of the tile: corners px..px+1 of every block, px = (x - 2) >> 2 (step > 0)
+        const int pxl = (x0 + ix0 * step - 2) >> 2, pxh = ((x0 + ix1 * step - 2) >> 2) + 1;
+        const int pzl = (z0 + iz0 * step - 2) >> 2, pzh = ((z0 + iz1 * step - 2) >> 2) + 1;
+        const int pyl = (y - 2) >> 2;
+        const int npx = pxh - pxl + 1, npz = pzh - pzl + 1;
+        const bool shared = (int64_t)npx * npz * 2 <= kZoomPts;
+        __syncthreads();                                  // the previous tile's readers are done
+        if (shared)
+            for (int i = threadIdx.x; i < npx * npz * 2; i += kWG) {
+                const int u = i % npx, v = (i / npx) % npz, w = i / (npx * npz);
+                F[i] = dev::zoom_fiddles(W.zoom_seed, pxl + u, pyl + w, pzl + v);
+            }
+        __syncthreads();
+        const int ix = ix0 + (int)threadIdx.x % kZoomTX, iz = iz0 + (int)threadIdx.x / kZoomTX;
+        const bool live = ix <= ix1 && iz <= iz1;
         bool bad = false;
         if (live) {
-            const int x = x0 + (int)(it % nx) * step, z = z0 + (int)(it / nx) * step;
+            const int x = x0 + ix * step, z = z0 + iz * step;
             int q[3];
-            dev::zoom_quart(W.zoom_seed, x, y, z, q);
+            if (shared)
+                dev::zoom_pick(x, y, z, [&](int, int cx, int cy, int cz) {
+                    return F[((cy - pyl) * npz + (cz - pzl)) * npx + (cx - pxl)];
+                }, q);
+            else
+                dev::zoom_quart(W.zoom_seed, x, y, z, q);
             const int qy = clamp_qy(W, q[1]);
             const int layer = qy == L.qys[0] ? 0 : 1;
             // the lattice covers every cell (chunk) the zoom can pick; the range check only guards it
@@ -774,7 +803,7 @@ __global__ __launch_bounds__(kWG) void k_zoom_lattice(PWorld W, PJob L, int x0, 
                 b = L.out[((int64_t)layer * L.nqz + (az - L.q0z)) * L.nqx + (ax - L.q0x)];
             }
             bad = b == 255;
-            out[it] = (uint8_t)b;
+            out[(int64_t)iz * nx + ix] = (uint8_t)b;
         }
         count_bad(nbad, bad);
     }
@@ -802,7 +831,9 @@ hipError_t launch_zoom_lattice(const PWorld &W, const PJob &L, int x0, int z0, i
                                int32_t *nbad, hipStream_t st) {
     const int64_t n = (int64_t)nx * nz;
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_zoom_lattice, dim3(grid_blocks(n)), dim3(kWG), 0, st, W, L, x0, z0, nx, n, step, y, out, nbad);
+    const int64_t ntiles = (int64_t)((nx + kZoomTX - 1) / kZoomTX) * ((nz + kZoomTZ - 1) / kZoomTZ);
+    const int blocks = (int)(ntiles < 8192 ? ntiles : 8192);
+    hipLaunchKernelGGL(k_zoom_tiles, dim3(blocks), dim3(kWG), 0, st, W, L, x0, z0, nx, nz, step, y, out, nbad);
     return hipGetLastError();
 }
 
