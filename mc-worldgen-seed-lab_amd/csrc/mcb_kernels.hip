@@ -448,40 +448,36 @@ __global__ __launch_bounds__(256) void k_targets(const McbWorld *__restrict__ W,
 __global__ __launch_bounds__(256) void k_zoom(const McbRequest *__restrict__ zreqs, const McbRequest *__restrict__ qreqs,
                                               const McbTile *__restrict__ tiles, int ntiles, const uint8_t *__restrict__ quart,
                                               uint8_t *__restrict__ out) {
+    // offsets of the tile's lattice points, per layer: a 16 x 16 tile spans at most 6 x 6 corners in x, z
+    __shared__ ZFid F[8 * 2 * 6 * 6];
     for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
         const McbTile tl = tiles[ti];
         const McbRequest &Z = zreqs[tl.req];
         const McbRequest &Q = qreqs[tl.req];
-        const int ix = tl.tx * 16 + (int)threadIdx.x % 16, iz = tl.tz * 16 + (int)threadIdx.x / 16;
-        if (ix >= Z.nx || iz >= Z.nz) continue;
+        const int hx0 = tl.tx * 16, hz0 = tl.tz * 16;
+        const int hx1 = min(hx0 + 16, Z.nx) - 1, hz1 = min(hz0 + 16, Z.nz) - 1;
+        const int pxl = (Z.x0 + hx0 - 2) >> 2, npx = ((Z.x0 + hx1 - 2) >> 2) + 2 - pxl;
+        const int pzl = (Z.z0 + hz0 - 2) >> 2, npz = ((Z.z0 + hz1 - 2) >> 2) + 2 - pzl;
+        const int npl = npx * npz;
         const int64_t hs = Z.hashed_seed;
+        __syncthreads();                                  // the previous tile's readers are done
+        for (int i = threadIdx.x; i < Z.ny * 2 * npl; i += 256) {
+            const int u = i % npx, v = (i / npx) % npz, w = i / npl;      // w = 2 * layer + corner y
+            F[i] = zoom_fiddles(hs, pxl + u, ((Z.y[w >> 1] - 2) >> 2) + (w & 1), pzl + v);
+        }
+        __syncthreads();
+        const int ix = hx0 + (int)threadIdx.x % 16, iz = hz0 + (int)threadIdx.x / 16;
+        if (ix > hx1 || iz > hz1) continue;
         const int x = Z.x0 + ix, z = Z.z0 + iz;
-        const int ax = x - 2, az = z - 2;
-        const int px = ax >> 2, pz = az >> 2;
-        const double fx = (double)(ax & 3) / 4.0, fz = (double)(az & 3) / 4.0;
         const int64_t qplane = (int64_t)Q.nx * Q.nz;
         for (int j = 0; j < Z.ny; j++) {
-            const int ay = Z.y[j] - 2, py = ay >> 2;
-            const double fy = (double)(ay & 3) / 4.0;
-            int best = 0;
-            double bd = __builtin_inf();
-            for (int c = 0; c < 8; c++) {
-                const int ux = (c >> 2) & 1, uy = (c >> 1) & 1, uz = c & 1;
-                const int64_t cx = px + ux, cy = py + uy, cz = pz + uz;
-                double dx = ux ? fx - 1.0 : fx, dy = uy ? fy - 1.0 : fy, dz = uz ? fz - 1.0 : fz;
-                int64_t r = hs;
-                r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
-                r = zstep(r, cx); r = zstep(r, cy); r = zstep(r, cz);
-                double ox = zfiddle(r);
-                r = zstep(r, hs);
-                double oy = zfiddle(r);
-                r = zstep(r, hs);
-                double oz = zfiddle(r);
-                double d = (dz + oz) * (dz + oz) + (dy + oy) * (dy + oy) + (dx + ox) * (dx + ox);
-                if (bd > d) { best = c; bd = d; }
-            }
-            const int qxi = px + ((best >> 2) & 1) - Q.x0, qzi = pz + (best & 1) - Q.z0;
-            const int layer = 2 * j + ((best >> 1) & 1);
+            const int py = (Z.y[j] - 2) >> 2;
+            int q[3];
+            zoom_pick(x, Z.y[j], z, [&](int, int cx, int cy, int cz) {
+                return F[((2 * j + (cy - py)) * npz + (cz - pzl)) * npx + (cx - pxl)];
+            }, q);
+            const int qxi = q[0] - Q.x0, qzi = q[2] - Q.z0;
+            const int layer = 2 * j + (q[1] - py);
             out[Z.out_off + j * (int64_t)Z.nx * Z.nz + (int64_t)iz * Z.nx + ix] =
                 quart[Q.out_off + layer * qplane + (int64_t)qzi * Q.nx + qxi];
         }
