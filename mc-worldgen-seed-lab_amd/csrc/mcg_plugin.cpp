@@ -11,6 +11,7 @@
 //    instead of evaluating a cell per output point; per point when the grid step makes that cheaper;
 //  * device buffers of a world persist between calls and grow on demand.
 #include <algorithm>
+#include <map>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -386,6 +387,51 @@ bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
     return true;
 }
 
+// The tree and its cell table depend only on the tree, not on the seed: one device copy per (device,
+// table) serves every world built over it for the rest of the process (a world per call, as the chunk
+// path makes, would otherwise upload several megabytes each time).  Guarded by g_mu.
+struct SharedTables {
+    char *d = nullptr;
+    size_t ntree = 0;
+    const PNode *tree = nullptr;
+    const int32_t *cbp = nullptr;
+    const uint32_t *centry = nullptr;
+    const uint16_t *clist = nullptr;
+};
+// null with *failed false: not shared (the world uploads its own copy); *failed: a device error
+const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
+    *failed = false;
+    static std::map<std::pair<int, const mcb::CellTable *>, SharedTables> cache;
+    const auto key = std::make_pair(g_dev, H.cells.get());
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second.ntree == H.tree.size() ? &it->second : nullptr;
+    const mcb::CellTable &ct = *H.cells;
+    const size_t b[4] = {H.tree.size() * sizeof(PNode), ct.bp.size() * sizeof(int32_t), ct.entry.size() * sizeof(uint32_t),
+                         ct.list.size() * sizeof(uint16_t)};
+    const void *src[4] = {H.tree.data(), ct.bp.data(), ct.entry.data(), ct.list.data()};
+    size_t off[4], total = 0;
+    for (int i = 0; i < 4; i++) { off[i] = total; total += (b[i] + 255) & ~(size_t)255; }
+    SharedTables s;
+    if (hipMalloc((void **)&s.d, std::max<size_t>(total, 256)) != hipSuccess) {
+        set_error("shared table allocation failed");
+        *failed = true;
+        return nullptr;
+    }
+    for (int i = 0; i < 4; i++)
+        if (b[i] && hipMemcpy(s.d + off[i], src[i], b[i], hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(s.d);
+            set_error("shared table upload failed");
+            *failed = true;
+            return nullptr;
+        }
+    s.ntree = H.tree.size();
+    s.tree = (const PNode *)(s.d + off[0]);
+    s.cbp = (const int32_t *)(s.d + off[1]);
+    s.centry = (const uint32_t *)(s.d + off[2]);
+    s.clist = (const uint16_t *)(s.d + off[3]);
+    return &(cache[key] = s);
+}
+
 // device half: buffers, stream, uploads
 bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     memset(&g->W, 0, sizeof g->W);
@@ -413,7 +459,11 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
                            al16((int64_t)p->nlayers * (int64_t)sizeof(McgLayer)) + al16((int64_t)p->noct * 24);
     g->perm_lds = gbytes + vbytes + pbytes <= kMaxLdsBytes;
     g->lds = gbytes + vbytes + (g->perm_lds ? (int)pbytes : 0);
-    // one device allocation: ins, noise, layers, octs, sp, splf, spc, arr, thr, tree
+    // one device allocation: ins, noise, layers, octs, sp, splf, spc, arr, thr, tree (the tree and cell
+    // table come from the shared copy when the tree has a cell table)
+    bool failed = false;
+    const SharedTables *sh = H.cells ? shared_tables(H, &failed) : nullptr;
+    if (failed) return false;
     struct Part { const void *src; size_t bytes; size_t off; };
     std::vector<Part> parts = {
         {C.ins.data(), C.ins.size() * sizeof(PIns), 0},          {p->noise, (size_t)p->nnoise * sizeof(McgNoise), 0},
@@ -425,6 +475,8 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
         {H.cells ? H.cells->entry.data() : nullptr, H.cells ? H.cells->entry.size() * sizeof(uint32_t) : 0, 0},
         {H.cells ? H.cells->list.data() : nullptr, H.cells ? H.cells->list.size() * sizeof(uint16_t) : 0, 0}};
     size_t total = 0;
+    if (sh)
+        for (int i = 9; i < 13; i++) parts[i].bytes = 0;
     for (auto &q : parts) { q.off = total; total += (q.bytes + 255) & ~(size_t)255; }
     HCHK(hipMalloc((void **)&g->d_data, std::max<size_t>(total, 256)));
     for (auto &q : parts)
@@ -439,11 +491,11 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     P.spc = (const int32_t *)at(6);
     P.arr = (const int32_t *)at(7);
     P.thr = (const double *)at(8);
-    P.tree = (const PNode *)at(9);
+    P.tree = sh ? sh->tree : (const PNode *)at(9);
     if (H.cells) {
-        P.cbp = (const int32_t *)at(10);
-        P.centry = (const uint32_t *)at(11);
-        P.clist = (const uint16_t *)at(12);
+        P.cbp = sh ? sh->cbp : (const int32_t *)at(10);
+        P.centry = sh ? sh->centry : (const uint32_t *)at(11);
+        P.clist = sh ? sh->clist : (const uint16_t *)at(12);
         for (int k = 0; k < 6; k++) { P.cnb[k] = H.cells->nb[k]; P.coff[k] = H.cells->off[k]; }
     }
     P.nins = (int32_t)C.ins.size();
