@@ -137,10 +137,15 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
     }
     const int tile_h = 256 / tile_w;
     int staged = -1;
-    // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables
-    const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t_end = min(ntiles, ((int)blockIdx.x + 1) * per);
-    for (int ti = (int)blockIdx.x * per; ti < t_end; ti++) {
+    // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables.
+    // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): block b runs on XCD
+    // b % 8.  Ranking the blocks XCD by XCD gives each XCD a contiguous run of tiles, so the tiles of one
+    // seed share an L2 and all but the first stage its state from there (a bijection of the blocks).
+    const int nwg = (int)gridDim.x;
+    const int vb = nwg % 8 == 0 ? ((int)blockIdx.x % 8) * (nwg / 8) + (int)blockIdx.x / 8 : (int)blockIdx.x;
+    const int per = (ntiles + nwg - 1) / nwg;
+    const int t_end = min(ntiles, (vb + 1) * per);
+    for (int ti = vb * per; ti < t_end; ti++) {
         const McbTile tl = tiles[ti];
         const McbRequest &R = reqs[tl.req];
         const int slot = R.slot;
