@@ -849,6 +849,11 @@ __device__ __forceinline__ double zfiddle(int64_t r) { return ((double)((r >> 24
 
 // Quart cell BiomeManager.getBiome(x, y, z) reads for block (x, y, z) under hashed seed hs: the lattice
 // corner of least fiddled distance, first minimum in corner order (engine/cuda/mc_gpu.cuh:96-118).
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): block b runs on XCD
+// b % 8.  xcd_rank ranks the blocks XCD by XCD, so consecutive ranks (tiles of one seed, neighbouring
+// rows of one request) run on one XCD and share its L2; a bijection of [0, nwg).
+__device__ __forceinline__ int xcd_rank(int b, int nwg) { return nwg % 8 == 0 ? (b % 8) * (nwg / 8) + b / 8 : b; }
+
 // The three offsets of lattice point (cx, cy, cz): they depend only on the point and the hashed seed, so
 // kernels over grids compute each once and share it between the blocks around the point.
 struct ZFid {

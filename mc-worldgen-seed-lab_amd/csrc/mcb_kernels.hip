@@ -137,12 +137,10 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
     }
     const int tile_h = 256 / tile_w;
     int staged = -1;
-    // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables.
-    // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2): block b runs on XCD
-    // b % 8.  Ranking the blocks XCD by XCD gives each XCD a contiguous run of tiles, so the tiles of one
-    // seed share an L2 and all but the first stage its state from there (a bijection of the blocks).
+    // contiguous tile range per workgroup, so consecutive tiles of one seed reuse the staged tables; the
+    // XCD-aware rank puts the tiles of one seed on one L2, so all but the first stage its state from there
     const int nwg = (int)gridDim.x;
-    const int vb = nwg % 8 == 0 ? ((int)blockIdx.x % 8) * (nwg / 8) + (int)blockIdx.x / 8 : (int)blockIdx.x;
+    const int vb = xcd_rank((int)blockIdx.x, nwg);
     const int per = (ntiles + nwg - 1) / nwg;
     const int t_end = min(ntiles, (vb + 1) * per);
     for (int ti = vb * per; ti < t_end; ti++) {
@@ -211,9 +209,10 @@ __global__ __launch_bounds__(256) void k_grid_end(const McbWorld *__restrict__ W
     __shared__ int isl_n;
     const int tile_h = 256 / tile_w;
     int staged = -1;
+    const int vb = xcd_rank((int)blockIdx.x, (int)gridDim.x);      // tiles of one seed on one L2
     const int per = (ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int t_end = min(ntiles, ((int)blockIdx.x + 1) * per);
-    for (int ti = (int)blockIdx.x * per; ti < t_end; ti++) {
+    const int t_end = min(ntiles, (vb + 1) * per);
+    for (int ti = vb * per; ti < t_end; ti++) {
         const McbTile tl = tiles[ti];
         const McbRequest &R = reqs[tl.req];
         __syncthreads();                                  // previous tile's list fully consumed
@@ -455,7 +454,8 @@ __global__ __launch_bounds__(256) void k_zoom(const McbRequest *__restrict__ zre
                                               uint8_t *__restrict__ out) {
     // offsets of the tile's lattice points, per layer: a 16 x 16 tile spans at most 6 x 6 corners in x, z
     __shared__ ZFid F[8 * 2 * 6 * 6];
-    for (int ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
+    // neighbouring tiles of a request (rows of one output line) on one L2
+    for (int ti = xcd_rank((int)blockIdx.x, (int)gridDim.x); ti < ntiles; ti += gridDim.x) {
         const McbTile tl = tiles[ti];
         const McbRequest &Z = zreqs[tl.req];
         const McbRequest &Q = qreqs[tl.req];
