@@ -451,11 +451,11 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     if (W->source == MCG_BS_FIXED) return true;
     const McgProg *p = H.p;
     const Compiled &C = H.C;
-    // LDS: gradient table and value slots always; the doubled permutation tables with copies of the noise
+    // LDS: gradient table and value slots always; the permutation tables with copies of the noise
     // records, layer records and octave offsets when they fit beside them
     const int gbytes = grad_lds_bytes(g->newf), vbytes = C.nslot * kWG * (g->newf ? 4 : 8);
     auto al16 = [](int64_t v) { return (v + 15) & ~(int64_t)15; };
-    const int64_t pbytes = (int64_t)p->noct * 512 + al16((int64_t)p->nnoise * (int64_t)sizeof(McgNoise)) +
+    const int64_t pbytes = (int64_t)p->noct * (kPermDoubled ? 512 : 256) + al16((int64_t)p->nnoise * (int64_t)sizeof(McgNoise)) +
                            al16((int64_t)p->nlayers * (int64_t)sizeof(McgLayer)) + al16((int64_t)p->noct * 24);
     g->perm_lds = gbytes + vbytes + pbytes <= kMaxLdsBytes;
     g->lds = gbytes + vbytes + (g->perm_lds ? (int)pbytes : 0);

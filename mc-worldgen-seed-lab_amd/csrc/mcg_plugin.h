@@ -21,6 +21,10 @@ constexpr int kWG = 256;                 // lanes per workgroup of the plugin ke
 constexpr int kChainCap = 12;            // tied cells followed back along the fill order (reference: 12)
 constexpr int kSplineDepth = 16;         // spline nesting accepted (the reference's frame stack holds 16)
 constexpr int kMaxLdsBytes = 64 * 1024;  // LDS per workgroup the kernels may use
+// Staged permutation tables: single (256 B, masked indices) or doubled (512 B, no masks).  Single measured
+// faster on MI355X (26.3 grid 3.43 -> 3.25 ms, 26.1/26.2 2048^2 step-4 grids 4.4 -> 3.8 ms): the smaller LDS
+// footprint fits one more workgroup per CU, which outweighs the masks in this latency-bound kernel.
+constexpr bool kPermDoubled = false;
 // LDS gradient table of k_cells: 256 entries of 8 bytes (float noise) or 16 bytes (double noise)
 constexpr int grad_lds_bytes(bool newf) { return newf ? 256 * 8 : 256 * 16; }
 

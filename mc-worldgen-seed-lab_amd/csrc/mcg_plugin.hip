@@ -10,10 +10,10 @@
 //
 // MI355X layout: the compiled program (PIns) is read with scalar loads, every lane running the same
 // instruction; live values sit in LDS slots, column per lane (slot s of lane l at s*256 + l, no bank
-// conflicts); the octave permutation tables are staged in LDS once per workgroup when they fit, each
-// twice in a row (512 bytes) so the lattice hash needs no index masks, and a gradient table per
-// permutation value (the engine's: f32/f16 entries for float noise, 16-byte double entries for the
-// legacy path) replaces the bit-field extraction of every corner's gradient.
+// conflicts); the octave permutation tables are staged in LDS once per workgroup when they fit
+// (kPermDoubled chooses single or doubled tables), and a gradient table per permutation value (the
+// engine's: f32/f16 entries for float noise, 16-byte double entries for the legacy path) replaces the
+// bit-field extraction of every corner's gradient.
 #include "mcg_plugin.h"
 #include "mcb_device.h"
 
@@ -25,7 +25,7 @@ using dev::GradEnt;
 using dev::gdot_d;
 using dev::gdot_f;
 
-// dynamic LDS layout of k_cells: gradient table at 0, then the permutation tables (doubled, when staged),
+// dynamic LDS layout of k_cells: gradient table at 0, then the permutation tables (when staged),
 // then the value slots
 template <bool NEWF> struct GradOf { typedef GradEnt T; };
 template <> struct GradOf<false> { typedef GradD T; };
@@ -164,14 +164,16 @@ __device__ __forceinline__ double octave_d(const uint8_t *P, const GradD *G, con
 }
 #undef PP
 
-// lds = base of the dynamic LDS: gradient table, then (PL) the doubled permutation tables and copies of
+// bytes per staged permutation table: doubled (no index masks) or single (masked, a smaller LDS footprint)
+constexpr int kPermStride = kPermDoubled ? 512 : 256;
+// lds = base of the dynamic LDS: gradient table, then (PL) the permutation tables and copies of
 // the noise records, layer records and octave offsets; the octave loop reads its uniform records there (a
 // broadcast LDS read) instead of through a chain of global loads.
 template <bool NEWF, bool PL>
 __device__ __forceinline__ const uint8_t *perm_of(const PWorld &W, const uint8_t *lds, int oct) {
-    return PL ? lds + grad_bytes<NEWF>() + oct * 512 : W.octs[oct].p;
+    return PL ? lds + grad_bytes<NEWF>() + oct * kPermStride : W.octs[oct].p;
 }
-__device__ __forceinline__ int rec_base(const PWorld &W, int gbytes) { return gbytes + W.noct * 512; }
+__device__ __forceinline__ int rec_base(const PWorld &W, int gbytes) { return gbytes + W.noct * kPermStride; }
 __device__ __forceinline__ int al16(int v) { return (v + 15) & ~15; }
 template <bool NEWF, bool PL>
 __device__ __forceinline__ const McgNoise *noise_of(const PWorld &W, const uint8_t *lds, int i) {
@@ -201,7 +203,7 @@ __device__ float ns_get(const PWorld &W, const uint8_t *lp, int nsi, double x, d
     for (int i = 0; i < mn.n0; i++) {
         const McgLayer L = *layer_of<true, PL>(W, lp, mn.l0 + i);
         const double f = L.a;
-        const float g = octave_f<PL>(perm_of<true, PL>(W, lp, L.oct), (const GradEnt *)lp, off_of<true, PL>(W, lp, L.oct), L.kind,
+        const float g = octave_f<PL && kPermDoubled>(perm_of<true, PL>(W, lp, L.oct), (const GradEnt *)lp, off_of<true, PL>(W, lp, L.oct), L.kind,
                                      L.b, x * f, y * f, z * f);
         v += L.amp_f * g;
     }
@@ -216,7 +218,7 @@ __device__ double old_perlin(const PWorld &W, const uint8_t *lp, int l0, int n, 
         const McgLayer L = *layer_of<false, PL>(W, lp, l0 + i);
         if (L.oct >= 0) {
             const double factor = L.a;
-            const double nv = octave_d<PL>(perm_of<false, PL>(W, lp, L.oct), (const GradD *)lp, off_of<false, PL>(W, lp, L.oct),
+            const double nv = octave_d<PL && kPermDoubled>(perm_of<false, PL>(W, lp, L.oct), (const GradD *)lp, off_of<false, PL>(W, lp, L.oct),
                                            wrap_old(x * factor), wrap_old(y * factor),
                                        wrap_old(z * factor), 0.0 * factor, 0.0 * factor);
             value += L.amp_d * nv * L.b;
@@ -692,11 +694,12 @@ __global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
     typedef typename VT_of<NEWF>::T VT;
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int gbytes = grad_bytes<NEWF>();
-    const int pbytes = PL ? W.noct * 512 + rec_bytes(W) : 0;
+    const int pbytes = PL ? W.noct * kPermStride + rec_bytes(W) : 0;
     dev::init_grad((typename GradOf<NEWF>::T *)lds);
     if (PL) {
         uint32_t *dst = (uint32_t *)(lds + gbytes);     // word w of octave o: table bytes 4(w & 63).., twice
-        for (int i = threadIdx.x; i < W.noct * 128; i += kWG) dst[i] = ((const uint32_t *)W.octs[i >> 7].p)[i & 63];
+        const int wpo = kPermStride / 4;                 // words per staged octave
+        for (int i = threadIdx.x; i < W.noct * wpo; i += kWG) dst[i] = ((const uint32_t *)W.octs[i / wpo].p)[i & 63];
         // records: noise and layers copied word by word, octave offsets from the head of each McgOct
         uint32_t *rn = (uint32_t *)noise_of<NEWF, true>(W, lds, 0), *rl = (uint32_t *)layer_of<NEWF, true>(W, lds, 0);
         const int nn = W.nnoise * (int)sizeof(McgNoise) / 4, nl = W.nlayers * (int)sizeof(McgLayer) / 4;
