@@ -360,6 +360,12 @@ int mcb_server_create(McbServer **out, McbContext *ctx, int64_t batch_bytes, int
         MCB_HIPCHK(hipMalloc((void **)&L.d_desc, (size_t)L.desc_cap));
         MCB_HIPCHK(hipHostMalloc((void **)&L.h_out, (size_t)batch_bytes, hipHostMallocDefault));
         MCB_HIPCHK(hipHostMalloc((void **)&L.h_desc, (size_t)L.desc_cap, hipHostMallocDefault));
+        // the runtime sets up a stream's queues on its first commands: a fill and both copy directions now,
+        // so that no request pays for them
+        MCB_HIPCHK(hipMemsetAsync(L.d_out, 0, 256, L.st));
+        MCB_HIPCHK(hipMemcpyAsync(L.d_desc, L.h_desc, 256, hipMemcpyHostToDevice, L.st));
+        MCB_HIPCHK(hipMemcpyAsync(L.h_out, L.d_out, 256, hipMemcpyDeviceToHost, L.st));
+        MCB_HIPCHK(hipStreamSynchronize(L.st));
     }
     McbServer *s = S.release();
     s->dispatcher = std::thread(dispatcher_main, s);
