@@ -330,6 +330,17 @@ __device__ __forceinline__ float dist_value(int m, float dx, float dy, float dz)
     }
 }
 
+// Header fields of an instruction record (opcode, operand slots, coordinate overrides): wave-uniform,
+// read through the scalar cache by way of the constant address space.  The rest of the record stays
+// vector loads, which keeps the scalar register count within bounds.
+struct PHdr { int32_t op, dst, a, b, c, cm, cx, cy, cz; };
+typedef const __attribute__((address_space(4))) int32_t *const_i32p;
+__device__ __forceinline__ PHdr phdr(const PIns *p) {
+    const const_i32p q = (const_i32p)p;
+    return PHdr{q[0], q[1], q[2], q[3], q[4], q[15], q[16], q[17], q[18]};
+}
+static_assert(offsetof(PIns, cm) == 15 * 4 && offsetof(PIns, cz) == 18 * 4, "PIns header layout");
+
 // 26.3+ float program (mcgpu_dev.cuh:305-374; df_new.c:740-836 with caches off)
 template <bool PL, int SPD>
 __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox, int oy, int oz) {
@@ -337,18 +348,19 @@ __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox
 #pragma unroll 1
     for (int i = 0; i < W.nins; i++) {
         const PIns s = W.ins[i];
-        const int bx = (s.cm & 1) ? s.cx : ox, by = (s.cm & 2) ? s.cy : oy, bz = (s.cm & 4) ? s.cz : oz;
+        const PHdr h = phdr(W.ins + i);
+        const int bx = (h.cm & 1) ? h.cx : ox, by = (h.cm & 2) ? h.cy : oy, bz = (h.cm & 4) ? h.cz : oz;
         float r;
-        switch (s.op) {
+        switch (h.op) {
         case MCGK_CONST: r = s.f0; break;
         case MCGK_NOISE: r = ns_get<PL>(W, lp, s.ns, bx * s.d0, by * s.d1, bz * s.d0); break;
         case MCGK_NOISE_XZ: {
-            const double nx = bx * s.d0 + (double)V(s.a), ny = by * s.d1, nz = bz * s.d0 + (double)V(s.c);
+            const double nx = bx * s.d0 + (double)V(h.a), ny = by * s.d1, nz = bz * s.d0 + (double)V(h.c);
             r = ns_get<PL>(W, lp, s.ns, nx, ny, nz);
             break;
         }
         case MCGK_NOISE_XYZ: {
-            const double nx = bx * s.d0 + (double)V(s.a), ny = by * s.d1 + (double)V(s.b), nz = bz * s.d0 + (double)V(s.c);
+            const double nx = bx * s.d0 + (double)V(h.a), ny = by * s.d1 + (double)V(h.b), nz = bz * s.d0 + (double)V(h.c);
             r = ns_get<PL>(W, lp, s.ns, nx, ny, nz);
             break;
         }
@@ -359,49 +371,49 @@ __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox
             r = grad_value(s, s.i0 == 0 ? bx : (s.i0 == 1 ? by : bz));
             break;
         case MCGK_CTX_ALPHA: case MCGK_CTX_OFFSET: case MCGK_CTX_BEARD: r = s.f0; break;
-        case MCGK_ABS: r = fabsf(V(s.a)); break;
-        case MCGK_SQUARE: { const float v = V(s.a); r = v * v; break; }
-        case MCGK_CUBE: { const float v = V(s.a); r = v * v * v; break; }
-        case MCGK_SQRT: r = (float)sqrt((double)V(s.a)); break;
-        case MCGK_LEAKY: { const float v = V(s.a); r = v > 0.0f ? v : v * s.f0; break; }
-        case MCGK_RECIP: r = 1.0f / V(s.a); break;
-        case MCGK_NEG: r = -V(s.a); break;
-        case MCGK_SQUEEZE: r = squeeze_f(V(s.a)); break;
-        case MCGK_SIGN: r = jsignf(V(s.a)); break;
-        case MCGK_ADD: r = V(s.a) + V(s.b); break;
-        case MCGK_CADD: r = V(s.a) + s.f0; break;
-        case MCGK_CSUB: r = s.f0 - V(s.a); break;
-        case MCGK_SUB: r = V(s.a) - V(s.b); break;
-        case MCGK_MUL: { const float l = V(s.a); r = l == 0.0f ? 0.0f : l * V(s.b); break; }
-        case MCGK_CMUL: r = V(s.a) * s.f0; break;
-        case MCGK_DIV: { const float l = V(s.a); r = l == 0.0f ? 0.0f : l / V(s.b); break; }
-        case MCGK_CDIV: r = s.f0 / V(s.a); break;
-        case MCGK_MIN: { const float l = V(s.a); r = l <= s.f0 ? l : jminf(l, V(s.b)); break; }
-        case MCGK_CMIN: r = jminf(V(s.a), s.f0); break;
-        case MCGK_MAX: { const float l = V(s.a); r = l >= s.f0 ? l : jmaxf(l, V(s.b)); break; }
-        case MCGK_CMAX: r = jmaxf(V(s.a), s.f0); break;
+        case MCGK_ABS: r = fabsf(V(h.a)); break;
+        case MCGK_SQUARE: { const float v = V(h.a); r = v * v; break; }
+        case MCGK_CUBE: { const float v = V(h.a); r = v * v * v; break; }
+        case MCGK_SQRT: r = (float)sqrt((double)V(h.a)); break;
+        case MCGK_LEAKY: { const float v = V(h.a); r = v > 0.0f ? v : v * s.f0; break; }
+        case MCGK_RECIP: r = 1.0f / V(h.a); break;
+        case MCGK_NEG: r = -V(h.a); break;
+        case MCGK_SQUEEZE: r = squeeze_f(V(h.a)); break;
+        case MCGK_SIGN: r = jsignf(V(h.a)); break;
+        case MCGK_ADD: r = V(h.a) + V(h.b); break;
+        case MCGK_CADD: r = V(h.a) + s.f0; break;
+        case MCGK_CSUB: r = s.f0 - V(h.a); break;
+        case MCGK_SUB: r = V(h.a) - V(h.b); break;
+        case MCGK_MUL: { const float l = V(h.a); r = l == 0.0f ? 0.0f : l * V(h.b); break; }
+        case MCGK_CMUL: r = V(h.a) * s.f0; break;
+        case MCGK_DIV: { const float l = V(h.a); r = l == 0.0f ? 0.0f : l / V(h.b); break; }
+        case MCGK_CDIV: r = s.f0 / V(h.a); break;
+        case MCGK_MIN: { const float l = V(h.a); r = l <= s.f0 ? l : jminf(l, V(h.b)); break; }
+        case MCGK_CMIN: r = jminf(V(h.a), s.f0); break;
+        case MCGK_MAX: { const float l = V(h.a); r = l >= s.f0 ? l : jmaxf(l, V(h.b)); break; }
+        case MCGK_CMAX: r = jmaxf(V(h.a), s.f0); break;
         case MCGK_SPLINE: r = spline<float, SPD>(W, s.sp, vals, s.arr0); break;
         case MCGK_LERP: {
-            const float a = V(s.a);
-            r = a == 0.0f ? V(s.b) : (a == 1.0f ? V(s.c) : jlerpf(a, V(s.b), V(s.c)));
+            const float a = V(h.a);
+            r = a == 0.0f ? V(h.b) : (a == 1.0f ? V(h.c) : jlerpf(a, V(h.b), V(h.c)));
             break;
         }
         case MCGK_LERP_CF: {
-            const float a = V(s.a);
-            r = a == 0.0f ? s.f0 : (a == 1.0f ? V(s.c) : jlerpf(a, s.f0, V(s.c)));
+            const float a = V(h.a);
+            r = a == 0.0f ? s.f0 : (a == 1.0f ? V(h.c) : jlerpf(a, s.f0, V(h.c)));
             break;
         }
         case MCGK_LERP_CS: {
-            const float a = V(s.a);
-            r = a == 0.0f ? V(s.b) : (a == 1.0f ? s.f0 : jlerpf(a, V(s.b), s.f0));
+            const float a = V(h.a);
+            r = a == 0.0f ? V(h.b) : (a == 1.0f ? s.f0 : jlerpf(a, V(h.b), s.f0));
             break;
         }
-        case MCGK_CLAMP: r = jclampf(V(s.a), s.f0, s.f1); break;
-        case MCGK_RANGE_C: { const float v = V(s.a); r = (v >= s.f0 && v < s.f1) ? s.f2 : s.f3; break; }
-        case MCGK_RANGE: { const float v = V(s.a); r = (v >= s.f0 && v < s.f1) ? V(s.b) : V(s.c); break; }
-        case MCGK_ISEL1: { const float v = V(s.a); r = v < s.f0 ? V(s.b) : V(s.c); break; }
+        case MCGK_CLAMP: r = jclampf(V(h.a), s.f0, s.f1); break;
+        case MCGK_RANGE_C: { const float v = V(h.a); r = (v >= s.f0 && v < s.f1) ? s.f2 : s.f3; break; }
+        case MCGK_RANGE: { const float v = V(h.a); r = (v >= s.f0 && v < s.f1) ? V(h.b) : V(h.c); break; }
+        case MCGK_ISEL1: { const float v = V(h.a); r = v < s.f0 ? V(h.b) : V(h.c); break; }
         case MCGK_ISEL: {
-            const float v = V(s.a);
+            const float v = V(h.a);
             int k = 0;
             for (; k < s.nthr; k++)
                 if (v < (float)W.thr[s.thr0 + k]) break;
@@ -409,9 +421,9 @@ __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox
             r = V(W.arr[s.arr0 + k]);
             break;
         }
-        default: r = V(s.a); break;            // CACHE, BLEND_DENSITY, SLICE_*: the argument's value
+        default: r = V(h.a); break;            // CACHE, BLEND_DENSITY, SLICE_*: the argument's value
         }
-        V(s.dst) = r;
+        V(h.dst) = r;
     }
 #undef V
 }
@@ -431,12 +443,13 @@ __device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int b
 #pragma unroll 1
     for (int i = 0; i < W.nins; i++) {
         const PIns s = W.ins[i];
+        const PHdr h = phdr(W.ins + i);
         double r;
-        switch (s.op) {
+        switch (h.op) {
         case MCGO_CONST: r = s.d0; break;
         case MCGO_NOISE: r = old_normal<PL>(W, lp, s.ns, bx * s.d0, by * s.d1, bz * s.d0); break;
         case MCGO_SHIFTED_NOISE: {
-            const double x = bx * s.d0 + V(s.a), y = by * s.d1 + V(s.b), z = bz * s.d0 + V(s.c);
+            const double x = bx * s.d0 + V(h.a), y = by * s.d1 + V(h.b), z = bz * s.d0 + V(h.c);
             r = old_normal<PL>(W, lp, s.ns, x, y, z);
             break;
         }
@@ -448,29 +461,29 @@ __device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int b
             break;
         case MCGO_Y_GRADIENT: r = clamped_map((double)by, (double)s.i0, (double)s.i1, s.d0, s.d1); break;
         case MCGO_WEIRD_SCALED: {
-            const double rr = weird_rarity(s.i0, V(s.a));
+            const double rr = weird_rarity(s.i0, V(h.a));
             r = rr * fabs(old_normal<PL>(W, lp, s.ns, bx / rr, by / rr, bz / rr));
             break;
         }
         case MCGO_BLEND_ALPHA: r = 1.0; break;
         case MCGO_ZERO: r = 0.0; break;
-        case MCGO_ABS: r = fabs(V(s.a)); break;
-        case MCGO_SQUARE: { const double v = V(s.a); r = v * v; break; }
-        case MCGO_CUBE: { const double v = V(s.a); r = v * v * v; break; }
-        case MCGO_HALF_NEG: { const double v = V(s.a); r = v > 0.0 ? v : v * 0.5; break; }
-        case MCGO_QUARTER_NEG: { const double v = V(s.a); r = v > 0.0 ? v : v * 0.25; break; }
-        case MCGO_RECIP: r = 1.0 / V(s.a); break;
-        case MCGO_SQUEEZE: { const double c = jclamp(V(s.a), -1.0, 1.0); r = c / 2.0 - c * c * c / 24.0; break; }
-        case MCGO_CLAMP: r = jclamp(V(s.a), s.d0, s.d1); break;
-        case MCGO_ADD: r = V(s.a) + V(s.b); break;
-        case MCGO_MUL: { const double v = V(s.a); r = v == 0.0 ? 0.0 : v * V(s.b); break; }
-        case MCGO_MIN: { const double v = V(s.a); r = v < s.d1 ? v : jmin(v, V(s.b)); break; }
-        case MCGO_MAX: { const double v = V(s.a); r = v > s.d1 ? v : jmax(v, V(s.b)); break; }
-        case MCGO_CADD: r = V(s.a) + s.d0; break;
-        case MCGO_CMUL: r = V(s.a) * s.d0; break;
-        case MCGO_RANGE: { const double v = V(s.a); r = (v >= s.d0 && v < s.d1) ? V(s.b) : V(s.c); break; }
+        case MCGO_ABS: r = fabs(V(h.a)); break;
+        case MCGO_SQUARE: { const double v = V(h.a); r = v * v; break; }
+        case MCGO_CUBE: { const double v = V(h.a); r = v * v * v; break; }
+        case MCGO_HALF_NEG: { const double v = V(h.a); r = v > 0.0 ? v : v * 0.5; break; }
+        case MCGO_QUARTER_NEG: { const double v = V(h.a); r = v > 0.0 ? v : v * 0.25; break; }
+        case MCGO_RECIP: r = 1.0 / V(h.a); break;
+        case MCGO_SQUEEZE: { const double c = jclamp(V(h.a), -1.0, 1.0); r = c / 2.0 - c * c * c / 24.0; break; }
+        case MCGO_CLAMP: r = jclamp(V(h.a), s.d0, s.d1); break;
+        case MCGO_ADD: r = V(h.a) + V(h.b); break;
+        case MCGO_MUL: { const double v = V(h.a); r = v == 0.0 ? 0.0 : v * V(h.b); break; }
+        case MCGO_MIN: { const double v = V(h.a); r = v < s.d1 ? v : jmin(v, V(h.b)); break; }
+        case MCGO_MAX: { const double v = V(h.a); r = v > s.d1 ? v : jmax(v, V(h.b)); break; }
+        case MCGO_CADD: r = V(h.a) + s.d0; break;
+        case MCGO_CMUL: r = V(h.a) * s.d0; break;
+        case MCGO_RANGE: { const double v = V(h.a); r = (v >= s.d0 && v < s.d1) ? V(h.b) : V(h.c); break; }
         case MCGO_ISEL: {
-            const double v = V(s.a);
+            const double v = V(h.a);
             int k = 0;
             for (; k < s.nthr; k++)
                 if (v < W.thr[s.thr0 + k]) break;
@@ -479,9 +492,9 @@ __device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int b
             break;
         }
         case MCGO_SPLINE: r = (double)spline<double, SPD>(W, s.sp, vals, s.arr0); break;
-        default: r = V(s.a); break;            // PASS
+        default: r = V(h.a); break;            // PASS
         }
-        V(s.dst) = r;
+        V(h.dst) = r;
     }
 #undef V
 }
