@@ -96,6 +96,28 @@ def test_large_grid_sampled():
     assert np.array_equal(got, again)
 
 
+@pytest.mark.parametrize("version,dim,step", [("26.3", "overworld", 1), ("26.3", "overworld", 4), ("26.1", "overworld", 2),
+                                              ("26.3", "end", 1), ("26.3", "nether", 16)])
+def test_large_grid_equals_row_chunks(version, dim, step):
+    """A grid of more than 1 MiB (one lattice over all its quart rows, or one per-point launch) equals
+    the same rows requested as grids of 128 rows, whose lattices start at other quart rows; the row
+    count is not a multiple of 128 and a chunk edge falls inside a quart row."""
+    V = mcgprog.vanilla(version, dim, 5)
+    W = mcgpu.BiomeWorld(V.world)
+    nx, nz = 1100, 1031
+    x0, z0, y = (-3001, 70_003, 64) if dim != "end" else (2203, -1905, 60)
+    print("large grid", version, dim, step, flush=True)
+    got, bad = W.grid(x0, z0, nx, nz, step, y)
+    parts, pbad = [], 0
+    for r in range(0, nz, 128):
+        g, b = W.grid(x0, z0 + r * step, nx, min(128, nz - r), step, y)
+        parts.append(g)
+        pbad += b
+    want = np.concatenate(parts, 0)
+    assert bad == pbad
+    assert np.array_equal(got, want), int((got != want).sum())
+
+
 # ---------------------------------------------------------------------------------------------------
 # node kinds through probe trees
 # ---------------------------------------------------------------------------------------------------
