@@ -333,13 +333,16 @@ __device__ __forceinline__ float dist_value(int m, float dx, float dy, float dz)
 // Header fields of an instruction record (opcode, operand slots, coordinate overrides): wave-uniform,
 // read through the scalar cache by way of the constant address space.  The rest of the record stays
 // vector loads, which keeps the scalar register count within bounds.
-struct PHdr { int32_t op, dst, a, b, c, cm, cx, cy, cz; };
+struct PHdr { int32_t op, dst, a, b, c, cm, cx, cy, cz, ns; float f0; double d0, d1; };
 typedef const __attribute__((address_space(4))) int32_t *const_i32p;
 __device__ __forceinline__ PHdr phdr(const PIns *p) {
     const const_i32p q = (const_i32p)p;
-    return PHdr{q[0], q[1], q[2], q[3], q[4], q[15], q[16], q[17], q[18]};
+    const __attribute__((address_space(4))) double *qd = (const __attribute__((address_space(4))) double *)p;
+    return PHdr{q[0], q[1], q[2], q[3], q[4], q[15], q[16], q[17], q[18], q[5], __int_as_float(q[20]), qd[12], qd[13]};
 }
-static_assert(offsetof(PIns, cm) == 15 * 4 && offsetof(PIns, cz) == 18 * 4, "PIns header layout");
+static_assert(offsetof(PIns, cm) == 15 * 4 && offsetof(PIns, cz) == 18 * 4 && offsetof(PIns, ns) == 5 * 4 &&
+                  offsetof(PIns, f0) == 20 * 4 && offsetof(PIns, d0) == 12 * 8 && offsetof(PIns, d1) == 13 * 8,
+              "PIns header layout");
 
 // 26.3+ float program (mcgpu_dev.cuh:305-374; df_new.c:740-836 with caches off)
 template <bool PL, int SPD>
@@ -352,46 +355,46 @@ __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox
         const int bx = (h.cm & 1) ? h.cx : ox, by = (h.cm & 2) ? h.cy : oy, bz = (h.cm & 4) ? h.cz : oz;
         float r;
         switch (h.op) {
-        case MCGK_CONST: r = s.f0; break;
-        case MCGK_NOISE: r = ns_get<PL>(W, lp, s.ns, bx * s.d0, by * s.d1, bz * s.d0); break;
+        case MCGK_CONST: r = h.f0; break;
+        case MCGK_NOISE: r = ns_get<PL>(W, lp, h.ns, bx * h.d0, by * h.d1, bz * h.d0); break;
         case MCGK_NOISE_XZ: {
-            const double nx = bx * s.d0 + (double)V(h.a), ny = by * s.d1, nz = bz * s.d0 + (double)V(h.c);
-            r = ns_get<PL>(W, lp, s.ns, nx, ny, nz);
+            const double nx = bx * h.d0 + (double)V(h.a), ny = by * h.d1, nz = bz * h.d0 + (double)V(h.c);
+            r = ns_get<PL>(W, lp, h.ns, nx, ny, nz);
             break;
         }
         case MCGK_NOISE_XYZ: {
-            const double nx = bx * s.d0 + (double)V(h.a), ny = by * s.d1 + (double)V(h.b), nz = bz * s.d0 + (double)V(h.c);
-            r = ns_get<PL>(W, lp, s.ns, nx, ny, nz);
+            const double nx = bx * h.d0 + (double)V(h.a), ny = by * h.d1 + (double)V(h.b), nz = bz * h.d0 + (double)V(h.c);
+            r = ns_get<PL>(W, lp, h.ns, nx, ny, nz);
             break;
         }
-        case MCGK_SHIFT_B: r = ns_get<PL>(W, lp, s.ns, bz * s.d0, bx * s.d0, 0.0) * 4.0f; break;
+        case MCGK_SHIFT_B: r = ns_get<PL>(W, lp, h.ns, bz * h.d0, bx * h.d0, 0.0) * 4.0f; break;
         case MCGK_END: r = (end_height_j<false>(perm_of<true, PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0f) / 128.0f; break;
         case MCGK_DIST: r = dist_value(s.i3, (float)(s.i0 - bx), (float)(s.i1 - by), (float)(s.i2 - bz)); break;
         case MCGK_GRAD_CLAMP: case MCGK_GRAD_REPEAT: case MCGK_GRAD_MIRROR:
             r = grad_value(s, s.i0 == 0 ? bx : (s.i0 == 1 ? by : bz));
             break;
-        case MCGK_CTX_ALPHA: case MCGK_CTX_OFFSET: case MCGK_CTX_BEARD: r = s.f0; break;
+        case MCGK_CTX_ALPHA: case MCGK_CTX_OFFSET: case MCGK_CTX_BEARD: r = h.f0; break;
         case MCGK_ABS: r = fabsf(V(h.a)); break;
         case MCGK_SQUARE: { const float v = V(h.a); r = v * v; break; }
         case MCGK_CUBE: { const float v = V(h.a); r = v * v * v; break; }
         case MCGK_SQRT: r = (float)sqrt((double)V(h.a)); break;
-        case MCGK_LEAKY: { const float v = V(h.a); r = v > 0.0f ? v : v * s.f0; break; }
+        case MCGK_LEAKY: { const float v = V(h.a); r = v > 0.0f ? v : v * h.f0; break; }
         case MCGK_RECIP: r = 1.0f / V(h.a); break;
         case MCGK_NEG: r = -V(h.a); break;
         case MCGK_SQUEEZE: r = squeeze_f(V(h.a)); break;
         case MCGK_SIGN: r = jsignf(V(h.a)); break;
         case MCGK_ADD: r = V(h.a) + V(h.b); break;
-        case MCGK_CADD: r = V(h.a) + s.f0; break;
-        case MCGK_CSUB: r = s.f0 - V(h.a); break;
+        case MCGK_CADD: r = V(h.a) + h.f0; break;
+        case MCGK_CSUB: r = h.f0 - V(h.a); break;
         case MCGK_SUB: r = V(h.a) - V(h.b); break;
         case MCGK_MUL: { const float l = V(h.a); r = l == 0.0f ? 0.0f : l * V(h.b); break; }
-        case MCGK_CMUL: r = V(h.a) * s.f0; break;
+        case MCGK_CMUL: r = V(h.a) * h.f0; break;
         case MCGK_DIV: { const float l = V(h.a); r = l == 0.0f ? 0.0f : l / V(h.b); break; }
-        case MCGK_CDIV: r = s.f0 / V(h.a); break;
-        case MCGK_MIN: { const float l = V(h.a); r = l <= s.f0 ? l : jminf(l, V(h.b)); break; }
-        case MCGK_CMIN: r = jminf(V(h.a), s.f0); break;
-        case MCGK_MAX: { const float l = V(h.a); r = l >= s.f0 ? l : jmaxf(l, V(h.b)); break; }
-        case MCGK_CMAX: r = jmaxf(V(h.a), s.f0); break;
+        case MCGK_CDIV: r = h.f0 / V(h.a); break;
+        case MCGK_MIN: { const float l = V(h.a); r = l <= h.f0 ? l : jminf(l, V(h.b)); break; }
+        case MCGK_CMIN: r = jminf(V(h.a), h.f0); break;
+        case MCGK_MAX: { const float l = V(h.a); r = l >= h.f0 ? l : jmaxf(l, V(h.b)); break; }
+        case MCGK_CMAX: r = jmaxf(V(h.a), h.f0); break;
         case MCGK_SPLINE: r = spline<float, SPD>(W, s.sp, vals, s.arr0); break;
         case MCGK_LERP: {
             const float a = V(h.a);
@@ -400,18 +403,18 @@ __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox
         }
         case MCGK_LERP_CF: {
             const float a = V(h.a);
-            r = a == 0.0f ? s.f0 : (a == 1.0f ? V(h.c) : jlerpf(a, s.f0, V(h.c)));
+            r = a == 0.0f ? h.f0 : (a == 1.0f ? V(h.c) : jlerpf(a, h.f0, V(h.c)));
             break;
         }
         case MCGK_LERP_CS: {
             const float a = V(h.a);
-            r = a == 0.0f ? V(h.b) : (a == 1.0f ? s.f0 : jlerpf(a, V(h.b), s.f0));
+            r = a == 0.0f ? V(h.b) : (a == 1.0f ? h.f0 : jlerpf(a, V(h.b), h.f0));
             break;
         }
-        case MCGK_CLAMP: r = jclampf(V(h.a), s.f0, s.f1); break;
-        case MCGK_RANGE_C: { const float v = V(h.a); r = (v >= s.f0 && v < s.f1) ? s.f2 : s.f3; break; }
-        case MCGK_RANGE: { const float v = V(h.a); r = (v >= s.f0 && v < s.f1) ? V(h.b) : V(h.c); break; }
-        case MCGK_ISEL1: { const float v = V(h.a); r = v < s.f0 ? V(h.b) : V(h.c); break; }
+        case MCGK_CLAMP: r = jclampf(V(h.a), h.f0, s.f1); break;
+        case MCGK_RANGE_C: { const float v = V(h.a); r = (v >= h.f0 && v < s.f1) ? s.f2 : s.f3; break; }
+        case MCGK_RANGE: { const float v = V(h.a); r = (v >= h.f0 && v < s.f1) ? V(h.b) : V(h.c); break; }
+        case MCGK_ISEL1: { const float v = V(h.a); r = v < h.f0 ? V(h.b) : V(h.c); break; }
         case MCGK_ISEL: {
             const float v = V(h.a);
             int k = 0;
@@ -446,23 +449,23 @@ __device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int b
         const PHdr h = phdr(W.ins + i);
         double r;
         switch (h.op) {
-        case MCGO_CONST: r = s.d0; break;
-        case MCGO_NOISE: r = old_normal<PL>(W, lp, s.ns, bx * s.d0, by * s.d1, bz * s.d0); break;
+        case MCGO_CONST: r = h.d0; break;
+        case MCGO_NOISE: r = old_normal<PL>(W, lp, h.ns, bx * h.d0, by * h.d1, bz * h.d0); break;
         case MCGO_SHIFTED_NOISE: {
-            const double x = bx * s.d0 + V(h.a), y = by * s.d1 + V(h.b), z = bz * s.d0 + V(h.c);
-            r = old_normal<PL>(W, lp, s.ns, x, y, z);
+            const double x = bx * h.d0 + V(h.a), y = by * h.d1 + V(h.b), z = bz * h.d0 + V(h.c);
+            r = old_normal<PL>(W, lp, h.ns, x, y, z);
             break;
         }
-        case MCGO_SHIFT_A: r = old_normal<PL>(W, lp, s.ns, (double)bx * s.d0, 0.0 * s.d0, (double)bz * s.d0) * 4.0; break;
-        case MCGO_SHIFT_B: r = old_normal<PL>(W, lp, s.ns, (double)bz * s.d0, (double)bx * s.d0, 0.0 * s.d0) * 4.0; break;
-        case MCGO_SHIFT: r = old_normal<PL>(W, lp, s.ns, (double)bx * s.d0, (double)by * s.d1, (double)bz * s.d0) * 4.0; break;
+        case MCGO_SHIFT_A: r = old_normal<PL>(W, lp, h.ns, (double)bx * h.d0, 0.0 * h.d0, (double)bz * h.d0) * 4.0; break;
+        case MCGO_SHIFT_B: r = old_normal<PL>(W, lp, h.ns, (double)bz * h.d0, (double)bx * h.d0, 0.0 * h.d0) * 4.0; break;
+        case MCGO_SHIFT: r = old_normal<PL>(W, lp, h.ns, (double)bx * h.d0, (double)by * h.d1, (double)bz * h.d0) * 4.0; break;
         case MCGO_END_ISLANDS:
             r = ((double)end_height_j<true>(perm_of<false, PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0) / 128.0;
             break;
-        case MCGO_Y_GRADIENT: r = clamped_map((double)by, (double)s.i0, (double)s.i1, s.d0, s.d1); break;
+        case MCGO_Y_GRADIENT: r = clamped_map((double)by, (double)s.i0, (double)s.i1, h.d0, h.d1); break;
         case MCGO_WEIRD_SCALED: {
             const double rr = weird_rarity(s.i0, V(h.a));
-            r = rr * fabs(old_normal<PL>(W, lp, s.ns, bx / rr, by / rr, bz / rr));
+            r = rr * fabs(old_normal<PL>(W, lp, h.ns, bx / rr, by / rr, bz / rr));
             break;
         }
         case MCGO_BLEND_ALPHA: r = 1.0; break;
@@ -474,14 +477,14 @@ __device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int b
         case MCGO_QUARTER_NEG: { const double v = V(h.a); r = v > 0.0 ? v : v * 0.25; break; }
         case MCGO_RECIP: r = 1.0 / V(h.a); break;
         case MCGO_SQUEEZE: { const double c = jclamp(V(h.a), -1.0, 1.0); r = c / 2.0 - c * c * c / 24.0; break; }
-        case MCGO_CLAMP: r = jclamp(V(h.a), s.d0, s.d1); break;
+        case MCGO_CLAMP: r = jclamp(V(h.a), h.d0, h.d1); break;
         case MCGO_ADD: r = V(h.a) + V(h.b); break;
         case MCGO_MUL: { const double v = V(h.a); r = v == 0.0 ? 0.0 : v * V(h.b); break; }
-        case MCGO_MIN: { const double v = V(h.a); r = v < s.d1 ? v : jmin(v, V(h.b)); break; }
-        case MCGO_MAX: { const double v = V(h.a); r = v > s.d1 ? v : jmax(v, V(h.b)); break; }
-        case MCGO_CADD: r = V(h.a) + s.d0; break;
-        case MCGO_CMUL: r = V(h.a) * s.d0; break;
-        case MCGO_RANGE: { const double v = V(h.a); r = (v >= s.d0 && v < s.d1) ? V(h.b) : V(h.c); break; }
+        case MCGO_MIN: { const double v = V(h.a); r = v < h.d1 ? v : jmin(v, V(h.b)); break; }
+        case MCGO_MAX: { const double v = V(h.a); r = v > h.d1 ? v : jmax(v, V(h.b)); break; }
+        case MCGO_CADD: r = V(h.a) + h.d0; break;
+        case MCGO_CMUL: r = V(h.a) * h.d0; break;
+        case MCGO_RANGE: { const double v = V(h.a); r = (v >= h.d0 && v < h.d1) ? V(h.b) : V(h.c); break; }
         case MCGO_ISEL: {
             const double v = V(h.a);
             int k = 0;
