@@ -107,7 +107,8 @@ int uses_old(int k) {
 }
 
 struct Compiled {
-    std::vector<PIns> ins;
+    std::vector<PIns> ins;           // the program, then its y-dependent instructions again
+    int nins = 0;
     std::vector<int32_t> arr;        // slot numbers
     int nslot = 0, spl_depth = 1;
     int root_slot[6] = {-1, -1, -1, -1, -1, -1};
@@ -144,6 +145,23 @@ bool check_noise(const McgProg *p, int ns) {
     const bool ok = p->old ? (range_ok(mn.l0, mn.n0, true) && range_ok(mn.l1, mn.n1, true)) : range_ok(mn.l0, mn.n0, false);
     if (!ok) set_error("noise %d: layers or octaves out of range", ns);
     return ok;
+}
+
+// false when node s may read the block y (the y-reading kinds of eval_new / eval_old)
+bool y_free(bool old, const McgNode &s) {
+    if (!old) {
+        switch (s.k) {
+        case MCGK_NOISE: case MCGK_NOISE_XZ: case MCGK_NOISE_XYZ: return (s.cm & 2) != 0 || s.d1 == 0.0;
+        case MCGK_DIST: return (s.cm & 2) != 0;
+        case MCGK_GRAD_CLAMP: case MCGK_GRAD_REPEAT: case MCGK_GRAD_MIRROR: return s.i0 != 1 || (s.cm & 2) != 0;
+        default: return true;
+        }
+    }
+    switch (s.k) {
+    case MCGO_NOISE: case MCGO_SHIFTED_NOISE: case MCGO_SHIFT: return s.d1 == 0.0;
+    case MCGO_Y_GRADIENT: case MCGO_WEIRD_SCALED: return false;
+    default: return true;
+    }
 }
 
 bool compile_prog(const McgProg *p, Compiled &C) {
@@ -202,21 +220,64 @@ bool compile_prog(const McgProg *p, Compiled &C) {
         if (r < -1 || r >= n) { set_error("root %d out of range", k); return false; }
         if (r >= 0) last[r] = n;                              // live to the end
     }
-    // linear-scan slot assignment: a value's slot is free again after its last reader
-    std::vector<int> slot(n, -1), free_slots;
-    std::vector<std::vector<int>> dies(n + 1);
-    for (int j = 0; j < n; j++)
-        if (last[j] >= 0 && last[j] < n) dies[last[j]].push_back(j);
-    int nslot = 0;
+    // y-independent nodes: no read of the block y (or y fixed by the coordinate mask, or scaled by 0, which
+    // gives +-0 and the same noise value), and every operand y-independent.  A lane that evaluates the
+    // same (x, z) again at another y skips them; the ones a y-dependent node or a root reads keep their
+    // values between evaluations in slots of their own.
+    std::vector<char> yind(n, 0), keep(n, 0);
+    auto operands = [&](int i, auto &&f) {
+        const McgNode &s = p->nodes[i];
+        const int u = p->old ? uses_old(s.k) : uses_new(s.k);
+        if (u & OP_A) f(s.a);
+        if (u & OP_B) f(s.b);
+        if (u & OP_C) f(s.c);
+        if (u & (OP_SPL | OP_ISEL))
+            for (int q = 0; q < s.narr; q++) f(p->arr[s.arr0 + q]);
+    };
     for (int i = 0; i < n; i++) {
-        for (int j : dies[i]) free_slots.push_back(slot[j]);  // read by node i: the kernel reads before it writes
-        std::sort(free_slots.begin(), free_slots.end(), std::greater<int>());
-        free_slots.erase(std::unique(free_slots.begin(), free_slots.end()), free_slots.end());
-        int d;
-        if (!free_slots.empty()) { d = free_slots.back(); free_slots.pop_back(); }
-        else d = nslot++;
-        slot[i] = d;
-        if (last[i] < 0) free_slots.push_back(d);             // never read: overwritten by a later node
+        bool ind = y_free(p->old != 0, p->nodes[i]);
+        operands(i, [&](int j) { ind = ind && yind[j]; });
+        yind[i] = ind;
+    }
+    for (int i = 0; i < n; i++)
+        if (!yind[i]) operands(i, [&](int j) { if (yind[j]) keep[j] = 1; });
+    for (int k = 0; k < 6; k++) {
+        const int r = k < p->nroots ? p->root[k] : -1;
+        if (r >= 0 && yind[r]) keep[r] = 1;
+    }
+    // slots: the kept values first, one each; the rest by linear scan (a value's slot is free again
+    // after its last reader)
+    auto assign = [&](const std::vector<char> &kept, std::vector<int> &slot) {
+        std::vector<int> free_slots;
+        int ns = 0;
+        slot.assign(n, -1);
+        for (int j = 0; j < n; j++)
+            if (kept[j]) slot[j] = ns++;
+        std::vector<std::vector<int>> dies(n + 1);
+        for (int j = 0; j < n; j++)
+            if (!kept[j] && last[j] >= 0 && last[j] < n) dies[last[j]].push_back(j);
+        for (int i = 0; i < n; i++) {
+            for (int j : dies[i]) free_slots.push_back(slot[j]);  // read by node i: the kernel reads before it writes
+            if (kept[i]) continue;
+            std::sort(free_slots.begin(), free_slots.end(), std::greater<int>());
+            free_slots.erase(std::unique(free_slots.begin(), free_slots.end()), free_slots.end());
+            int d;
+            if (!free_slots.empty()) { d = free_slots.back(); free_slots.pop_back(); }
+            else d = ns++;
+            slot[i] = d;
+            if (last[i] < 0) free_slots.push_back(d);             // never read: overwritten by a later node
+        }
+        return ns;
+    };
+    // The kept slots cost LDS: they are used when they add at most two slots and stay within 16 (which fits
+    // beside any table); otherwise the program is evaluated in full every time, as without them.
+    std::vector<int> slot, slot_plain;
+    int nslot = assign(keep, slot);
+    const int nplain = assign(std::vector<char>(n, 0), slot_plain);
+    if (nslot > nplain + 2 || nslot > 16) {
+        slot.swap(slot_plain);
+        nslot = nplain;
+        std::fill(yind.begin(), yind.end(), 0);
     }
     C.nslot = std::max(nslot, 1);
     C.arr.assign(std::max(p->narr, 1), 0);
@@ -233,7 +294,7 @@ bool compile_prog(const McgProg *p, Compiled &C) {
         q.c = (u & OP_C) ? slot[s.c] : 0;
         q.ns = s.ns; q.sp = s.sp; q.arr0 = s.arr0; q.narr = s.narr; q.thr0 = s.thr0; q.nthr = s.nthr;
         q.i0 = s.i0; q.i1 = s.i1; q.i2 = s.i2; q.i3 = s.i3;
-        q.cm = p->old ? 0 : s.cm; q.cx = s.cx; q.cy = s.cy; q.cz = s.cz; q.oct = s.oct;
+        q.cm = (p->old ? 0 : (s.cm & 7)) | (yind[i] ? kCmYFree : 0); q.cx = s.cx; q.cy = s.cy; q.cz = s.cz; q.oct = s.oct;
         q.f0 = s.f0; q.f1 = s.f1; q.f2 = s.f2; q.f3 = s.f3;
         q.d0 = s.d0; q.d1 = s.d1;
         if (u & (OP_SPL | OP_ISEL))
@@ -243,6 +304,11 @@ bool compile_prog(const McgProg *p, Compiled &C) {
         const int r = k < p->nroots ? p->root[k] : -1;
         C.root_slot[k] = r >= 0 ? slot[r] : -1;
     }
+    // after the program, copies of its y-dependent instructions: what a wave whose lanes all repeat their
+    // x and z runs
+    C.nins = n;
+    for (int i = 0; i < n; i++)
+        if (!yind[i]) C.ins.push_back(C.ins[i]);
     return true;
 }
 
@@ -498,7 +564,8 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
         P.clist = sh ? sh->clist : (const uint16_t *)at(12);
         for (int k = 0; k < 6; k++) { P.cnb[k] = H.cells->nb[k]; P.coff[k] = H.cells->off[k]; }
     }
-    P.nins = (int32_t)C.ins.size();
+    P.nins = (int32_t)C.nins;
+    P.nydep = (int32_t)C.ins.size() - C.nins;
     P.nnoise = p->nnoise;
     P.nlayers = p->nlayers;
     P.noct = p->noct;

@@ -29,6 +29,9 @@ constexpr bool kPermDoubled = false;
 constexpr int grad_lds_bytes(bool newf) { return newf ? 256 * 8 : 256 * 16; }
 
 // one program node, compiled (128 bytes; operand fields are value slots)
+// PIns.cm: bits 0-2 = the node's x/y/z override mask; kCmYFree = the node's value does not depend on
+// the block y (skipped when a lane evaluates the same x and z again)
+constexpr int32_t kCmYFree = 8;
 struct PIns {
     int32_t op, dst, a, b, c;
     int32_t ns, sp, arr0, narr, thr0, nthr;
@@ -68,6 +71,7 @@ struct PWorld {
     const uint16_t *clist;
     int32_t cnb[6], coff[6];
     int32_t nins, noct, ntree, nslot, spl_depth;
+    int32_t nydep;                 // ins[nins .. nins + nydep): the y-dependent instructions again
     int32_t nnoise, nlayers;
     int32_t chain_cap;             // tied cells followed back before a cell is reported unresolved
     int32_t root_slot[6];          // -1: field absent

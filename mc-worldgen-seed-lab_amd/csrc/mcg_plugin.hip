@@ -346,12 +346,16 @@ static_assert(offsetof(PIns, cm) == 15 * 4 && offsetof(PIns, cz) == 18 * 4 && of
 
 // 26.3+ float program (mcgpu_dev.cuh:305-374; df_new.c:740-836 with caches off)
 template <bool PL, int SPD>
-__device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox, int oy, int oz) {
+__device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox, int oy, int oz, bool same) {
 #define V(x) vals[(x) * kWG]
+    // a wave whose lanes all repeat their x and z runs only the y-dependent copies
+    const bool all = __all(same);
+    const int i0 = all ? W.nins : 0, i1 = all ? W.nins + W.nydep : W.nins;
 #pragma unroll 1
-    for (int i = 0; i < W.nins; i++) {
+    for (int i = i0; i < i1; i++) {
         const PIns s = W.ins[i];
         const PHdr h = phdr(W.ins + i);
+        if (same && (h.cm & kCmYFree)) continue;           // value kept from this lane's evaluation at the same x, z
         const int bx = (h.cm & 1) ? h.cx : ox, by = (h.cm & 2) ? h.cy : oy, bz = (h.cm & 4) ? h.cz : oz;
         float r;
         switch (h.op) {
@@ -441,12 +445,16 @@ __device__ __forceinline__ double weird_rarity(int type, double r) {
     return r < -0.75 ? 0.5 : (r < -0.5 ? 0.75 : (r < 0.5 ? 1.0 : (r < 0.75 ? 2.0 : 3.0)));
 }
 template <bool PL, int SPD>
-__device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int bx, int by, int bz) {
+__device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int bx, int by, int bz, bool same) {
 #define V(x) vals[(x) * kWG]
+    // a wave whose lanes all repeat their x and z runs only the y-dependent copies
+    const bool all = __all(same);
+    const int i0 = all ? W.nins : 0, i1 = all ? W.nins + W.nydep : W.nins;
 #pragma unroll 1
-    for (int i = 0; i < W.nins; i++) {
+    for (int i = i0; i < i1; i++) {
         const PIns s = W.ins[i];
         const PHdr h = phdr(W.ins + i);
+        if (same && (h.cm & kCmYFree)) continue;           // value kept from this lane's evaluation at the same x, z
         double r;
         switch (h.op) {
         case MCGO_CONST: r = h.d0; break;
@@ -508,9 +516,9 @@ template <> struct VT_of<false> { typedef double T; };
 
 template <bool NEWF, bool PL, int SPD>
 __device__ __forceinline__ void eval_prog(const PWorld &W, const uint8_t *lp, typename VT_of<NEWF>::T *vals, int bx, int by,
-                                          int bz) {
-    if constexpr (NEWF) eval_new<PL, SPD>(W, lp, vals, bx, by, bz);
-    else eval_old<PL, SPD>(W, lp, vals, bx, by, bz);
+                                          int bz, bool same) {
+    if constexpr (NEWF) eval_new<PL, SPD>(W, lp, vals, bx, by, bz, same);
+    else eval_old<PL, SPD>(W, lp, vals, bx, by, bz, same);
 }
 
 // ---------------------------------------------------------------------------------------------------
@@ -706,7 +714,7 @@ __device__ __forceinline__ void count_bad(int32_t *nbad, bool bad) {
 // cells of a lattice, points, or grid points.  Each lane walks its items (stride gridDim*256) and runs
 // one program evaluation per loop trip; a lane whose item finishes starts its next item in the same trip.
 template <bool NEWF, bool PL, int SPD>
-__global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NEWF && SPD <= 6 ? 5 : 1))) void k_cells(PWorld W, PJob J) {
     typedef typename VT_of<NEWF>::T VT;
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr int gbytes = grad_bytes<NEWF>();
@@ -733,22 +741,33 @@ __global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
     c.st = ST_DONE;
     c.result = 0;
     bool have = false;
+    // a lattice item is a column: its two quart y layers run back to back in the lane (layer 1 reuses the
+    // y-independent values of layer 0); out[layer * per + column]
+    const bool lat = J.gen == GEN_LATTICE;
+    const int64_t per = (int64_t)J.nqx * J.nqz, nitems = lat ? per : J.n;
+    int layer = 0;
+    int lx = 0, lz = 0;                                  // x, z of the lane's last evaluation
+    bool lv = false;
     for (;;) {
         // finish the current item, start the next one (items whose biome needs no evaluation finish here)
         while (c.st == ST_DONE) {
             if (have) {
-                J.out[it] = c.result < 0 ? (uint8_t)255 : (uint8_t)c.result;
+                J.out[lat ? it + layer * per : it] = c.result < 0 ? (uint8_t)255 : (uint8_t)c.result;
                 if (J.nbad && c.result < 0) atomicAdd(J.nbad, 1);
-                it += stride;
+                if (lat && layer == 0) {
+                    layer = 1;
+                } else {
+                    layer = 0;
+                    it += stride;
+                }
             }
-            have = it < J.n;
+            have = it < nitems;
             if (!have) break;
             if (J.gen == GEN_CHUNKS) {
                 const int ch = (int)(it / J.cells), r = (int)(it % J.cells);
                 cell_start(W, J.xyz[2 * ch] * 4 + (r & 3), W.min_qy + (r >> 4), J.xyz[2 * ch + 1] * 4 + ((r >> 2) & 3), c);
-            } else if (J.gen == GEN_LATTICE) {
-                const int64_t per = (int64_t)J.nqx * J.nqz;
-                const int layer = (int)(it / per), r = (int)(it % per);
+            } else if (lat) {
+                const int r = (int)it;
                 const int ax = J.q0x + r % J.nqx, az = J.q0z + r / J.nqx, qy = J.qys[layer];
                 if (W.source == MCG_BS_END) chunk_start(W, ax, qy, az, c);       // a lattice of chunks
                 else cell_start(W, ax, qy, az, c);
@@ -767,7 +786,8 @@ __global__ __launch_bounds__(kWG) void k_cells(PWorld W, PJob J) {
         if (!have) break;
         int x, y, z;
         cell_point(W, c, x, y, z);
-        eval_prog<NEWF, PL, SPD>(W, lp, vals, x, y, z);
+        eval_prog<NEWF, PL, SPD>(W, lp, vals, x, y, z, lv && x == lx && z == lz);
+        lx = x; lz = z; lv = true;
         cell_step<VT>(W, vals, c);
     }
 }
@@ -835,8 +855,9 @@ static int grid_blocks(int64_t n) {
 
 template <bool NEWF, bool PL>
 static hipError_t launch_t(const PWorld &W, const PJob &J, int spd, int lds, hipStream_t st) {
-    if (spd <= 6) hipLaunchKernelGGL((k_cells<NEWF, PL, 6>), dim3(grid_blocks(J.n)), dim3(kWG), lds, st, W, J);
-    else hipLaunchKernelGGL((k_cells<NEWF, PL, kSplineDepth>), dim3(grid_blocks(J.n)), dim3(kWG), lds, st, W, J);
+    const int64_t items = J.gen == GEN_LATTICE ? (int64_t)J.nqx * J.nqz : J.n;     // lattice: columns
+    if (spd <= 6) hipLaunchKernelGGL((k_cells<NEWF, PL, 6>), dim3(grid_blocks(items)), dim3(kWG), lds, st, W, J);
+    else hipLaunchKernelGGL((k_cells<NEWF, PL, kSplineDepth>), dim3(grid_blocks(items)), dim3(kWG), lds, st, W, J);
     return hipGetLastError();
 }
 
