@@ -296,6 +296,38 @@ def test_node_kinds_exact(builder, version, old):
     probe(V, P, nodes, old, xyz)
 
 
+@pytest.mark.parametrize("builder,version,old", [(kitchen_new, "26.3", False), (kitchen_old, "26.1", True)])
+def test_node_kinds_lattice_grid(builder, version, old):
+    """Step-1 grids evaluate both quart layers of a lattice column in one lane, the second skipping the
+    nodes whose value cannot depend on y.  Every node kind, as a climate field over the vanilla tree,
+    must give the oracle's grid, including the y-dependent ones (y gradients, distances, noises with a y
+    scale) and everything computed from them.  Fields are scaled by powers of two as in the probes, so no
+    squared parameter distance leaves the 64-bit range: beyond it the plugin ABI's unsigned sums
+    (mcgpu_dev.cuh:464-473) and libmcgen's signed ones, which the oracle follows, part ways."""
+    V = mcgprog.vanilla(version, "overworld", 31)
+    P, nodes = builder(V.prog)
+    names = list(nodes)
+    x0, z0, nx, nz, y = -1203, 2051, 67, 60, 70
+    cells = [(qx, qy, qz) for qx in range((x0 - 2) >> 2, ((x0 + nx - 3) >> 2) + 2, 5)
+             for qz in range((z0 - 2) >> 2, ((z0 + nz - 3) >> 2) + 2, 5) for qy in ((y - 2) >> 2, ((y - 2) >> 2) + 1)]
+    for g in range(0, len(names), 6):
+        group = names[g:g + 6]
+        group += [group[-1]] * (6 - len(group))
+        roots = [nodes[k] for k in group]
+        plain = mcgprog.world(P.build(roots), V.world)
+        ref0 = oracle.McgWorldRef(C.addressof(plain), keep=plain)
+        vals = np.array([ref0.climate(4 * c[0], 4 * c[1], 4 * c[2]) for c in cells])
+        peak = np.maximum(np.where(np.isfinite(vals), np.abs(vals), 0.0).max(axis=0), 1e-3)
+        scale = [2.0 ** min(13, int(np.floor(np.log2(1.5e9 / (1e4 * p))))) for p in peak]
+        W = mcgprog.world(P.build([_scale(P, r, s, old) for r, s in zip(roots, scale)]), V.world)
+        ref = oracle.McgWorldRef(C.addressof(W), keep=W)
+        print("lattice grid", group, flush=True)
+        got, bad = mcgpu.BiomeWorld(W).grid(x0, z0, nx, nz, 1, y)
+        want, wbad = ref.grid(x0, z0, nx, nz, 1, y)
+        assert bad == wbad, group
+        assert np.array_equal(got, want), (group, int((got != want).sum()))
+
+
 # ---------------------------------------------------------------------------------------------------
 # ties, NaN, fixed source
 # ---------------------------------------------------------------------------------------------------
