@@ -70,6 +70,31 @@ def signed64(v):
     return v - (1 << 64) if v >= (1 << 63) else v
 
 
+def fnv1a64_rows(a):
+    """FNV-1a-64 of each row of a 2-D uint8 array (the per-seed hash of the engine fixtures)."""
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    h = np.full(a.shape[0], 0xcbf29ce484222325, np.uint64)
+    prime = np.uint64(0x100000001b3)
+    with np.errstate(over="ignore"):
+        for k in range(a.shape[1]):
+            h ^= a[:, k].astype(np.uint64)
+            h *= prime
+    return h
+
+
+def check_batched(version, work, host):
+    """Per-seed hashes of the timed output against the reference CPU engine's on the same inputs
+    (tests/golden/engine/batched_<version>.npz, engine/cuda/mcg_host.c built from its sources).
+    Returns the number of seeds whose output differs (0 = bit-exact), or None without the fixture."""
+    path = os.path.join(REPO, "tests", "golden", "engine", "batched_%s.npz" % version)
+    if not os.path.exists(path):
+        return None
+    with np.load(path) as z:
+        seeds, want = z["seeds"], z["hashes"]
+    assert seeds.tolist() == [w[0] for w in work], "fixture seeds differ from the workload"
+    return int((fnv1a64_rows(host.reshape(len(work), -1)) != want).sum())
+
+
 def uniform(d, r):
     return d % (2 * r + 1) - r
 
@@ -260,6 +285,10 @@ def bench_batched(args, torch, mcbiome):
         f64 = FLOPS[v]["f64"] * cols + PER_SAMPLE[v].get("f64", 0) * samp
         kern[v] = {"ms": ms, "f32_tflops": f32 / ms / 1e9, "f64_tflops": f64 / ms / 1e9,
                    "samples_per_s": samp / (ms / 1e3), "out_gbps": samp / (ms / 1e3) / 1e9}
+    # bit-exactness of the timed output (the last step's), per seed against the reference engine
+    mismatched = {v: check_batched(v, work, outs[v].cpu().numpy()) for v in VERSIONS}
+    bit_exact = all(m == 0 for m in mismatched.values())
+
     k = kern["26.3"]
     roofline = {"bound": "valu", "achieved": round(k["f32_tflops"], 3), "peak": PEAK["f32"], "unit": "TFLOP/s",
                 "frac": round(k["f32_tflops"] / PEAK["f32"], 4), "traffic": TRAFFIC_26_3,
@@ -296,6 +325,8 @@ def bench_batched(args, torch, mcbiome):
                    "autotuned": tuned, "graph": bool(args.graph), "parallelism": "single GPU"},
         "per_call_samples_per_s": round(per_call, 1) if per_call else None,
         "batched_over_per_call": round(value / per_call, 2) if per_call else None,
+        "bit_exact": bit_exact,
+        "mismatched_seeds": mismatched,
         "per_version_kernel_samples_per_s": {v: round(kern[v]["samples_per_s"], 1) for v in VERSIONS},
         "tile_latency_ms": lat,
         "roofline": roofline,

@@ -53,6 +53,17 @@ int mcg_biome_id(const char *name);
  * out_tg optional: 6 i32 per sample (t,h,c,e,d,w; Nether t,h,0,0,0,0; End zeros). */
 void mcg_ref_points(const McgHost *H, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out_biome,
                     int32_t *out_tg);
+/* replaces mcg_ref_tie_mask (engine/cuda/mcg_host.h:51, mcg_host.c:85-96): out = mask of the biome ids of
+ * every leaf at the least distance from the quantized target tg = {t,h,c,e,d,w} (bit b of out[b >> 6]),
+ * by brute force over all leaves with the reference's 64-bit integer distances.  Overworld and Nether. */
+void mcg_ref_tie_mask(const McgHost *H, const int32_t tg[6], uint64_t out[2]);
+/* The reference's mcg_ref_points / mcg_ref_tie_mask return void and never fail; these run on the device and
+ * can (no device, a device error).  By default a failure prints its message and calls abort(), so that it
+ * can never pass for a result.  A caller that wants control installs a handler: it receives the message,
+ * the call returns, and its outputs hold 0xFF bytes (not a biome id) / zero masks.  No counterpart in the
+ * reference.  fn = NULL restores the default. */
+typedef void (*McgRefErrorFn)(const char *message, void *user);
+void mcg_ref_set_error_handler(McgRefErrorFn fn, void *user);
 
 /* ================================================================================================
  * Native engine API (batched, device buffers).  Counterparts: gb_open / gb_points_gpu of
@@ -81,6 +92,9 @@ int mcb_engine_cells(const McbEngine *e, int64_t *cells, int64_t *list_len, doub
  * Overworld and Nether; |target| <= MCB_TARGET_MAX (distances stay exact in double). */
 #define MCB_TARGET_MAX (1 << 24)
 int mcb_targets(McbEngine *e, const int32_t *tg6, int n, uint8_t *out_biome);
+/* Tie masks of n quantized targets (mcg_ref_tie_mask batched): out_masks[2i], out_masks[2i+1] = biome-id
+ * mask of every least-distance leaf of target i.  Host buffers, synchronous; any int32 targets. */
+int mcb_tie_masks(McbEngine *e, const int32_t *tg6, int n, uint64_t *out_masks);
 
 /* End island field (engine/mc_end.h:66-106) over an nx x nz block grid of every seed, blocks
  * (bx0 + ix*step, bz0 + iz*step): kind 0 = getHeightValue at section (bx/8, bz/8) (C division),

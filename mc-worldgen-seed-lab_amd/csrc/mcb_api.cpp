@@ -21,13 +21,18 @@ using namespace mcb;
 namespace mcb {
 static thread_local std::string g_err;
 
-int fail(int code, const char *fmt, ...) {
+int failv(int code, const char *fmt, va_list ap) {
     char buf[512];
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    g_err = buf;
+    return code;
+}
+
+int fail(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
+    failv(code, fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -234,6 +239,7 @@ void mcb_close(McbEngine *e) {
     (void)hipFree(e->d_out);
     (void)hipFree(e->d_tg);
     (void)hipFree(e->d_field);
+    (void)hipFree(e->d_masks);
     (void)hipFree(e->d_feas);
     (void)hipFree(e->d_coord_lut);
     (void)hipFree(e->d_obs);
@@ -274,6 +280,19 @@ int mcb_targets(McbEngine *e, const int32_t *tg6, int n, uint8_t *out_biome) {
     MCB_HIPCHK(hipMemcpy(e->d_tg, tg6, (size_t)n * 6 * sizeof(int32_t), hipMemcpyHostToDevice));
     MCB_HIPCHK(launch_targets(e->w, e->d_tg, n, e->d_out, nullptr));
     MCB_HIPCHK(hipMemcpy(out_biome, e->d_out, (size_t)n, hipMemcpyDeviceToHost));
+    return MCB_OK;
+}
+
+int mcb_tie_masks(McbEngine *e, const int32_t *tg6, int n, uint64_t *out_masks) {
+    if (!e || n < 0 || (n && (!tg6 || !out_masks))) return fail(MCB_E_ARG, "bad arguments");
+    if (e->w.dim == MCB_END) return fail(MCB_E_ARG, "the End has no climate targets");
+    if (n == 0) return MCB_OK;
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (int rc = dev_grow(&e->d_tg, &e->tg_cap, (size_t)n * 6)) return rc;
+    if (int rc = dev_grow(&e->d_masks, &e->masks_cap, (size_t)n * 2)) return rc;
+    MCB_HIPCHK(hipMemcpy(e->d_tg, tg6, (size_t)n * 6 * sizeof(int32_t), hipMemcpyHostToDevice));
+    MCB_HIPCHK(launch_tie_masks(e->w, e->d_tg, n, e->d_masks, nullptr));
+    MCB_HIPCHK(hipMemcpy(out_masks, e->d_masks, (size_t)n * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return MCB_OK;
 }
 
@@ -349,17 +368,46 @@ void mcg_host_free(McgHost *H) {
     H->impl = nullptr;
 }
 
+// The reference's void entry points cannot fail; these can (no device, a device error).  A failure must
+// not look like a result: with no handler installed it is printed and the process aborts; with one, the
+// handler gets the message and the outputs hold 0xFF bytes (not a biome id) / empty masks.
+static McgRefErrorFn g_ref_err_fn = nullptr;
+static void *g_ref_err_ud = nullptr;
+
+void mcg_ref_set_error_handler(McgRefErrorFn fn, void *ud) {
+    g_ref_err_fn = fn;
+    g_ref_err_ud = ud;
+}
+
+static void ref_failure(const char *where) {
+    char msg[600];
+    snprintf(msg, sizeof msg, "%s: %s", where, g_err.c_str());
+    if (!g_ref_err_fn) {
+        fprintf(stderr, "%s\n", msg);
+        abort();
+    }
+    g_ref_err_fn(msg, g_ref_err_ud);
+}
+
 void mcg_ref_points(const McgHost *H, const int64_t *seeds, int ns, const int32_t *pts, int np, uint8_t *out_biome,
                     int32_t *out_tg) {
     if (!H || !H->impl) {
-        fprintf(stderr, "mcg_ref_points: engine not loaded\n");
-        abort();
+        fail(MCB_E_ARG, "engine not loaded");
+    } else if (mcb_points((McbEngine *)H->impl, seeds, ns, pts, np, out_biome, out_tg) == MCB_OK) {
+        return;
     }
-    if (mcb_points((McbEngine *)H->impl, seeds, ns, pts, np, out_biome, out_tg) != MCB_OK) {
-        // the reference returns void; a device failure must not look like a result
-        fprintf(stderr, "mcg_ref_points: %s\n", g_err.c_str());
-        abort();
+    if (out_biome && ns > 0 && np > 0) memset(out_biome, 0xFF, (size_t)ns * np);
+    ref_failure("mcg_ref_points");
+}
+
+void mcg_ref_tie_mask(const McgHost *H, const int32_t tg[6], uint64_t out[2]) {
+    if (out) out[0] = out[1] = 0;
+    if (!H || !H->impl || !tg || !out) {
+        fail(MCB_E_ARG, "bad arguments");
+    } else if (mcb_tie_masks((McbEngine *)H->impl, tg, 1, out) == MCB_OK) {
+        return;
     }
+    ref_failure("mcg_ref_tie_mask");
 }
 
 }  // extern "C"

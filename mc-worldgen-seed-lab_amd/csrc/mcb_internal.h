@@ -3,6 +3,8 @@
 #define MCB_INTERNAL_H
 #include <hip/hip_runtime.h>
 
+#include <cstdarg>
+
 #include <memory>
 #include <mutex>
 #include <string>
@@ -59,6 +61,7 @@ struct DevWorld {
 
 // error reporting: sets the thread-local message returned by mcb_last_error, returns `code`
 int fail(int code, const char *fmt, ...);
+int failv(int code, const char *fmt, va_list ap);
 std::string default_data_dir();
 
 #define MCB_HIPCHK(x)                                                                                    \
@@ -88,6 +91,7 @@ hipError_t launch_points(const DevWorld &w, const McbSlot *d_slots, const int32_
                          const int32_t *d_pts, int np, uint8_t *d_out, int32_t *d_tg, hipStream_t stream);
 // biomes of n quantized climate targets (6 int32 each: t,h,c,e,d,w); Overworld and Nether only
 hipError_t launch_targets(const DevWorld &w, const int32_t *d_tg, int n, uint8_t *d_out, hipStream_t stream);
+hipError_t launch_tie_masks(const DevWorld &w, const int32_t *d_tg, int n, uint64_t *d_out, hipStream_t stream);
 // End island field of ns seeds (slots 0..ns-1) over an nx x nz block grid; kind 0 height, 1 erosion
 hipError_t launch_end_field(const DevWorld &w, const McbSlot *d_slots, int ns, int bx0, int bz0, int nx, int nz, int step,
                             int kind, double *d_out, hipStream_t stream);
@@ -129,6 +133,8 @@ struct McbEngine {
     size_t cell_entries = 0, cell_list_len = 0;
     double *d_field = nullptr;
     size_t field_cap = 0;
+    uint64_t *d_masks = nullptr;
+    size_t masks_cap = 0;
     // host copies kept for the seed search's prefix tables
     std::shared_ptr<const mcb::CellTable> cells;
     std::vector<McbNode> tree_host;

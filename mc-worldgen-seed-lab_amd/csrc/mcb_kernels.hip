@@ -591,6 +591,63 @@ hipError_t launch_targets(const DevWorld &w, const int32_t *tg, int n, uint8_t *
     return hipGetLastError();
 }
 
+// Tie masks by brute force over every leaf (engine/cuda/mcg_host.c:85-96): one workgroup per target,
+// lanes stride over the pre-order node array; a leaf's distance is the reference's 64-bit integer sum of
+// per-dimension squared interval distances (wrapping like the reference's i64 on overflow), compared
+// signed.  Each lane keeps (least distance, biome mask); the workgroup folds them in LDS: the smaller
+// distance wins, equal distances OR their masks.
+__global__ __launch_bounds__(256) void k_tie_masks(const McbNode *__restrict__ N, int n, const int32_t *__restrict__ tg,
+                                                   uint64_t *__restrict__ out) {
+    __shared__ int64_t sd[256];
+    __shared__ uint64_t s0[256], s1[256];
+    const int lane = (int)threadIdx.x;
+    const int32_t *t = tg + 6 * (int64_t)blockIdx.x;
+    int64_t t6[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) t6[k] = t[k];
+    int64_t best = INT64_MAX;
+    uint64_t m0 = 0, m1 = 0;
+    for (int j = lane; j < n; j += 256) {
+        const McbNode nd = N[j];
+        if (!nd.leaf) continue;
+        uint64_t d = nd.d6;
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const int64_t lo = nd.lo[k], hi = nd.hi[k];
+            const int64_t x = t6[k] < lo ? lo - t6[k] : (t6[k] > hi ? t6[k] - hi : 0);
+            d += (uint64_t)x * (uint64_t)x;
+        }
+        const int64_t ds = (int64_t)d;
+        if (ds < best) { best = ds; m0 = m1 = 0; }
+        if (ds == best) {
+            const int b = nd.leaf & 0xFF;
+            if (b < 64) m0 |= 1ULL << b; else m1 |= 1ULL << (b - 64);
+        }
+    }
+    sd[lane] = best;
+    s0[lane] = m0;
+    s1[lane] = m1;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (lane < w) {
+            const int64_t a = sd[lane], b = sd[lane + w];
+            if (b < a) { sd[lane] = b; s0[lane] = s0[lane + w]; s1[lane] = s1[lane + w]; }
+            else if (b == a) { s0[lane] |= s0[lane + w]; s1[lane] |= s1[lane + w]; }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) {
+        out[2 * (int64_t)blockIdx.x] = s0[0];
+        out[2 * (int64_t)blockIdx.x + 1] = s1[0];
+    }
+}
+
+hipError_t launch_tie_masks(const DevWorld &w, const int32_t *tg, int n, uint64_t *out, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tie_masks, dim3(n), dim3(256), 0, st, w.d_tree, w.host.tree_n, tg, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_end_field(const DevWorld &w, const McbSlot *slots, int ns, int bx0, int bz0, int nx, int nz, int step,
                             int kind, double *out, hipStream_t st) {
     if (ns <= 0 || nx <= 0 || nz <= 0) return hipSuccess;

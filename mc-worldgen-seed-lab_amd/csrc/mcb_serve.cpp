@@ -21,6 +21,7 @@
 #include <memory>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "mcb_internal.h"
@@ -53,6 +54,7 @@ struct SlotPool {
     std::vector<int64_t> init_batch;
     std::list<int> lru;                                   // front = most recent
     std::vector<std::list<int>::iterator> pos;
+    int npinned = 0;                                      // slots with pins > 0
     int64_t hits = 0, misses = 0;
 
     int init(McbEngine *eng, int capacity) {
@@ -73,7 +75,7 @@ struct SlotPool {
         if (it != slot_of.end() && seed_of[it->second] == seed) {
             int s = it->second;
             lru.splice(lru.begin(), lru, pos[s]);
-            pins[s]++;
+            if (pins[s]++ == 0) npinned++;
             hits++;
             *fresh = 0;
             return s;
@@ -85,14 +87,29 @@ struct SlotPool {
             seed_of[s] = seed;
             slot_of[seed] = s;
             lru.splice(lru.begin(), lru, pos[s]);
-            pins[s]++;
+            if (pins[s]++ == 0) npinned++;
             misses++;
             *fresh = 1;
             return s;
         }
         return -1;
     }
-    void release(int s) { if (pins[s] > 0) pins[s]--; }
+    void release(int s) {
+        if (pins[s] > 0 && --pins[s] == 0) npinned--;
+    }
+    // forget a slot's seed: its state was never written (the batch that acquired it failed)
+    void unmap(int s) {
+        auto it = slot_of.find(seed_of[s]);
+        if (it != slot_of.end() && it->second == s) slot_of.erase(it);
+        seed_of[s] = INT64_MIN;
+        init_lane[s] = -1;
+        init_batch[s] = -1;
+    }
+    // true when `seed` holds a pinned slot (a batch in flight uses it)
+    bool pinned_seed(int64_t seed) const {
+        auto it = slot_of.find(seed);
+        return it != slot_of.end() && seed_of[it->second] == seed && pins[it->second] > 0;
+    }
 };
 
 struct Lane {
@@ -105,6 +122,7 @@ struct Lane {
     int64_t desc_cap = 0;
     std::vector<Req> reqs;
     std::vector<std::pair<int, int>> pinned;              // (world, slot)
+    std::vector<std::pair<int, int>> fresh;               // (world, slot) acquired unmapped by this batch
     bool busy = false;
     bool launched = false;                                // batch on the device, results not yet copied
     std::thread completer;
@@ -122,6 +140,7 @@ struct McbServer {
     std::condition_variable cv_submit, cv_done, cv_lane;
     std::deque<Req> pending;
     int inflight = 0;                                      // lanes with a launched batch
+    int64_t completions = 0;                               // batches finished (slots unpinned)
     std::unordered_map<int64_t, std::pair<int, double>> results;   // ticket -> (status, latency ms)
     std::vector<double> latencies;
     int64_t next_ticket = 1;
@@ -180,6 +199,7 @@ int launch_batch(McbServer *S, Lane &L) {
             if (slot < 0) return fail(MCB_E_LIMIT, "seed slot pool exhausted");
             L.pinned.push_back({B.world, slot});
             if (fresh) {
+                L.fresh.push_back({B.world, slot});
                 B.init_seeds.push_back(q.r.seed);
                 B.init_slots.push_back(slot);
                 B.pool->init_lane[slot] = my_lane;
@@ -252,7 +272,16 @@ int launch_batch(McbServer *S, Lane &L) {
     return 0;
 }
 
+// A failed batch leaves no seed mapped to a slot whose state it never wrote, and drops its pins.
+void abandon_batch(McbServer *S, Lane &L) {
+    for (auto &p : L.fresh) S->pools[p.first].unmap(p.second);
+    for (auto &p : L.pinned) S->pools[p.first].release(p.second);
+    L.fresh.clear();
+    L.pinned.clear();
+}
+
 void dispatcher_main(McbServer *S) {
+    int64_t blocked_at = -1;     // completions count when admission last found no free seed slot
     for (;;) {
         int li = -1;
         {
@@ -260,24 +289,50 @@ void dispatcher_main(McbServer *S) {
             S->cv_submit.wait(lk, [&] {
                 if (S->stop) return true;
                 if (S->pending.empty()) return false;
+                if (blocked_at >= 0 && S->completions == blocked_at) return false;
                 for (auto &L : S->lanes) if (!L.busy) return true;
                 return false;
             });
-            if (S->stop && S->pending.empty()) return;
             if (S->stop) return;
+            blocked_at = -1;
             for (size_t i = 0; i < S->lanes.size(); i++) if (!S->lanes[i].busy) { li = (int)i; break; }
             Lane &L = S->lanes[li];
             L.reqs.clear();
             L.pinned.clear();
+            L.fresh.clear();
             int64_t out = 0, stage = 0;
-            // memory-based admission: arrival order, while the lane arena holds the outputs and staging
+            // per world: the batch's distinct seeds, and how many of them need a slot that no batch in
+            // flight has pinned (an unmapped seed evicts one; a resident unpinned one pins its own)
+            std::unordered_map<int, std::unordered_set<int64_t>> seen;
+            std::unordered_map<int, int> need;
+            // admission in arrival order, while the lane arena holds the outputs and zoom staging
+            // (memory) and the world's unpinned seed slots hold the batch's new distinct seeds
             while (!S->pending.empty()) {
                 Req &q = S->pending.front();
                 if (!L.reqs.empty() && (out + q.out_bytes > S->batch_bytes || stage + q.stage_bytes > S->batch_bytes)) break;
+                const int w = q.r.world;
+                const bool new_seed = seen[w].count(q.r.seed) == 0;
+                bool takes_slot = false;
+                if (new_seed) {
+                    auto pit = S->pools.find(w);
+                    const SlotPool *P = (pit != S->pools.end() && pit->second.e) ? &pit->second : nullptr;
+                    const int unpinned = P ? P->cap - P->npinned : S->slot_capacity;
+                    takes_slot = !(P && P->pinned_seed(q.r.seed));
+                    if (takes_slot && need[w] + 1 > unpinned) break;
+                }
+                if (new_seed) {
+                    seen[w].insert(q.r.seed);
+                    if (takes_slot) need[w]++;
+                }
                 out += q.out_bytes;
                 stage += q.stage_bytes;
                 L.reqs.push_back(q);
                 S->pending.pop_front();
+            }
+            if (L.reqs.empty()) {
+                // every slot the next request could take is pinned by batches in flight: wait for one
+                blocked_at = S->completions;
+                continue;
             }
             L.busy = true;
             L.batch_id = ++S->batches;
@@ -290,7 +345,8 @@ void dispatcher_main(McbServer *S) {
             (void)hipStreamSynchronize(L.st);
             S->fatal = mcb_last_error();
             for (auto &q : L.reqs) S->results[q.ticket] = {rc, 0.0};
-            for (auto &p : L.pinned) S->pools[p.first].release(p.second);
+            abandon_batch(S, L);
+            S->completions++;
             L.busy = false;
             S->cv_done.notify_all();
             continue;
@@ -326,7 +382,13 @@ void completer_main(McbServer *S, int li) {
             S->results[L.reqs[i].ticket] = {e == hipSuccess ? 0 : (int)MCB_E_DEVICE, ms[i]};
             if (e == hipSuccess) S->latencies.push_back(ms[i]);
         }
-        for (auto &p : L.pinned) S->pools[p.first].release(p.second);
+        if (e != hipSuccess) {
+            S->fatal = hipGetErrorString(e);
+            abandon_batch(S, L);
+        } else {
+            for (auto &p : L.pinned) S->pools[p.first].release(p.second);
+        }
+        S->completions++;
         L.busy = false;
         S->cv_done.notify_all();
         S->cv_submit.notify_all();

@@ -27,43 +27,71 @@ using namespace mcb::plug;
 
 namespace {
 
-thread_local char g_err[512];
-void set_error(const char *fmt, ...) {
+// ---------------------------------------------------------------------------------------------------
+// errors and the device the plugin works on (contract of include/mcgpu.h)
+// ---------------------------------------------------------------------------------------------------
+// The plugin reports through the engine's per-thread error slot: mcgpu_last_error() and
+// mcb_last_error() read the same text.  Always returns false, so a failing step can `return
+// plugin_error(...)`.
+bool plugin_error(const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    mcb::failv(MCB_E_DEVICE, fmt, ap);
     va_end(ap);
+    return false;
 }
-std::mutex g_mu;
-int g_dev = -1;
-bool g_inited = false;
 
-bool ensure_init() {
-    if (g_inited) {
-        if (hipSetDevice(g_dev) != hipSuccess) { set_error("hipSetDevice(%d) failed", g_dev); return false; }
+// One device per process.  mcgpu_select binds it, once, before any work; the first entry point that
+// needs a device without a selection binds device 0.  Binding is checked by a memory query on the
+// device, which also makes the runtime create its context there, so a bad device fails at selection
+// rather than inside the first grid.
+class DeviceBinding {
+public:
+    int device() const { return dev_; }
+    bool bind(int want) {
+        const int n = mcb_device_count();
+        if (n <= 0) return plugin_error("no HIP device");
+        if (want < 0 || want >= n) return plugin_error("no device %d (this process sees %d)", want, n);
+        size_t free_b = 0, total_b = 0;
+        hipError_t e = hipSetDevice(want);
+        if (e == hipSuccess) e = hipMemGetInfo(&free_b, &total_b);
+        if (e != hipSuccess) return plugin_error("device %d unusable: %s", want, hipGetErrorString(e));
+        dev_ = want;
         return true;
     }
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess || n <= 0) {
-        set_error("no HIP devices: %s", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
-        return false;
+    // make the bound device (binding device 0 if none is) current on the calling thread
+    bool make_current() {
+        if (dev_ < 0) return bind(0);
+        const hipError_t e = hipSetDevice(dev_);
+        return e == hipSuccess || plugin_error("device %d: %s", dev_, hipGetErrorString(e));
     }
-    if (g_dev < 0) g_dev = 0;
-    if (g_dev >= n) { set_error("device %d does not exist (%d devices)", g_dev, n); return false; }
-    e = hipSetDevice(g_dev);
-    if (e != hipSuccess) { set_error("hipSetDevice(%d): %s", g_dev, hipGetErrorString(e)); return false; }
-    e = hipFree(nullptr);
-    if (e != hipSuccess) { set_error("device %d initialisation: %s", g_dev, hipGetErrorString(e)); return false; }
-    g_inited = true;
-    return true;
+
+private:
+    int dev_ = -1;
+};
+
+DeviceBinding &binding() {
+    static DeviceBinding b;
+    return b;
 }
+
+// Every entry point runs inside a Gate: the process-wide lock (entries are serialised, as the
+// reference's are) and, for entries that touch the device, the bound device made current.
+std::mutex &gate_mutex() {
+    static std::mutex m;
+    return m;
+}
+struct Gate {
+    std::unique_lock<std::mutex> lock;
+    bool ready;
+    explicit Gate(bool needs_device) : lock(gate_mutex()), ready(!needs_device || binding().make_current()) {}
+};
 
 #define HCHK(call)                                                                                  \
     do {                                                                                            \
         hipError_t e_ = (call);                                                                     \
         if (e_ != hipSuccess) {                                                                     \
-            set_error("%s: %s", #call, hipGetErrorString(e_));                                     \
+            plugin_error("%s: %s", #call, hipGetErrorString(e_));                                     \
             return false;                                                                           \
         }                                                                                           \
     } while (0)
@@ -115,14 +143,14 @@ struct Compiled {
 };
 
 bool spline_depth(const McgProg *p, int spi, int narr_node, int level, int *depth) {
-    if (spi < 0 || spi >= p->nsp) { set_error("spline index %d out of range", spi); return false; }
-    if (level > kSplineDepth) { set_error("spline nesting deeper than %d", kSplineDepth); return false; }
+    if (spi < 0 || spi >= p->nsp) { plugin_error("spline index %d out of range", spi); return false; }
+    if (level > kSplineDepth) { plugin_error("spline nesting deeper than %d", kSplineDepth); return false; }
     *depth = std::max(*depth, level);
     const McgSpline &s = p->sp[spi];
     if (s.is_const) return true;
     if (s.n < 1 || s.loc0 < 0 || (int64_t)s.loc0 + 2 * (int64_t)s.n > p->nsplf || s.val0 < 0 ||
         (int64_t)s.val0 + s.n > p->nspc || s.coord < 0 || s.coord >= narr_node) {
-        set_error("spline %d: points or coordinate out of range", spi);
+        plugin_error("spline %d: points or coordinate out of range", spi);
         return false;
     }
     for (int i = 0; i < s.n; i++)
@@ -131,7 +159,7 @@ bool spline_depth(const McgProg *p, int spi, int narr_node, int level, int *dept
 }
 
 bool check_noise(const McgProg *p, int ns) {
-    if (ns < 0 || ns >= p->nnoise) { set_error("noise index %d out of range", ns); return false; }
+    if (ns < 0 || ns >= p->nnoise) { plugin_error("noise index %d out of range", ns); return false; }
     const McgNoise &mn = p->noise[ns];
     auto range_ok = [&](int l0, int n, bool allow_empty) {
         if (n < 0 || l0 < 0 || (int64_t)l0 + n > p->nlayers) return false;
@@ -143,7 +171,7 @@ bool check_noise(const McgProg *p, int ns) {
         return true;
     };
     const bool ok = p->old ? (range_ok(mn.l0, mn.n0, true) && range_ok(mn.l1, mn.n1, true)) : range_ok(mn.l0, mn.n0, false);
-    if (!ok) set_error("noise %d: layers or octaves out of range", ns);
+    if (!ok) plugin_error("noise %d: layers or octaves out of range", ns);
     return ok;
 }
 
@@ -170,30 +198,30 @@ bool compile_prog(const McgProg *p, Compiled &C) {
         p->nspc < 0 || p->narr < 0 || p->nthr < 0 || (p->nnoise && !p->noise) || (p->nlayers && !p->layers) ||
         (p->noct && !p->octs) || (p->nsp && !p->sp) || (p->nsplf && !p->splf) || (p->nspc && !p->spc) ||
         (p->narr && !p->arr) || (p->nthr && !p->thr)) {
-        set_error("malformed program (counts or arrays)");
+        plugin_error("malformed program (counts or arrays)");
         return false;
     }
     std::vector<int> last(n, -1);
     auto use = [&](int user, int j) -> bool {
-        if (j < 0 || j >= user) { set_error("node %d reads node %d (must be an earlier node)", user, j); return false; }
+        if (j < 0 || j >= user) { plugin_error("node %d reads node %d (must be an earlier node)", user, j); return false; }
         last[j] = std::max(last[j], user);
         return true;
     };
     for (int i = 0; i < n; i++) {
         const McgNode &s = p->nodes[i];
         if (p->old) {
-            if (s.k < 0 || s.k >= MCGO__COUNT) { set_error("unknown node kind %d", s.k); return false; }
+            if (s.k < 0 || s.k >= MCGO__COUNT) { plugin_error("unknown node kind %d", s.k); return false; }
         } else {
             switch (s.k) {
             case MCGK_INTERP: case MCGK_FTS: case MCGK_LOG: case MCGK_POW_CB: case MCGK_POW_CE: case MCGK_POW:
             case MCGK_ROUND_INT: case MCGK_ROUND:
-                set_error("node kind %d is not supported by the point evaluator", s.k);
+                plugin_error("node kind %d is not supported by the point evaluator", s.k);
                 return false;
             default:
-                if (s.k < 0 || s.k >= MCGK__COUNT) { set_error("unknown node kind %d", s.k); return false; }
+                if (s.k < 0 || s.k >= MCGK__COUNT) { plugin_error("unknown node kind %d", s.k); return false; }
             }
             if ((s.k == MCGK_GRAD_REPEAT || s.k == MCGK_GRAD_MIRROR) && s.i2 == 0) {
-                set_error("node %d: gradient period 0", i);
+                plugin_error("node %d: gradient period 0", i);
                 return false;
             }
         }
@@ -202,22 +230,22 @@ bool compile_prog(const McgProg *p, Compiled &C) {
         if ((u & OP_B) && !use(i, s.b)) return false;
         if ((u & OP_C) && !use(i, s.c)) return false;
         if ((u & OP_NS) && !check_noise(p, s.ns)) return false;
-        if ((u & OP_OCT) && (s.oct < 0 || s.oct >= p->noct)) { set_error("node %d: octave out of range", i); return false; }
+        if ((u & OP_OCT) && (s.oct < 0 || s.oct >= p->noct)) { plugin_error("node %d: octave out of range", i); return false; }
         if (u & (OP_SPL | OP_ISEL)) {
-            if (s.narr < 1 || s.arr0 < 0 || (int64_t)s.arr0 + s.narr > p->narr) { set_error("node %d: list out of range", i); return false; }
+            if (s.narr < 1 || s.arr0 < 0 || (int64_t)s.arr0 + s.narr > p->narr) { plugin_error("node %d: list out of range", i); return false; }
             for (int q = 0; q < s.narr; q++)
                 if (!use(i, p->arr[s.arr0 + q])) return false;
         }
         if ((u & OP_ISEL) && (s.nthr < 0 || s.thr0 < 0 || (int64_t)s.thr0 + s.nthr > p->nthr)) {
-            set_error("node %d: thresholds out of range", i);
+            plugin_error("node %d: thresholds out of range", i);
             return false;
         }
-        if ((u & OP_ISEL) && s.nthr >= s.narr) { set_error("node %d: more thresholds than choices", i); return false; }
+        if ((u & OP_ISEL) && s.nthr >= s.narr) { plugin_error("node %d: more thresholds than choices", i); return false; }
         if ((u & OP_SPL) && !spline_depth(p, s.sp, s.narr, 1, &C.spl_depth)) return false;
     }
     for (int k = 0; k < 6; k++) {
         const int r = k < p->nroots ? p->root[k] : -1;
-        if (r < -1 || r >= n) { set_error("root %d out of range", k); return false; }
+        if (r < -1 || r >= n) { plugin_error("root %d out of range", k); return false; }
         if (r >= 0) last[r] = n;                              // live to the end
     }
     // y-independent nodes: no read of the block y (or y fixed by the coordinate mask, or scaled by 0, which
@@ -314,17 +342,17 @@ bool compile_prog(const McgProg *p, Compiled &C) {
 
 // BFS tree (children consecutive) -> pre-order with escape links, children in their stored order
 bool convert_tree(const McgTreeNode *T, int n, int root, std::vector<PNode> &out) {
-    if (n <= 0 || !T || root < 0 || root >= n) { set_error("malformed R-tree"); return false; }
+    if (n <= 0 || !T || root < 0 || root >= n) { plugin_error("malformed R-tree"); return false; }
     std::vector<char> seen(n, 0);
     std::vector<std::pair<int, int>> st;          // (node, next child) ; out index kept in order stack
     std::vector<int> outidx;
     out.clear();
     auto enter = [&](int v) -> bool {
-        if (seen[v]) { set_error("R-tree node %d reached twice", v); return false; }
+        if (seen[v]) { plugin_error("R-tree node %d reached twice", v); return false; }
         seen[v] = 1;
         const McgTreeNode &m = T[v];
         if (m.count < 0 || (m.count > 0 && (m.first < 0 || (int64_t)m.first + m.count > n))) {
-            set_error("R-tree node %d: children out of range", v);
+            plugin_error("R-tree node %d: children out of range", v);
             return false;
         }
         PNode p;
@@ -426,26 +454,26 @@ std::shared_ptr<const mcb::CellTable> plugin_cells(const std::vector<PNode> &T) 
 }
 bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
     if (W->source != MCG_BS_MULTI && W->source != MCG_BS_END && W->source != MCG_BS_FIXED) {
-        set_error("unknown biome source %d", W->source);
+        plugin_error("unknown biome source %d", W->source);
         return false;
     }
-    if (W->qh <= 0) { set_error("world height must be positive"); return false; }
+    if (W->qh <= 0) { plugin_error("world height must be positive"); return false; }
     if (W->source == MCG_BS_FIXED) return true;
     const McgProg *p = W->prog;
-    if (!p) { set_error("no program"); return false; }
-    if ((p->old != 0) == (W->newf != 0)) { set_error("program branch (old=%d) does not match newf=%d", p->old, W->newf); return false; }
+    if (!p) { plugin_error("no program"); return false; }
+    if ((p->old != 0) == (W->newf != 0)) { plugin_error("program branch (old=%d) does not match newf=%d", p->old, W->newf); return false; }
     if (!compile_prog(p, H.C)) return false;
-    if (W->source == MCG_BS_END && H.C.root_slot[3] < 0) { set_error("no erosion field"); return false; }
+    if (W->source == MCG_BS_END && H.C.root_slot[3] < 0) { plugin_error("no erosion field"); return false; }
     if (W->source == MCG_BS_MULTI)
         for (int k = 0; k < 6; k++)
-            if (H.C.root_slot[k] < 0) { set_error("no climate field %d", k); return false; }
+            if (H.C.root_slot[k] < 0) { plugin_error("no climate field %d", k); return false; }
     if (W->source == MCG_BS_MULTI) {
         if (!convert_tree(W->tree, W->ntree, W->tree_root, H.tree)) return false;
         H.cells = plugin_cells(H.tree);
     }
     const int vbytes = H.C.nslot * kWG * (W->newf ? 4 : 8);
     if (grad_lds_bytes(W->newf != 0) + vbytes > kMaxLdsBytes) {
-        set_error("program keeps %d values alive at once; at most %d fit", H.C.nslot,
+        plugin_error("program keeps %d values alive at once; at most %d fit", H.C.nslot,
                   (kMaxLdsBytes - grad_lds_bytes(W->newf != 0)) / (kWG * (W->newf ? 4 : 8)));
         return false;
     }
@@ -455,7 +483,7 @@ bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
 
 // The tree and its cell table depend only on the tree, not on the seed: one device copy per (device,
 // table) serves every world built over it for the rest of the process (a world per call, as the chunk
-// path makes, would otherwise upload several megabytes each time).  Guarded by g_mu.
+// path makes, would otherwise upload several megabytes each time).  Guarded by the entry gate.
 struct SharedTables {
     char *d = nullptr;
     size_t ntree = 0;
@@ -468,7 +496,7 @@ struct SharedTables {
 const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
     *failed = false;
     static std::map<std::pair<int, const mcb::CellTable *>, SharedTables> cache;
-    const auto key = std::make_pair(g_dev, H.cells.get());
+    const auto key = std::make_pair(binding().device(), H.cells.get());
     auto it = cache.find(key);
     if (it != cache.end()) return it->second.ntree == H.tree.size() ? &it->second : nullptr;
     const mcb::CellTable &ct = *H.cells;
@@ -479,14 +507,14 @@ const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
     for (int i = 0; i < 4; i++) { off[i] = total; total += (b[i] + 255) & ~(size_t)255; }
     SharedTables s;
     if (hipMalloc((void **)&s.d, std::max<size_t>(total, 256)) != hipSuccess) {
-        set_error("shared table allocation failed");
+        plugin_error("shared table allocation failed");
         *failed = true;
         return nullptr;
     }
     for (int i = 0; i < 4; i++)
         if (b[i] && hipMemcpy(s.d + off[i], src[i], b[i], hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipFree(s.d);
-            set_error("shared table upload failed");
+            plugin_error("shared table upload failed");
             *failed = true;
             return nullptr;
         }
@@ -671,124 +699,128 @@ bool run_chunks(GBiome *g, int n, const int32_t *cxz, uint8_t *out, int *nunres)
 }  // namespace
 
 void *mcb::plug::world_new(const McgBiomeWorld *W, int chain_cap, char *err, size_t errlen) {
-    std::lock_guard<std::mutex> lk(g_mu);
+    Gate gate(true);
     HostWorld H;
     std::unique_ptr<GBiome> g(new GBiome());
-    if (W && prepare_world(W, H) && ensure_init() && build_world(g.get(), W, H)) {
+    if (!W) plugin_error("null world");
+    else if (gate.ready && prepare_world(W, H) && build_world(g.get(), W, H)) {
         g->W.chain_cap = chain_cap;
         return g.release();
     }
-    if (!W) set_error("null world");
     g->release();
-    if (err && errlen) snprintf(err, errlen, "%s", g_err);
+    if (err && errlen) snprintf(err, errlen, "%s", mcb_last_error());
     return nullptr;
 }
 
 int mcb::plug::world_chunks(void *h, int n, const int32_t *cxz, uint8_t *out, int *nunres) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (!h || n < 0 || (n && (!cxz || !out))) { set_error("bad arguments"); return 1; }
+    Gate gate(true);
+    if (!h || n < 0 || (n && (!cxz || !out))) return plugin_error("bad arguments") ? 0 : 1;
     if (n == 0) {
         if (nunres) *nunres = 0;
         return 0;
     }
-    if (!ensure_init()) return 1;
-    return run_chunks((GBiome *)h, n, cxz, out, nunres) ? 0 : 1;
+    return gate.ready && run_chunks((GBiome *)h, n, cxz, out, nunres) ? 0 : 1;
 }
 
 int mcb::plug::chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, uint8_t *out, int *nunres, char *err,
                             size_t errlen) {
-    std::lock_guard<std::mutex> lk(g_mu);
+    Gate gate(true);
     bool ok = false;
     HostWorld H;
     std::unique_ptr<GBiome> g(new GBiome());
-    if (n < 0 || (n && (!cxz || !out))) set_error("bad arguments");
-    else if (prepare_world(W, H) && ensure_init() && build_world(g.get(), W, H)) {
+    if (n < 0 || (n && (!cxz || !out))) plugin_error("bad arguments");
+    else if (gate.ready && prepare_world(W, H) && build_world(g.get(), W, H)) {
         g->W.chain_cap = 1 << 30;            // a chunk's fill order is finite: the walk ends at its first cell
         ok = n == 0 || run_chunks(g.get(), n, cxz, out, nunres);
         if (n == 0 && nunres) *nunres = 0;
     }
     g->release();
-    if (!ok && err && errlen) snprintf(err, errlen, "%s", g_err);
+    if (!ok && err && errlen) snprintf(err, errlen, "%s", mcb_last_error());
     return ok ? 0 : 1;
 }
+
+namespace {
+size_t to_mib(size_t bytes) { return bytes / ((size_t)1 << 20); }
+}  // namespace
 
 extern "C" {
 
 int mcgpu_abi(void) { return MCGPU_ABI; }
 
-int mcgpu_device_count(void) {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
-    return n;
-}
+int mcgpu_device_count(void) { return mcb_device_count(); }
 
-int mcgpu_device_info(int i, char *name, size_t namelen, int *cc_major, int *cc_minor, size_t *mem_mb) {
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, i) != hipSuccess) { (void)hipGetLastError(); return 1; }
-    if (name && namelen) { strncpy(name, p.name, namelen - 1); name[namelen - 1] = 0; }
-    if (cc_major) *cc_major = p.major;
-    if (cc_minor) *cc_minor = p.minor;
-    if (mem_mb) *mem_mb = p.totalGlobalMem >> 20;
+int mcgpu_device_info(int index, char *name, size_t namelen, int *cc_major, int *cc_minor, size_t *mem_mb) {
+    if (index < 0 || index >= mcb_device_count()) return plugin_error("no device %d", index) ? 0 : 1;
+    int major = 0, minor = 0;
+    size_t bytes = 0;
+    char label[256] = {0};
+    if (hipDeviceGetAttribute(&major, hipDeviceAttributeComputeCapabilityMajor, index) != hipSuccess ||
+        hipDeviceGetAttribute(&minor, hipDeviceAttributeComputeCapabilityMinor, index) != hipSuccess ||
+        hipDeviceTotalMem(&bytes, index) != hipSuccess || hipDeviceGetName(label, (int)sizeof label - 1, index) != hipSuccess)
+        return plugin_error("device %d: properties unavailable", index) ? 0 : 1;
+    if (name && namelen) snprintf(name, namelen, "%s", label);
+    if (cc_major) *cc_major = major;
+    if (cc_minor) *cc_minor = minor;
+    if (mem_mb) *mem_mb = to_mib(bytes);
     return 0;
 }
 
 int mcgpu_select(int device) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (g_inited && device == g_dev) return 0;
-    if (g_inited) { set_error("device already selected (%d)", g_dev); return 1; }
-    g_dev = device;
-    return ensure_init() ? 0 : 1;
+    Gate gate(false);
+    const int bound = binding().device();
+    if (bound >= 0)
+        return device == bound ? 0 : (plugin_error("device %d is bound; the selection is made once, before any work", bound) ? 0 : 1);
+    return binding().bind(device) ? 0 : 1;
 }
 
-int mcgpu_selected(void) { return g_inited ? g_dev : -1; }
+int mcgpu_selected(void) {
+    Gate gate(false);
+    return binding().device();
+}
 
-const char *mcgpu_last_error(void) { return g_err; }
+const char *mcgpu_last_error(void) { return mcb_last_error(); }
 
 int mcgpu_mem_info(size_t *free_mb, size_t *total_mb) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (!ensure_init()) return 1;
-    size_t f = 0, t = 0;
-    if (hipMemGetInfo(&f, &t) != hipSuccess) { set_error("hipMemGetInfo failed"); return 1; }
-    if (free_mb) *free_mb = f >> 20;
-    if (total_mb) *total_mb = t >> 20;
+    Gate gate(true);
+    if (!gate.ready) return 1;
+    size_t free_b = 0, total_b = 0;
+    const hipError_t e = hipMemGetInfo(&free_b, &total_b);
+    if (e != hipSuccess) return plugin_error("memory query: %s", hipGetErrorString(e)) ? 0 : 1;
+    if (free_mb) *free_mb = to_mib(free_b);
+    if (total_mb) *total_mb = to_mib(total_b);
     return 0;
 }
 
 void *mcgpu_biome_new(const McgBiomeWorld *W, char *err, size_t errlen) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto fail = [&]() -> void * {
-        if (err && errlen) snprintf(err, errlen, "%s", g_err);
-        return nullptr;
-    };
-    if (!W) { set_error("null world"); return fail(); }
+    Gate gate(false);
     HostWorld H;
-    if (!prepare_world(W, H)) return fail();               // everything about the description, on the host
-    if (!ensure_init()) return fail();
     std::unique_ptr<GBiome> g(new GBiome());
-    if (!build_world(g.get(), W, H)) { g->release(); return fail(); }
-    return g.release();
+    // the description is checked on the host before anything touches the device
+    if (!W) plugin_error("null world");
+    else if (prepare_world(W, H) && binding().make_current() && build_world(g.get(), W, H)) return g.release();
+    g->release();
+    if (err && errlen) snprintf(err, errlen, "%s", mcb_last_error());
+    return nullptr;
 }
 
 void mcgpu_biome_free(void *h) {
     if (!h) return;
-    std::lock_guard<std::mutex> lk(g_mu);
+    Gate gate(true);
     GBiome *g = (GBiome *)h;
-    if (ensure_init()) g->release();
+    if (gate.ready) g->release();
     delete g;
 }
 
 int mcgpu_biome_grid(void *h, int x0, int z0, int nx, int nz, int step, int y, unsigned char *out, int *nunres) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (!h || nx <= 0 || nz <= 0 || step <= 0 || !out) { set_error("bad arguments"); return 1; }
-    if (!ensure_init()) return 1;
-    return run_grid((GBiome *)h, x0, z0, nx, nz, step, y, out, nunres) ? 0 : 1;
+    Gate gate(true);
+    if (!h || nx <= 0 || nz <= 0 || step <= 0 || !out) return plugin_error("bad arguments") ? 0 : 1;
+    return gate.ready && run_grid((GBiome *)h, x0, z0, nx, nz, step, y, out, nunres) ? 0 : 1;
 }
 
 int mcgpu_biome_points(void *h, int n, const int *xyz, unsigned char *out, int *nunres) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (!h || n < 0 || (n > 0 && (!xyz || !out))) { set_error("bad arguments"); return 1; }
-    if (!ensure_init()) return 1;
-    return run_points((GBiome *)h, n, xyz, out, nunres) ? 0 : 1;
+    Gate gate(true);
+    if (!h || n < 0 || (n > 0 && (!xyz || !out))) return plugin_error("bad arguments") ? 0 : 1;
+    return gate.ready && run_points((GBiome *)h, n, xyz, out, nunres) ? 0 : 1;
 }
 
 }  // extern "C"

@@ -134,6 +134,9 @@ def lib():
         L.mcg_host_load.argtypes = [C.POINTER(McgHost), C.c_int, C.c_int, C.c_char_p]
         L.mcg_host_free.argtypes = [C.POINTER(McgHost)]
         L.mcg_ref_points.argtypes = [C.POINTER(McgHost), i64p, C.c_int, i32p, C.c_int, u8p, i32p]
+        L.mcg_ref_tie_mask.argtypes = [C.POINTER(McgHost), i32p, C.POINTER(C.c_uint64)]
+        L.mcg_ref_set_error_handler.argtypes = [C.c_void_p, C.c_void_p]
+        L.mcb_tie_masks.argtypes = [P, i32p, C.c_int, C.POINTER(C.c_uint64)]
         L.mcg_biome_name.restype = C.c_char_p
         L.mcg_biome_id.argtypes = [C.c_char_p]
         L.mcg_version_name.restype = C.c_char_p
@@ -195,6 +198,13 @@ class Host:
                              _p(tg, C.c_int32) if want_tg else None)
         out = out.reshape(ns, np_)
         return (out, tg.reshape(ns, np_, 6)) if want_tg else out
+
+    def tie_mask(self, tg6):
+        """mcg_ref_tie_mask: (mask lo, mask hi) of the biomes of every least-distance leaf."""
+        t = np.ascontiguousarray(tg6, dtype=np.int32).reshape(6)
+        m = (C.c_uint64 * 2)()
+        lib().mcg_ref_tie_mask(C.byref(self.h), _p(t, C.c_int32), m)
+        return m[0], m[1]
 
     def close(self):
         if self.h.impl:
@@ -272,6 +282,13 @@ class Engine:
         out = np.zeros(t.shape[0], np.uint8)
         _check(lib().mcb_targets(self.h, _p(t, C.c_int32), t.shape[0], _p(out, C.c_uint8)))
         return out
+
+    def tie_masks(self, tg):
+        """(n, 2) uint64 biome masks of every least-distance leaf of each quantized target."""
+        t = np.ascontiguousarray(tg, dtype=np.int32).reshape(-1, 6)
+        out = np.zeros(2 * t.shape[0], np.uint64)
+        _check(lib().mcb_tie_masks(self.h, _p(t, C.c_int32), t.shape[0], out.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return out.reshape(-1, 2)
 
     def search(self, obs, seeds=None, base=0, n=0, structural=False, flags=0, max_found=1 << 16):
         """Seed search: candidates = `seeds` (list), or base|i<<48 (structural), or base+i for i < n.

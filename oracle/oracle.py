@@ -255,6 +255,7 @@ class RefEngine:
             L.mcg_host_free.argtypes = [C.c_void_p]
             L.mcg_ref_points.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int32), C.c_int,
                                          C.POINTER(C.c_uint8), C.POINTER(C.c_int32)]
+            L.mcg_ref_tie_mask.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_uint64)]
             RefEngine._L = L
         # The host struct embeds the climate spec and spline tables; 4 MiB covers it with margin.
         self.buf = C.create_string_buffer(4 << 20)
@@ -271,6 +272,12 @@ class RefEngine:
                                     _p(out, C.c_uint8), _p(tg, C.c_int32) if want_tg else None)
         out = out.reshape(ns, np_)
         return (out, tg.reshape(ns, np_, 6)) if want_tg else out
+
+    def tie_mask(self, tg6):
+        t = np.ascontiguousarray(tg6, dtype=np.int32).reshape(6)
+        m = (C.c_uint64 * 2)()
+        RefEngine._L.mcg_ref_tie_mask(self.buf, _p(t, C.c_int32), m)
+        return m[0], m[1]
 
     def close(self):
         if self.buf is not None:

@@ -3,13 +3,17 @@
 vectors/<V>/*      : the reference's own Java-oracle vectors (tests/vectors/<V>/ in the reference),
                      copied verbatim as data; misc_biome.jsonl keeps only its hashed/blockbiome records.
 unit/ref26{2,3}.txt: the reference's NormalNoise unit KATs (tests/unit/ in the reference).
-engine/*.npz       : outputs of the reference CPU engine built from its sources (oracle/_ref),
-                     written by tests/golden/make_engine_fixtures.py.
+engine/*.npz       : outputs of the reference CPU engine built from its sources (oracle/_ref) at the
+                     benchmark workloads' own inputs, written by tests/golden/make_engine_fixtures.py
+                     (plain_26.3: the one-seed 256x256 grid; batched_<V>: per-seed hashes of the
+                     4096-seed batch).
 """
 import json
 import os
 import struct
 from collections import defaultdict
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
@@ -110,3 +114,47 @@ UNIT_NOISES = {
     "ridge": (-7, [1, 2, 1, 0, 0, 0], 0.9147152149950137),
     "offset": (-3, [1, 1, 1, 0], 0.9381732587751005),
 }
+
+
+# ---- splitmix64 workloads and the per-seed output hash of the engine fixtures ----
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(k, n):
+    """First n outputs (unsigned) of the splitmix64 stream with state0 = k: state += 0x9E3779B97F4A7C15,
+    then the finalizer of engine/mc_rng.h:64-68 (the 30/27/31 shifts)."""
+    st, out = k & MASK64, []
+    for _ in range(n):
+        st = (st + 0x9E3779B97F4A7C15) & MASK64
+        z = st
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def signed64(v):
+    return v - (1 << 64) if v >= (1 << 63) else v
+
+
+def uniform(d, r):
+    """Uniform integer in [-r, r] from one unsigned draw."""
+    return d % (2 * r + 1) - r
+
+
+def fnv1a64_rows(a):
+    """FNV-1a-64 of each row of a 2-D uint8 array (offset basis 0xcbf29ce484222325, prime 0x100000001b3)."""
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    h = np.full(a.shape[0], 0xcbf29ce484222325, np.uint64)
+    prime = np.uint64(0x100000001b3)
+    with np.errstate(over="ignore"):
+        for k in range(a.shape[1]):
+            h ^= a[:, k].astype(np.uint64)
+            h *= prime
+    return h
+
+
+def engine_fixture(name):
+    """tests/golden/engine/<name>.npz as a dict of arrays."""
+    with np.load(os.path.join(GOLDEN, "engine", name + ".npz")) as z:
+        return {k: z[k] for k in z.files}

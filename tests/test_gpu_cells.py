@@ -70,3 +70,30 @@ def test_nether_targets_and_limits():
         eng.targets(np.array([[lim + 1, 0, 0, 0, 0, 0]], np.int32))
     with pytest.raises(mcbiome.EngineError):
         mcbiome.Engine("26.3", "the_end").targets(np.zeros((1, 6), np.int32))
+
+
+@pytest.mark.parametrize("version", G.VERSIONS)
+@pytest.mark.parametrize("dim", ["overworld", "the_nether"])
+def test_tie_masks_match_oracle(version, dim):
+    """mcb_tie_masks and the mcg_ref_tie_mask drop-in (engine/cuda/mcg_host.c:85-96) equal the oracle's
+    brute-force tie mask (pinned against the reference engine's on the CPU): adversarial targets on and
+    between breakpoints (many ties), targets of real samples, and int32-magnitude targets."""
+    eng = mcbiome.Engine(version, dim)
+    orc = O.Oracle(version, dim)
+    rng = np.random.default_rng(60 + G.VERSIONS.index(version))
+    if dim == "overworld":
+        tg = adversarial_targets(orc, rng, 4000)
+    else:
+        tg = rng.integers(-20000, 20000, (4000, 6)).astype(np.int32)
+        tg[:, 2:] = 0
+    far = rng.integers(-2**30, 2**30, (64, 6)).astype(np.int32)
+    tg = np.concatenate([tg, far]).astype(np.int32)
+    got = eng.tie_masks(tg)
+    for i, t in enumerate(tg):
+        _, m = orc.tie_mask(t)
+        assert (int(got[i, 0]), int(got[i, 1])) == m, (i, t.tolist())
+    host = mcbiome.Host(version, dim)
+    for t in tg[:50]:
+        assert host.tie_mask(t) == orc.tie_mask(t)[1]
+    n_ties = sum(bin(int(a)).count("1") + bin(int(b)).count("1") > 1 for a, b in got)
+    print("targets with several tied biomes:", n_ties, "of", len(tg), flush=True)

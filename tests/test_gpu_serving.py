@@ -116,3 +116,34 @@ def test_server_rejects_oversized_request():
     with pytest.raises(mcbiome.EngineError):
         srv.submit(r, np.zeros(128 * 128, np.uint8))
     srv.close()
+
+
+def test_server_slot_pressure_defers_and_stays_exact():
+    """More distinct seeds than seed slots: admission defers requests until slots are unpinned instead of
+    failing a batch, so no seed ever maps to a slot whose state was not written for it.  Each request is
+    submitted twice (the second round hits or re-initialises slots evicted meanwhile); every result
+    equals the request computed alone."""
+    rng = np.random.default_rng(77)
+    ctx = mcbiome.Context()
+    w = ctx.world("26.3", "overworld")
+    seeds = rng.integers(-2**63, 2**63 - 1, 24, dtype=np.int64)
+    regs = []
+    for i, s in enumerate(seeds):
+        r = mcbiome.make_region(int(s), int(rng.integers(-2**18, 2**18)), int(rng.integers(-2**18, 2**18)), 16, 16,
+                                [16, -8][: 1 + i % 2])
+        r.world = w
+        regs.append(r)
+    # a batch arena of 64 KiB holds every request at once; 5 slots hold 5 of the 24 seeds
+    srv = mcbiome.Server(ctx, batch_bytes=1 << 16, lanes=3, slot_capacity=5)
+    engines = {}
+    want = [_alone(engines, "26.3", "overworld", r) for r in regs]
+    for rnd in range(2):
+        bufs = [np.zeros(r.ny * r.nx * r.nz, np.uint8) for r in regs]
+        tickets = [srv.submit(r, b) for r, b in zip(regs, bufs)]
+        for t in tickets:
+            srv.wait(t)
+        for i, (r, b) in enumerate(zip(regs, bufs)):
+            assert np.array_equal(b.reshape(r.ny, r.nz, r.nx), want[i]), ("round", rnd, "request", i)
+    assert srv.stats()["batches"] >= 2 * (24 // 5)
+    srv.close()
+    ctx.close()
