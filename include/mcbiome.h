@@ -145,6 +145,15 @@ int mcb_points(McbEngine *e, const int64_t *seeds, int ns, const int32_t *pts, i
 /* Same with device buffers, asynchronous on `stream` (a hipStream_t, NULL = default stream). */
 int mcb_points_dev(McbEngine *e, const int64_t *d_seeds, int ns, const int32_t *d_pts, int np, uint8_t *d_out_biome,
                    int32_t *d_out_tg, void *stream);
+/* mcb_points_dev that allocates nothing: no counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-seed noise state and slot map live in one static device array per device (MCB_NOALLOC_MAX_SEEDS seed
+ * slots, 15.8 MB, held until the process ends); ns > MCB_NOALLOC_MAX_SEEDS fails with MCB_E_LIMIT.  Same
+ * results as mcb_points_dev.  Calls that overlap on one device share that array and race: order them
+ * (one stream, or events).  The engine's device must be current. */
+#define MCB_NOALLOC_MAX_SEEDS 1024
+int mcb_points_dev_noalloc(McbEngine *e, const int64_t *d_seeds, int ns, const int32_t *d_pts, int np,
+                           uint8_t *d_out_biome, int32_t *d_out_tg, void *stream);
 
 /* A region request.  zoom = 0: quart grid, out[out_off + j*nx*nz + iz*nx + ix] = biome at quart
  * (x0+ix, y[j], z0+iz).  zoom = 1: block grid at block resolution through BiomeManager with the

@@ -109,6 +109,7 @@ def lib():
                                     C.POINTER(C.c_double)]
         L.mcb_points.argtypes = [P, i64p, C.c_int, i32p, C.c_int, u8p, i32p]
         L.mcb_points_dev.argtypes = [P, P, C.c_int, P, C.c_int, P, P, P]
+        L.mcb_points_dev_noalloc.argtypes = [P, P, C.c_int, P, C.c_int, P, P, P]
         L.mcb_regions_bytes.restype = C.c_int64
         L.mcb_regions_bytes.argtypes = [C.POINTER(Region), C.c_int]
         L.mcb_region.argtypes = [P, C.POINTER(Region), u8p]
