@@ -96,6 +96,19 @@ int mcb_targets(McbEngine *e, const int32_t *tg6, int n, uint8_t *out_biome);
  * mask of every least-distance leaf of target i.  Host buffers, synchronous; any int32 targets. */
 int mcb_tie_masks(McbEngine *e, const int32_t *tg6, int n, uint64_t *out_masks);
 
+/* Raw climate (engine-header entry mc_climate_overworld_raw, engine/mc_climate.h:113-164), batched: for each
+ * seed s and block position p = (x, y, z) in xyz, out6[(s*np + p)*6 .. +6) = the sampler's floats
+ * {temperature, humidity, continentalness, erosion, depth, weirdness} (26.1/26.2: the double values cast
+ * to float, as the reference returns them).  Nether: the two legacy NormalNoises at (x*0.25, y*0.0, z*0.25)
+ * as floats (engine/cuda/mcg_host.c:63-74) in slots 0, 1 and zeros in 2..5.  Host buffers, synchronous. */
+int mcb_climate_raw(McbEngine *e, const int64_t *seeds, int ns, const int32_t *xyz, int np, float *out6);
+/* NormalNoise values (engine-header entries normal_get_d / normal_get_f, engine/mc_noise.h:286-324) of one of
+ * the world's climate noises, seeded as the world seeds it, at np double positions (x, y, z) for each seed:
+ * out[s*np + p] (26.3 float values widened to double).  noise: Overworld 0 offset, 1 temperature,
+ * 2 vegetation, 3 continentalness, 4 erosion, 5 ridge (Xoroshiro, engine/mc_climate.h:97-106); Nether 0
+ * temperature, 1 vegetation (legacy Random(seed), Random(seed + 1)).  Host buffers, synchronous. */
+int mcb_noise_values(McbEngine *e, int noise, const int64_t *seeds, int ns, const double *xyz, int np, double *out);
+
 /* End island field (engine/mc_end.h:66-106) over an nx x nz block grid of every seed, blocks
  * (bx0 + ix*step, bz0 + iz*step): kind 0 = getHeightValue at section (bx/8, bz/8) (C division),
  * kind 1 = the End's erosion value at the block (what TheEndBiomeSource reads; the terrain's End

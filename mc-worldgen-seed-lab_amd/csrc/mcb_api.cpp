@@ -240,6 +240,8 @@ void mcb_close(McbEngine *e) {
     (void)hipFree(e->d_tg);
     (void)hipFree(e->d_field);
     (void)hipFree(e->d_masks);
+    (void)hipFree(e->d_raw);
+    (void)hipFree(e->d_xyzd);
     (void)hipFree(e->d_feas);
     (void)hipFree(e->d_coord_lut);
     (void)hipFree(e->d_obs);
@@ -294,6 +296,61 @@ int mcb_tie_masks(McbEngine *e, const int32_t *tg6, int n, uint64_t *out_masks) 
     MCB_HIPCHK(launch_tie_masks(e->w, e->d_tg, n, e->d_masks, nullptr));
     MCB_HIPCHK(hipMemcpy(out_masks, e->d_masks, (size_t)n * 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return MCB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Seeds in chunks of at most kSeedChunk through the engine's slots: init, then `body(first seed, count)`.
+template <class F>
+int per_seed_chunks(McbEngine *e, const int64_t *seeds, int ns, F body) {
+    const int chunk = std::min(ns, kSeedChunk);
+    if (int rc = dev_grow(&e->d_slots, &e->slots_cap, (size_t)chunk)) return rc;
+    if (int rc = ensure_ident(e, chunk)) return rc;
+    if (int rc = dev_grow(&e->d_seeds, &e->seeds_cap, (size_t)chunk)) return rc;
+    for (int s0 = 0; s0 < ns; s0 += chunk) {
+        const int n = std::min(chunk, ns - s0);
+        MCB_HIPCHK(hipMemcpy(e->d_seeds, seeds + s0, (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice));
+        MCB_HIPCHK(launch_seed_init(e->w, e->d_seeds, e->d_ident, n, e->d_slots, nullptr));
+        if (int rc = body(s0, n)) return rc;
+    }
+    return MCB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mcb_climate_raw(McbEngine *e, const int64_t *seeds, int ns, const int32_t *xyz, int np, float *out6) {
+    if (!e || ns < 0 || np < 0 || (ns && np && (!seeds || !xyz || !out6))) return fail(MCB_E_ARG, "bad arguments");
+    if (e->w.dim == MCB_END) return fail(MCB_E_ARG, "the End has no climate noises");
+    if (ns == 0 || np == 0) return MCB_OK;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const int chunk = std::min(ns, kSeedChunk);
+    if (int rc = dev_grow(&e->d_pts, &e->pts_cap, (size_t)np * 3)) return rc;
+    if (int rc = dev_grow(&e->d_raw, &e->raw_cap, (size_t)chunk * np * 6)) return rc;
+    MCB_HIPCHK(hipMemcpy(e->d_pts, xyz, (size_t)np * 3 * sizeof(int32_t), hipMemcpyHostToDevice));
+    return per_seed_chunks(e, seeds, ns, [&](int s0, int n) -> int {
+        MCB_HIPCHK(launch_climate_raw(e->w, e->d_slots, n, e->d_pts, np, e->d_raw, nullptr));
+        MCB_HIPCHK(hipMemcpy(out6 + (size_t)s0 * np * 6, e->d_raw, (size_t)n * np * 6 * sizeof(float), hipMemcpyDeviceToHost));
+        return MCB_OK;
+    });
+}
+
+int mcb_noise_values(McbEngine *e, int noise, const int64_t *seeds, int ns, const double *xyz, int np, double *out) {
+    if (!e || ns < 0 || np < 0 || (ns && np && (!seeds || !xyz || !out))) return fail(MCB_E_ARG, "bad arguments");
+    if (e->w.dim == MCB_END) return fail(MCB_E_ARG, "the End has no climate noises");
+    if (noise < 0 || noise >= e->w.host.n_noise) return fail(MCB_E_ARG, "noise %d: this world has %d", noise, e->w.host.n_noise);
+    if (ns == 0 || np == 0) return MCB_OK;
+    std::lock_guard<std::mutex> lk(e->mu);
+    const int chunk = std::min(ns, kSeedChunk);
+    if (int rc = dev_grow(&e->d_xyzd, &e->xyzd_cap, (size_t)np * 3)) return rc;
+    if (int rc = dev_grow(&e->d_field, &e->field_cap, (size_t)chunk * np)) return rc;
+    MCB_HIPCHK(hipMemcpy(e->d_xyzd, xyz, (size_t)np * 3 * sizeof(double), hipMemcpyHostToDevice));
+    return per_seed_chunks(e, seeds, ns, [&](int s0, int n) -> int {
+        MCB_HIPCHK(launch_noise_values(e->w, e->d_slots, n, noise, e->d_xyzd, np, e->d_field, nullptr));
+        MCB_HIPCHK(hipMemcpy(out + (size_t)s0 * np, e->d_field, (size_t)n * np * sizeof(double), hipMemcpyDeviceToHost));
+        return MCB_OK;
+    });
 }
 
 int mcb_end_field(McbEngine *e, const int64_t *seeds, int ns, int bx0, int bz0, int nx, int nz, int step, int kind,

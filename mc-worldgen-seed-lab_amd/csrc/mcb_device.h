@@ -638,19 +638,23 @@ __device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, PS per
     column_finish<MODE>(W, SP, T, H, C, E, Wr, c);
 }
 
+// depth = clamped y gradient at block y + offset, as the float the sampler returns
 template <int MODE>
-__device__ __forceinline__ int ow_depth(const McbWorld *__restrict__ W, const Column &c, int qy) {
-    const int by = qy * 4;
+__device__ __forceinline__ float ow_depth_raw(const McbWorld *__restrict__ W, const Column &c, int by) {
     if (MODE) {
         int lo = min(W->gy0, W->gy1), hi = max(W->gy0, W->gy1);
         int yy = by < lo ? lo : (by > hi ? hi : by);
         float g = W->gv0_f + (float)(yy - W->gy0) * W->grad_factor_f;
-        return quantize(g + c.off_f);
+        return g + c.off_f;
     } else {
         double fac = ((double)by - (double)W->gy0) / ((double)W->gy1 - (double)W->gy0);
         double g = fac < 0.0 ? W->gv0_d : (fac > 1.0 ? W->gv1_d : W->gv0_d + fac * (W->gv1_d - W->gv0_d));
-        return quantize((float)(g + c.off_d));
+        return (float)(g + c.off_d);
     }
+}
+template <int MODE>
+__device__ __forceinline__ int ow_depth(const McbWorld *__restrict__ W, const Column &c, int qy) {
+    return quantize(ow_depth_raw<MODE>(W, c, qy * 4));
 }
 
 // Nether climate: two legacy NormalNoises at (bx*0.25, by*0.0, bz*0.25) (engine/cuda/mcg_host.c:63-74)

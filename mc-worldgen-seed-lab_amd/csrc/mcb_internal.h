@@ -92,6 +92,12 @@ hipError_t launch_points(const DevWorld &w, const McbSlot *d_slots, const int32_
 // biomes of n quantized climate targets (6 int32 each: t,h,c,e,d,w); Overworld and Nether only
 hipError_t launch_targets(const DevWorld &w, const int32_t *d_tg, int n, uint8_t *d_out, hipStream_t stream);
 hipError_t launch_tie_masks(const DevWorld &w, const int32_t *d_tg, int n, uint64_t *d_out, hipStream_t stream);
+// raw climate floats (6 per sample) at block positions, slots[s] = seed s (mcb_climate.hip)
+hipError_t launch_climate_raw(const DevWorld &w, const McbSlot *d_slots, int ns, const int32_t *d_xyz, int np, float *d_out,
+                              hipStream_t stream);
+// one NormalNoise of the world (index into McbWorld.noise) at double positions, slots[s] = seed s
+hipError_t launch_noise_values(const DevWorld &w, const McbSlot *d_slots, int ns, int noise, const double *d_xyz, int np,
+                               double *d_out, hipStream_t stream);
 // End island field of ns seeds (slots 0..ns-1) over an nx x nz block grid; kind 0 height, 1 erosion
 hipError_t launch_end_field(const DevWorld &w, const McbSlot *d_slots, int ns, int bx0, int bz0, int nx, int nz, int step,
                             int kind, double *d_out, hipStream_t stream);
@@ -135,6 +141,10 @@ struct McbEngine {
     size_t field_cap = 0;
     uint64_t *d_masks = nullptr;
     size_t masks_cap = 0;
+    float *d_raw = nullptr;                // raw climate floats
+    size_t raw_cap = 0;
+    double *d_xyzd = nullptr;              // double positions of noise queries
+    size_t xyzd_cap = 0;
     // host copies kept for the seed search's prefix tables
     std::shared_ptr<const mcb::CellTable> cells;
     std::vector<McbNode> tree_host;

@@ -138,6 +138,8 @@ def lib():
         L.mcg_ref_tie_mask.argtypes = [C.POINTER(McgHost), i32p, C.POINTER(C.c_uint64)]
         L.mcg_ref_set_error_handler.argtypes = [C.c_void_p, C.c_void_p]
         L.mcb_tie_masks.argtypes = [P, i32p, C.c_int, C.POINTER(C.c_uint64)]
+        L.mcb_climate_raw.argtypes = [P, i64p, C.c_int, i32p, C.c_int, C.POINTER(C.c_float)]
+        L.mcb_noise_values.argtypes = [P, C.c_int, i64p, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
         L.mcg_biome_name.restype = C.c_char_p
         L.mcg_biome_id.argtypes = [C.c_char_p]
         L.mcg_version_name.restype = C.c_char_p
@@ -283,6 +285,29 @@ class Engine:
         out = np.zeros(t.shape[0], np.uint8)
         _check(lib().mcb_targets(self.h, _p(t, C.c_int32), t.shape[0], _p(out, C.c_uint8)))
         return out
+
+    def climate_raw(self, seeds, xyz):
+        """(ns, np, 6) float32 raw climate at block positions xyz (np, 3)."""
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64).ravel()
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros(seeds.size * len(xyz) * 6, np.float32)
+        _check(lib().mcb_climate_raw(self.h, _p(seeds, C.c_int64), seeds.size, _p(xyz, C.c_int32), len(xyz),
+                                     _p(out, C.c_float)))
+        return out.reshape(seeds.size, len(xyz), 6)
+
+    NOISES = {"overworld": ["offset", "temperature", "vegetation", "continentalness", "erosion", "ridge"],
+              "the_nether": ["nether/temperature", "nether/vegetation"]}
+
+    def noise_values(self, noise, seeds, xyz):
+        """(ns, np) float64 NormalNoise values; noise = index or name (NOISES)."""
+        if isinstance(noise, str):
+            noise = self.NOISES["overworld" if self.dim == "overworld" else "the_nether"].index(noise)
+        seeds = np.ascontiguousarray(seeds, dtype=np.int64).ravel()
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(seeds.size * len(xyz), np.float64)
+        _check(lib().mcb_noise_values(self.h, int(noise), _p(seeds, C.c_int64), seeds.size, _p(xyz, C.c_double), len(xyz),
+                                      _p(out, C.c_double)))
+        return out.reshape(seeds.size, len(xyz))
 
     def tie_masks(self, tg):
         """(n, 2) uint64 biome masks of every least-distance leaf of each quantized target."""

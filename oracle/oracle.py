@@ -19,6 +19,7 @@ DATA_ROOT = os.path.join(REPO, "mc-worldgen-seed-lab_amd", "data")
 PKG_ROOT = os.path.join(REPO, "mc-worldgen-seed-lab_amd")
 LIB_PATH = os.path.join(HERE, "_build", "libmcoracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libmcg_ref.so")
+SHIM_PATH = os.path.join(HERE, "_ref", "libmcg_refshim.so")
 
 VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2}
 DIMS = {"overworld": 0, "the_nether": 1, "nether": 1, "the_end": 2, "end": 2}
@@ -283,6 +284,53 @@ class RefEngine:
         if self.buf is not None:
             RefEngine._L.mcg_host_free(self.buf)
             self.buf = None
+
+
+class RefShim:
+    """The reference's header-only engine (engine/*.h) behind oracle/ref_shim.c, built by `make -C oracle
+    ref` into oracle/_ref/libmcg_refshim.so: raw climate floats (mc_climate_overworld_raw) and NormalNoise
+    values (normal_get), which the reference's compiled driver does not export."""
+
+    _L = None
+    NOISES = {"offset": 0, "temperature": 1, "vegetation": 2, "continentalness": 3, "erosion": 4, "ridge": 5,
+              "nether/temperature": 6, "nether/vegetation": 7}
+
+    @classmethod
+    def available(cls):
+        return os.path.exists(SHIM_PATH)
+
+    def __init__(self, version="26.3", preset="overworld"):
+        if RefShim._L is None:
+            os.environ.setdefault("MCGEN_ROOT", PKG_ROOT)
+            L = C.CDLL(SHIM_PATH)
+            L.rs_open.restype = C.c_void_p
+            L.rs_open.argtypes = [C.c_int, C.c_char_p]
+            L.rs_close.argtypes = [C.c_void_p]
+            L.rs_climate_raw.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
+            L.rs_noise.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+            RefShim._L = L
+        self.h = RefShim._L.rs_open(VERSIONS[version], ("overworld" if preset == "normal" else preset).encode())
+        if not self.h:
+            raise RuntimeError("reference shim: cannot open %s %s" % (version, preset))
+
+    def climate_raw(self, seed, xyz):
+        """(n, 6) float32 raw climate at block positions xyz (n, 3)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.int32).reshape(-1, 3)
+        out = np.zeros(6 * len(xyz), np.float32)
+        RefShim._L.rs_climate_raw(self.h, int(seed), len(xyz), _p(xyz, C.c_int32), _p(out, C.c_float))
+        return out.reshape(-1, 6)
+
+    def noise(self, name, seed, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(xyz), np.float64)
+        if RefShim._L.rs_noise(self.h, self.NOISES[name], int(seed), len(xyz), _p(xyz, C.c_double), _p(out, C.c_double)):
+            raise KeyError(name)
+        return out
+
+    def close(self):
+        if self.h:
+            RefShim._L.rs_close(self.h)
+            self.h = None
 
 
 class McgWorldRef:
