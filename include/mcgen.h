@@ -5,8 +5,10 @@
  * What is the same: every function below has libmcgen's signature; biomes follow libmcgen's CPU semantics
  * (mcgen_biome_at: BiomeManager zoom, quart y clamped into the world, fitness ties resolved along the
  * chunk's fill order with no cap, libmcgen/src/biome.c:87-120; region biomes: world_chunk_biomes,
- * biome.c:125-157), computed on the GPU for the vanilla worlds of 26.1, 26.2 and 26.3.  Biome ids index
- * this library's table (mcgen_biome_name gives "minecraft:<name>").
+ * biome.c:125-157), computed on the GPU for the vanilla worlds of 26.1, 26.2 and 26.3.  Biome ids are
+ * libmcgen's: the version's biome registry sorted by name (libmcgen/src/pack.c:134-146), i.e. the biomes
+ * the version's Overworld and Nether place, the five End biomes and the_void (26.1: 65, 26.2: 66, 26.3: 67
+ * ids); mcgen_biome_count / mcgen_biome_name describe the registry of the McGen's version.
  *
  * What differs (this library implements libmcgen's biome stage only):
  *  - pack_dir is not read: the vanilla climate tables built into the library are used (version "26.1",

@@ -110,6 +110,29 @@ typedef struct McgBiomeWorld {
     int ntree; const McgTreeNode *tree; int tree_root;
 } McgBiomeWorld;
 
+/* World description for terrain (26.3+; mcgen_gpu_abi.h:105-123): the volume density program (root
+ * root_density = final_density, vein roots by index into vol->root), the veins' point "gap" program, the
+ * noise volume's y range [nmin, nmin + nh), the ore veins' positional random ("minecraft:ore"), the cache
+ * numbers whose cell values the CPU needs after the density (aquifer), and the function whose volume the
+ * CPU evaluates before the density (pre_root, -1 = none) with that volume's shape. */
+#define MCG_MAX_VEINS 4
+typedef struct McgVein { int density_root, richness_root, gap_root; int ore, raw, filler; float raw_chance; } McgVein;
+typedef struct McgTerrainWorld {
+    const McgProg *vol;
+    const McgProg *gap;
+    int nmin, nh;
+    int root_density;
+    int nveins; McgVein vein[MCG_MAX_VEINS];
+    uint64_t ore_lo, ore_hi;
+    int veins_on;
+    int cache_ids[32]; int ncache_ids;
+    int pre_root;
+    int pre_sx, pre_sy, pre_sz, pre_dx, pre_dy, pre_dz, pre_y0, pre_xoff, pre_zoff;
+} McgTerrainWorld;
+/* Beardifier callback: fills out with the volume vol = {sx, sy, sz, x0, y0, z0, dx, dy, dz} (block
+ * coordinates) for chunk chunk_index of the batch; returns 0 when the chunk has no Beardifier. */
+typedef int (*McgBeardFn)(void *ud, int chunk_index, const int vol[9], float *out);
+
 /* ---- exports bound by libmcgen/src/gpu_bridge.c:123-140 (all required) ---- */
 int mcgpu_abi(void);                                                       /* MCGPU_ABI */
 int mcgpu_device_count(void);

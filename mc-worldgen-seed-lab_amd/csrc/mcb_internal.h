@@ -163,9 +163,12 @@ struct McbEngine {
     size_t list_cap = 0;
 };
 
+struct McgBiomeWorld;
 namespace mcb {
 // identity slot map (slot i = seed i) of at least n entries in e->d_ident
 int ensure_ident(McbEngine *e, int n);
+// renumber the biome ids of a world made by mcb_plugin_world (leaves, End ids, fixed biome): id -> map[id]
+void plugin_world_renumber(McgBiomeWorld *W, const uint8_t map[256]);
 }  // namespace mcb
 
 #endif

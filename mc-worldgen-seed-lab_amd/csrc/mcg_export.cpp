@@ -298,6 +298,20 @@ int mcb_plugin_world(McbEngine *e, int64_t seed, McgBiomeWorld **out) {
 
 void mcb_plugin_world_free(McgBiomeWorld *W) { delete reinterpret_cast<Owned *>(W); }
 
+}  // extern "C"
+
+// Renumber the biome ids of an exported world (tree leaves, End ids, fixed biome) through map[old id].
+void mcb::plugin_world_renumber(McgBiomeWorld *W, const uint8_t map[256]) {
+    Owned *o = reinterpret_cast<Owned *>(W);
+    for (auto &t : o->tree)
+        if (t.count == 0 && t.biome >= 0 && t.biome < 256) t.biome = map[t.biome];
+    for (int k = 0; k < 5; k++)
+        if (W->end_ids[k] >= 0 && W->end_ids[k] < 256) W->end_ids[k] = map[W->end_ids[k]];
+    if (W->fixed_biome >= 0 && W->fixed_biome < 256) W->fixed_biome = map[W->fixed_biome];
+}
+
+extern "C" {
+
 // Whole chunks as libmcgen stores them (world_chunk_biomes, libmcgen/src/biome.c:125-157), through the
 // plugin kernels on the engine's vanilla world: per cell climate + search, ties resolved along the chunk's
 // fill order from its first cell.  The 26.3 chunk path samples the unshifted noises by volume, which sums

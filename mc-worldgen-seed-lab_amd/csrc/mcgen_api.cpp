@@ -9,7 +9,11 @@
 // outside this library: those entry points report unsupported or empty.
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -18,9 +22,58 @@
 #include "mcb_internal.h"
 #include "mcg_plugin.h"
 
+namespace {
+// A version's biome registry as libmcgen numbers it: the biome files of the version's pack, sorted by
+// name (libmcgen/src/pack.c:134-146 with list_dir_recursive's qsort, libmcgen/src/util.c:169-173).  The
+// vanilla files are the biomes the version's Overworld and Nether parameter lists place, the five End
+// biomes and the_void (26.1: 65, 26.2: 66, 26.3: 67).  from_engine maps the engine's ids to these.
+struct Registry {
+    std::vector<std::string> names;       // "minecraft:<name>", id order
+    uint8_t from_engine[256];
+};
+const char *const kUnplaced[6] = {"the_end", "end_highlands", "end_midlands", "small_end_islands", "end_barrens", "the_void"};
+
+int load_registry(int version, Registry &R, std::string &err) {
+    std::set<std::string> names(kUnplaced, kUnplaced + 6);
+    for (const char *dim : {"overworld", "nether"}) {
+        const std::string path = mcb::default_data_dir() + "/params/" + mcg_version_name(version) + "/" + dim + ".tsv";
+        std::ifstream f(path);
+        if (!f) { err = "cannot read " + path; return MCGEN_E_IO; }
+        std::string line;
+        while (std::getline(f, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::string b = line.substr(line.rfind('\t') + 1);
+            while (!b.empty() && (b.back() == '\r' || b.back() == ' ')) b.pop_back();
+            if (b.compare(0, 10, "minecraft:") == 0) b = b.substr(10);
+            names.insert(b);
+        }
+    }
+    for (auto &n : names) R.names.push_back("minecraft:" + n);          // std::set orders as strcmp
+    memset(R.from_engine, 255, sizeof R.from_engine);
+    for (int i = 0; i < mcg_biome_count() && i < 256; i++) {
+        const std::string full = std::string("minecraft:") + mcg_biome_name(i);
+        for (size_t j = 0; j < R.names.size(); j++)
+            if (R.names[j] == full) R.from_engine[i] = (uint8_t)j;
+    }
+    return MCGEN_OK;
+}
+
+const Registry *registry(int version, std::string &err) {
+    static std::mutex mu;
+    static std::map<int, Registry> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(version);
+    if (it != cache.end()) return &it->second;
+    Registry R;
+    if (load_registry(version, R, err)) return nullptr;
+    return &(cache[version] = std::move(R));
+}
+}  // namespace
+
 struct McGen {
     int version = 0;
     std::string vname;
+    const Registry *reg = nullptr;
 };
 
 struct McWorld {
@@ -55,14 +108,6 @@ int dim_of(const char *d) {
     if (!strcmp(s, "the_end") || !strcmp(s, "end")) return MCB_END;
     return -1;
 }
-const std::vector<std::string> &biome_names() {
-    static const std::vector<std::string> names = [] {
-        std::vector<std::string> v;
-        for (int i = 0; i < mcg_biome_count(); i++) v.push_back(std::string("minecraft:") + mcg_biome_name(i));
-        return v;
-    }();
-    return names;
-}
 int grid_code(const McWorld *w, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out) {
     if (!w || !w->h || nx <= 0 || nz <= 0 || step <= 0 || !out) return MCGEN_E_ARG;
     int bad = 0;
@@ -83,9 +128,13 @@ int mcgen_open(const char *pack_dir, const char *version, McGen **out, char *err
     *out = nullptr;
     const int v = mcg_version_from_name(version);
     if (v < 0) { put_err(err, errlen, "unsupported version %s (26.1, 26.2, 26.3)", version); return MCGEN_E_VERSION; }
+    std::string rerr;
+    const Registry *reg = registry(v, rerr);
+    if (!reg) { put_err(err, errlen, "%s", rerr.c_str()); return MCGEN_E_IO; }
     McGen *g = new McGen();
     g->version = v;
     g->vname = version;
+    g->reg = reg;
     *out = g;
     return MCGEN_OK;
 }
@@ -105,9 +154,9 @@ const char *mcgen_preset_name(const McGen *g, const char *dimension, int i) {
 int mcgen_block_state_count(const McGen *g) { (void)g; return 0; }
 const char *mcgen_block_state_name(const McGen *g, int id) { (void)g; (void)id; return nullptr; }
 int mcgen_block_state_from_name(const McGen *g, const char *name) { (void)g; (void)name; return -1; }
-int mcgen_biome_count(const McGen *g) { return g ? (int)biome_names().size() : 0; }
+int mcgen_biome_count(const McGen *g) { return g ? (int)g->reg->names.size() : 0; }
 const char *mcgen_biome_name(const McGen *g, int id) {
-    return g && id >= 0 && id < (int)biome_names().size() ? biome_names()[id].c_str() : nullptr;
+    return g && id >= 0 && id < (int)g->reg->names.size() ? g->reg->names[id].c_str() : nullptr;
 }
 int mcgen_tweak_count(const McGen *g) { (void)g; return 0; }
 const McTweakInfo *mcgen_tweak_info(const McGen *g, int i) { (void)g; (void)i; return nullptr; }
@@ -136,6 +185,7 @@ int mcgen_world_new(McGen *g, const char *dimension, const char *preset, const M
         mcb_close(e);
         return MCGEN_E_INTERNAL;
     }
+    mcb::plugin_world_renumber(pw, g->reg->from_engine);     // libmcgen's ids of this version
     char perr[512] = {0};
     void *h = mcb::plug::world_new(pw, 1 << 30, perr, sizeof perr);
     mcb_plugin_world_free(pw);

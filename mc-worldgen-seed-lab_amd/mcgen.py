@@ -87,6 +87,11 @@ class Gen:
     def world(self, dimension, preset, seed, tweaks=()):
         return World(self, dimension, preset, seed, tweaks)
 
+    def biome_names(self):
+        """The version's biome registry in id order ("minecraft:<name>")."""
+        L = lib()
+        return [L.mcgen_biome_name(self.h, i).decode() for i in range(L.mcgen_biome_count(self.h))]
+
     def close(self):
         if self.h:
             lib().mcgen_close(self.h)

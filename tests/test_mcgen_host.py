@@ -30,8 +30,9 @@ def test_open_versions_and_enumerations():
                                                                      b"minecraft:the_end"]
         assert [L.mcgen_preset_name(g.h, b"minecraft:overworld", i) for i in range(3)] == [b"normal", b"large_biomes", b"amplified"]
         assert L.mcgen_preset_count(g.h, b"minecraft:the_end") == 1
-        assert L.mcgen_biome_count(g.h) == 67
-        assert L.mcgen_biome_name(g.h, 41) == b"minecraft:plains" and L.mcgen_biome_name(g.h, 67) is None
+        n = {"26.1": 65, "26.2": 66, "26.3": 67}[v]
+        assert L.mcgen_biome_count(g.h) == n
+        assert L.mcgen_biome_name(g.h, 0) == b"minecraft:badlands" and L.mcgen_biome_name(g.h, n) is None
         assert L.mcgen_block_state_count(g.h) == 0 and L.mcgen_tweak_count(g.h) == 0
         g.close()
     with pytest.raises(mcgen.McgenError) as e:
@@ -56,3 +57,38 @@ def test_refusals_without_device():
     assert L.mcgen_biome_grid_cpu(None, 0, 0, 2, 2, 1, 64, out) == mcgen.MCGEN_E_UNSUPPORTED
     assert L.mcgen_gpu_set_compute(mcgen.MCGEN_COMPUTE_CPU, -1) == mcgen.MCGEN_E_UNSUPPORTED
     assert L.mcgen_gpu_set_compute(7, -1) == mcgen.MCGEN_E_ARG
+
+
+REF_DATA = "/root/reference/data"
+
+
+@pytest.mark.parametrize("version,count", [("26.1", 65), ("26.2", 66), ("26.3", 67)])
+def test_biome_ids_follow_the_versions_registry(version, count):
+    """libmcgen numbers biomes alphabetically over the version's biome files (libmcgen/src/pack.c:134-146,
+    libmcgen/src/util.c:169-173): the biomes the version's Overworld and Nether place, the five End biomes
+    and the_void.  Checked against the reference's own per-version list of placed biomes
+    (data/featureorder-<V>.json, all three dimensions) where the reference is present; 26.3's registry is
+    the engine's 67-name table."""
+    import json
+
+    import golden_io as G  # noqa: F401  (path set-up)
+    import oracle as O
+    names = mcgen.Gen(version).biome_names()
+    assert len(names) == count and names == sorted(names) and len(set(names)) == count
+    if version == "26.3":
+        assert names == ["minecraft:" + n for n in O.biome_names()]
+    # the placed biomes per dimension: the parameter lists' names (Overworld, Nether) and the End's five
+    data = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "mc-worldgen-seed-lab_amd", "data")
+    placed = {}
+    for dim, f in (("overworld", "overworld"), ("the_nether", "nether")):
+        rows = [l.rstrip("\n").split("\t")[-1] for l in open(os.path.join(data, "params", version, f + ".tsv"))
+                if l.strip() and not l.startswith("#")]
+        placed[dim] = {r if r.startswith("minecraft:") else "minecraft:" + r for r in rows}
+    placed["the_end"] = {"minecraft:" + b for b in ("the_end", "end_highlands", "end_midlands", "small_end_islands",
+                                                   "end_barrens")}
+    assert sorted(set.union(*placed.values()) | {"minecraft:the_void"}) == names
+    path = os.path.join(REF_DATA, "featureorder-%s.json" % version)
+    if os.path.exists(path):
+        # the reference's own per-dimension biome counts (the biomes each dimension's source places)
+        fo = json.load(open(path))
+        assert {d: fo[d]["biomes"] for d in placed} == {d: len(v) for d, v in placed.items()}
