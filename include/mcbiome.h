@@ -6,7 +6,8 @@
  * HIP device every compute entry point fails with MCB_E_DEVICE.
  *
  * Biome ids are u8 indices into the 67-name alphabetical table (mcg_biome_name), the ids of the
- * reference engine (engine/gen/mc_biome_names.h:4-5).  Versions: 0 = 26.1, 1 = 26.2, 2 = 26.3.
+ * reference engine (engine/gen/mc_biome_names.h:4-5).  Versions: 0 = 26.1, 1 = 26.2, 2 = 26.3,
+ * 3 = 26.4-snapshot-2 (engine/mc_common.h:35-38).
  * Dimensions: 0 = overworld, 1 = nether, 2 = end.  Search semantics are the reference CPU engine's:
  * each sample's R-tree search starts from no candidate (engine/mc_biomes.h:58-62).
  *
@@ -79,6 +80,13 @@ int mcb_device_count(void);
 /* One (version, dimension, preset) world on the current HIP device.  max_slots = seed-state slots
  * kept resident on the device (15,472 bytes each; 0 = default 8192). */
 int mcb_open(McbEngine **out, int version, int dim, const char *preset, const char *data_dir, int max_slots);
+/* The biome R-tree of a version's Overworld or Nether as the engine searches it, built on the host from
+ * the parameter list (Climate.RTree.create, engine/mc_biomes.h:64-212 as loaded by engine/mc_data.h:153-183)
+ * in DFS pre-order, 16 int32 per node: lo/hi of temperature, humidity, continentalness, erosion, depth,
+ * weirdness (upper bounds lowered by one for 26.4-snapshot-2), the squared distance of the offset interval
+ * to 0, the index just past the node's subtree, the leaf's biome id (-1 for a subtree), 0.  Returns the
+ * node count (out may be NULL to ask for it), or a negative MCB_E_* code.  Needs no device. */
+int mcb_tree(int version, int dim, const char *data_dir, int32_t *out, int cap);
 void mcb_close(McbEngine *e);
 int mcb_engine_info(const McbEngine *e, int *version, int *dim, int *mode, int *tree_nodes);
 /* Overworld climate cell table that replaces the per-sample R-tree search of engine/mc_biomes.h:31-62

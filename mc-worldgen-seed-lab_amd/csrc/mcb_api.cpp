@@ -180,12 +180,37 @@ int mcg_version_from_name(const char *s) {
     if (!strncmp(s, "26.1", 4)) return MCB_V26_1;
     if (!strncmp(s, "26.2", 4)) return MCB_V26_2;
     if (!strncmp(s, "26.3", 4)) return MCB_V26_3;
+    if (!strncmp(s, "26.4", 4)) return MCB_V26_4;
     return -1;
 }
-const char *mcg_version_name(int v) { return v == 0 ? "26.1" : v == 1 ? "26.2" : v == 2 ? "26.3" : "?"; }
+const char *mcg_version_name(int v) {
+    return v == 0 ? "26.1" : v == 1 ? "26.2" : v == 2 ? "26.3" : v == 3 ? "26.4-snapshot-2" : "?";
+}
 int mcg_biome_count(void) { return biome_count(); }
 const char *mcg_biome_name(int id) { return biome_name(id); }
 int mcg_biome_id(const char *name) { return name ? biome_id(name) : -1; }
+
+int mcb_tree(int version, int dim, const char *data_dir, int32_t *out, int cap) {
+    if (dim != MCB_OVERWORLD && dim != MCB_NETHER) return -fail(MCB_E_ARG, "the End has no biome tree");
+    McbWorld w;
+    std::vector<McbNode> tree;
+    std::string err;
+    if (build_world(data_dir ? data_dir : default_data_dir(), version, dim, "normal", &w, tree, err) != 0)
+        return -fail(MCB_E_DATA, "%s", err.c_str());
+    if (out) {
+        if ((int64_t)tree.size() * 16 > cap) return -fail(MCB_E_LIMIT, "%zu nodes need %zu ints", tree.size(), tree.size() * 16);
+        for (size_t i = 0; i < tree.size(); i++) {
+            int32_t *o = out + 16 * i;
+            const McbNode &n = tree[i];
+            for (int d = 0; d < 6; d++) { o[2 * d] = n.lo[d]; o[2 * d + 1] = n.hi[d]; }
+            o[12] = (int32_t)n.d6;
+            o[13] = n.escape;
+            o[14] = n.leaf ? (n.leaf & 0xFF) : -1;
+            o[15] = 0;
+        }
+    }
+    return (int)tree.size();
+}
 
 int mcb_open(McbEngine **out, int version, int dim, const char *preset, const char *data_dir, int max_slots) {
     (void)max_slots;
@@ -203,7 +228,9 @@ int mcb_open(McbEngine **out, int version, int dim, const char *preset, const ch
     e->w.version = version;
     e->w.dim = dim;
     e->w.preset = preset ? preset : "normal";
-    if (dim == MCB_OVERWORLD) {
+    // the cell table encodes closed parameter intervals; 26.4-snapshot-2's exclusive upper bounds (boxes
+    // with lo = hi + 1) are searched by the tree walk alone
+    if (dim == MCB_OVERWORLD && version < MCB_V26_4) {
         if (int rc = attach_cells(e, tree)) {
             mcb_close(e);
             return rc;

@@ -22,7 +22,7 @@
 #define MCB_CELL_MAX_BP 24      /* breakpoints per climate dimension in the cell table */
 #define MCB_CELL_LUT (6 * 256)  /* cell locator entries */
 
-enum { MCB_V26_1 = 0, MCB_V26_2 = 1, MCB_V26_3 = 2 };
+enum { MCB_V26_1 = 0, MCB_V26_2 = 1, MCB_V26_3 = 2, MCB_V26_4 = 3 };   /* MCB_V26_4 = 26.4-snapshot-2 */
 enum { MCB_OVERWORLD = 0, MCB_NETHER = 1, MCB_END = 2 };
 
 /* One NormalNoise: per-level constants, engine/mc_noise.h:142-155 */

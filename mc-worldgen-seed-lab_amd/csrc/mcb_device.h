@@ -399,8 +399,11 @@ __device__ __forceinline__ float spline_eval(const SplLds &S, int id, float c, f
 // ================================================================================================
 // squared distance of one parameter interval to the target: (t - clamp(t, lo, hi))^2, integer-valued
 // doubles (exact below 2^53)
+// One dimension's term of the fitness: the game's above = t - hi, below = lo - t, squared
+// (engine/mc_biomes.h:31-39).  max(above, below, 0) equals it for every box with lo <= hi + 1, which
+// covers 26.4-snapshot-2's boxes (lo = hi + 1 after its exclusive upper bound) as well as closed ones.
 __device__ __forceinline__ double dim_sq(int t, int lo, int hi, double acc) {
-    const double x = (double)(t - min(max(t, lo), hi));
+    const double x = (double)max(max(t - hi, lo - t), 0);
     return fma(x, x, acc);
 }
 __device__ __forceinline__ double node_dist(const McbNode &nd, const int *t) {

@@ -43,6 +43,17 @@ struct CellTable {
 int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &err, bool strict = false);
 std::shared_ptr<const CellTable> cells_for_tree(const std::vector<McbNode> &tree, std::string &err, bool strict = false);
 
+// One entry of a biome parameter list (params/<V>/<dim>.tsv): quantized intervals of temperature,
+// humidity, continentalness, erosion, depth, weirdness and the offset (lo = hi), and the biome id.
+struct ParamEntry {
+    int64_t lo[7], hi[7];
+    int biome;
+};
+// Climate.RTree of a parameter list in device pre-order (mcb_rtree.cpp); exclusive_hi: 26.4-snapshot-2's
+// upper bounds, lowered by one after the build
+int build_tree(const std::vector<ParamEntry> &list, int per_node, bool exclusive_hi, std::vector<McbNode> &out,
+               std::string &err);
+
 // BiomeManager.obfuscateSeed (SHA-256 of the seed's little-endian bytes, first 8 digest bytes LE)
 int64_t hashed_seed(int64_t seed);
 

@@ -2,9 +2,8 @@
 //
 // Builds one McbWorld (noise constants, octave-name hashes, depth/ridge constants, offset spline,
 // LCG jump constants) and the pre-order R-tree array for a (version, dimension, preset) from the
-// data directory: climate-<V>.txt (format: tools/extract_climate.py:1-16 of the reference),
-// params/<V>/<dim>.tsv (biome parameter list) and rtree_<V>_<dim>.json (the game's own R-tree,
-// leaves indexing the parameter list).
+// data directory: climate-<V>.txt (format: tools/extract_climate.py:1-16 of the reference) and
+// params/<V>/<dim>.tsv (the biome parameter list, from which mcb_rtree.cpp builds the R-tree).
 //
 // The constant derivations restate engine/mc_noise.h:179-252 (NormalNoise constants, 26.1/26.2 and
 // 26.3 forms) and engine/mc_data.h:84-146 (climate file), so that the kernels compute with the same
@@ -273,88 +272,14 @@ void lcg_jump(uint64_t n, uint64_t *mul, uint64_t *add) {
     *add = a;
 }
 
-// --- minimal reader for the tree dump: {"b":[14 ints], "c":[...]} | {"b":[14 ints], "i":N} ---
-struct JsonCursor {
-    const char *p;
-    void ws() { while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t' || *p == ',') p++; }
-    bool lit(char c) { ws(); if (*p != c) return false; p++; return true; }
-    bool key(std::string &k) {
-        ws();
-        if (*p != '"') return false;
-        const char *e = strchr(p + 1, '"');
-        if (!e) return false;
-        k.assign(p + 1, e);
-        p = e + 1;
-        return lit(':');
-    }
-    long long num() { ws(); char *e; long long v = strtoll(p, &e, 10); p = e; return v; }
-};
-
-struct TreeBuild {
-    std::vector<McbNode> nodes;
-    const std::vector<int> *leaf_biome;
-    std::string err;
-};
-
-static bool put_box(McbNode &n, const long long *b, std::string &err) {
-    for (int d = 0; d < 6; d++) {
-        if (b[2 * d] < -32768 || b[2 * d + 1] > 32767) { err = "tree bound outside int16"; return false; }
-        n.lo[d] = (int16_t)b[2 * d];
-        n.hi[d] = (int16_t)b[2 * d + 1];
-    }
-    long long lo = b[12], hi = b[13];
-    long long x = lo > 0 ? lo : (hi < 0 ? -hi : 0);     // distance of [lo,hi] to the target 0
-    n.d6 = (uint32_t)(x * x);
-    return true;
-}
-
-static bool parse_node(JsonCursor &c, TreeBuild &T) {
-    if (!c.lit('{')) { T.err = "tree: '{' expected"; return false; }
-    long long box[14];
-    int have_box = 0, leaf_param = -1;
-    size_t my = T.nodes.size();
-    T.nodes.push_back(McbNode());
-    std::string k;
-    while (true) {
-        c.ws();
-        if (*c.p == '}') { c.p++; break; }
-        if (!c.key(k)) { T.err = "tree: key expected"; return false; }
-        if (k == "b") {
-            c.lit('[');
-            for (int i = 0; i < 14; i++) box[i] = c.num();
-            c.lit(']');
-            have_box = 1;
-        } else if (k == "i") {
-            leaf_param = (int)c.num();
-        } else if (k == "c") {
-            c.lit('[');
-            while (true) {
-                c.ws();
-                if (*c.p == ']') { c.p++; break; }
-                if (!parse_node(c, T)) return false;
-            }
-        } else { T.err = "tree: unknown key " + k; return false; }
-    }
-    if (!have_box) { T.err = "tree: node without box"; return false; }
-    McbNode &n = T.nodes[my];
-    if (!put_box(n, box, T.err)) return false;
-    if (leaf_param >= 0) {
-        if (leaf_param >= (int)T.leaf_biome->size()) { T.err = "tree: leaf index out of range"; return false; }
-        n.leaf = (uint16_t)(0x8000 | (*T.leaf_biome)[leaf_param]);
-    } else n.leaf = 0;
-    if (T.nodes.size() > 65535) { T.err = "tree too large"; return false; }
-    n.escape = (uint16_t)T.nodes.size();
-    return true;
-}
-
 }  // namespace
 
 int build_world(const std::string &root, int version, int dim, const std::string &preset_in, McbWorld *w,
                 std::vector<McbNode> &tree, std::string &err) {
     memset(w, 0, sizeof *w);
     tree.clear();
-    static const char *vn[3] = {"26.1", "26.2", "26.3"};
-    if (version < 0 || version > 2) { err = "unsupported version"; return -1; }
+    static const char *vn[4] = {"26.1", "26.2", "26.3", "26.4-snapshot-2"};
+    if (version < 0 || version > MCB_V26_4) { err = "unsupported version"; return -1; }
     if (dim < 0 || dim > 2) { err = "unsupported dimension"; return -1; }
     w->version = version;
     w->dim = dim;
@@ -460,28 +385,24 @@ int build_world(const std::string &root, int version, int dim, const std::string
         const char *dn = dim == MCB_NETHER ? "nether" : "overworld";
         std::ifstream pf(root + "/params/" + vn[version] + "/" + dn + ".tsv");
         if (!pf) { err = std::string("cannot read params/") + vn[version] + "/" + dn + ".tsv"; return -1; }
-        std::vector<int> leaf_biome;
+        std::vector<ParamEntry> list;
         while (std::getline(pf, line)) {
             if (line.empty() || line[0] == '#') continue;
             auto t = split_ws(line);
             if (t.size() < 14) continue;
-            int b = biome_id(t[13].c_str());
-            if (b < 0) { err = "unknown biome " + t[13]; return -1; }
-            leaf_biome.push_back(b);
+            ParamEntry e;
+            for (int d = 0; d < 6; d++) {
+                e.lo[d] = strtoll(t[2 * d].c_str(), nullptr, 10);
+                e.hi[d] = strtoll(t[2 * d + 1].c_str(), nullptr, 10);
+            }
+            e.lo[6] = e.hi[6] = strtoll(t[12].c_str(), nullptr, 10);
+            e.biome = biome_id(t[13].c_str());
+            if (e.biome < 0) { err = "unknown biome " + t[13]; return -1; }
+            list.push_back(e);
         }
-        std::ifstream jf(root + "/rtree_" + vn[version] + "_" + (dim == MCB_NETHER ? "the_nether" : "overworld") + ".json");
-        if (!jf) { err = "cannot read tree dump"; return -1; }
-        std::stringstream ss;
-        ss << jf.rdbuf();
-        std::string js = ss.str();
-        size_t at = js.find("\"tree\"");
-        if (at == std::string::npos) { err = "tree dump without \"tree\""; return -1; }
-        JsonCursor c{js.c_str() + at + 6};
-        c.lit(':');
-        TreeBuild T;
-        T.leaf_biome = &leaf_biome;
-        if (!parse_node(c, T)) { err = T.err; return -1; }
-        tree.swap(T.nodes);
+        // Climate.RTree.CHILDREN_PER_NODE: 6 before 26.3, 19 from 26.3 (engine/mc_data.h:177);
+        // 26.4-snapshot-2 measures with exclusive upper bounds (engine/mc_data.h:178-180)
+        if (build_tree(list, version >= MCB_V26_3 ? 19 : 6, version >= MCB_V26_4, tree, err) != 0) return -1;
         w->tree_n = (int)tree.size();
     }
     return 0;

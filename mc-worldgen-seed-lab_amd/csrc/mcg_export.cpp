@@ -159,6 +159,8 @@ extern "C" {
 int mcb_plugin_world(McbEngine *e, int64_t seed, McgBiomeWorld **out) {
     if (!e || !out) return fail(MCB_E_ARG, "bad arguments");
     *out = nullptr;
+    // the plugin ABI's tree nodes carry closed intervals (libmcgen/gpu/mcgpu_dev.cuh:463-473)
+    if (e->w.version >= MCB_V26_4) return fail(MCB_E_ARG, "26.4-snapshot-2 trees (exclusive upper bounds) have no plugin form");
     const McbWorld &w = e->w.host;
     McbSlot slot;
     {

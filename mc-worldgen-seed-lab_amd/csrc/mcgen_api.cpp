@@ -127,7 +127,7 @@ int mcgen_open(const char *pack_dir, const char *version, McGen **out, char *err
     if (!out || !version) { put_err(err, errlen, "bad arguments"); return MCGEN_E_ARG; }
     *out = nullptr;
     const int v = mcg_version_from_name(version);
-    if (v < 0) { put_err(err, errlen, "unsupported version %s (26.1, 26.2, 26.3)", version); return MCGEN_E_VERSION; }
+    if (v < 0 || v > MCB_V26_3) { put_err(err, errlen, "unsupported version %s (26.1, 26.2, 26.3)", version); return MCGEN_E_VERSION; }
     std::string rerr;
     const Registry *reg = registry(v, rerr);
     if (!reg) { put_err(err, errlen, "%s", rerr.c_str()); return MCGEN_E_IO; }

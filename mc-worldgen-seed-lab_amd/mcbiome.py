@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libmcbiome_hip.so")
 DATA_DIR = os.path.join(HERE, "data")
 
-VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2}
+VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2, "26.4-snapshot-2": 3}
 DIMS = {"overworld": 0, "the_nether": 1, "nether": 1, "the_end": 2, "end": 2}
 
 
@@ -138,6 +138,7 @@ def lib():
         L.mcg_ref_tie_mask.argtypes = [C.POINTER(McgHost), i32p, C.POINTER(C.c_uint64)]
         L.mcg_ref_set_error_handler.argtypes = [C.c_void_p, C.c_void_p]
         L.mcb_tie_masks.argtypes = [P, i32p, C.c_int, C.POINTER(C.c_uint64)]
+        L.mcb_tree.argtypes = [C.c_int, C.c_int, C.c_char_p, i32p, C.c_int]
         L.mcb_climate_raw.argtypes = [P, i64p, C.c_int, i32p, C.c_int, C.POINTER(C.c_float)]
         L.mcb_noise_values.argtypes = [P, C.c_int, i64p, C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
         L.mcg_biome_name.restype = C.c_char_p
@@ -163,6 +164,18 @@ def biome_names():
 
 def device_count():
     return lib().mcb_device_count()
+
+
+def tree_nodes(version, dim, data_dir=None):
+    """mcb_tree: the engine's R-tree (built on the host from the parameter list), (n, 16) int32 rows
+    lo0, hi0 .. lo5, hi5, offset distance^2, escape index, leaf biome (-1 = subtree), 0."""
+    d = data_dir.encode() if data_dir else None
+    n = lib().mcb_tree(VERSIONS[version], DIMS[dim], d, None, 0)
+    if n < 0:
+        raise EngineError(lib().mcb_last_error().decode())
+    out = np.zeros(16 * n, np.int32)
+    _check(0 if lib().mcb_tree(VERSIONS[version], DIMS[dim], d, _p(out, C.c_int32), out.size) == n else 1)
+    return out.reshape(n, 16)
 
 
 def make_region(seed, x0, z0, nx, nz, ys, zoom=False, out_off=0):

@@ -879,7 +879,7 @@ static char *slurp(const char *path) {
     b[n] = 0; fclose(f);
     return b;
 }
-static const char *vname(int v) { return v == OR_V26_1 ? "26.1" : v == OR_V26_2 ? "26.2" : "26.3"; }
+static const char *vname(int v) { return v == OR_V26_1 ? "26.1" : v == OR_V26_2 ? "26.2" : v == OR_V26_3 ? "26.3" : "26.4-snapshot-2"; }   /* mc_data.h:11 */
 static const NoiseParams *find_noise(const OrWorld *w, const char *name) {
     for (int i = 0; i < w->nnoise; i++) if (!strcmp(w->noise[i].name, name)) return &w->noise[i];
     return NULL;
@@ -968,11 +968,15 @@ static int load_tree(OrWorld *w, const char *root) {
     /* Climate.RTree.CHILDREN_PER_NODE: 6 before 26.3, 19 from 26.3 (engine/mc_data.h:177) */
     T->root = rt_build(T, ids, T->nleaves, w->version >= OR_V26_3 ? 19 : 6);
     free(ids);
+    /* 26.4-snapshot-2: Parameter.distance's upper bound is exclusive (above = t - max + 1); the tree is built
+     * from the listed values and every node's upper bounds lowered by one (engine/mc_data.h:178-180) */
+    if (w->version >= OR_V26_4)
+        for (int i = 0; i < T->n; i++) for (int d = 0; d < RT_DIM; d++) T->node[i].hi[d] -= 1;
     return 0;
 }
 
 OrWorld *or_open(const char *root, int version, int dim, const char *preset) {
-    if (version < OR_V26_1 || version > OR_V26_3 || dim < 0 || dim > 2) return NULL;
+    if (version < OR_V26_1 || version > OR_V26_4 || dim < 0 || dim > 2) return NULL;
     OrWorld *w = calloc(1, sizeof *w);
     w->version = version; w->dim = dim; w->fmt_new = version >= OR_V26_3;
     const char *p = (!preset || !strcmp(preset, "normal")) ? "overworld" : preset;

@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-enum { OR_V26_1 = 0, OR_V26_2 = 1, OR_V26_3 = 2 };
+enum { OR_V26_1 = 0, OR_V26_2 = 1, OR_V26_3 = 2, OR_V26_4 = 3 };   /* OR_V26_4 = 26.4-snapshot-2 */
 enum { OR_OVERWORLD = 0, OR_NETHER = 1, OR_END = 2 };
 
 typedef struct OrWorld OrWorld;

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libmcoracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libmcg_ref.so")
 SHIM_PATH = os.path.join(HERE, "_ref", "libmcg_refshim.so")
 
-VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2}
+VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2, "26.4-snapshot-2": 3}
 DIMS = {"overworld": 0, "the_nether": 1, "nether": 1, "the_end": 2, "end": 2}
 
 _lib = None

@@ -13,6 +13,8 @@ import pytest
 import golden_io as G
 import oracle as O
 
+ALL_VERSIONS = G.VERSIONS + ("26.4-snapshot-2",)
+
 pytestmark = pytest.mark.skipif(not O.RefShim.available(), reason="oracle/_ref not built (make -C oracle ref)")
 
 
@@ -22,7 +24,7 @@ def _blocks(rng, n, lo, hi):
     return np.stack([xz[:, 0], rng.integers(-64, 320, n), xz[:, 1]], 1).astype(np.int32)
 
 
-@pytest.mark.parametrize("version", G.VERSIONS)
+@pytest.mark.parametrize("version", ALL_VERSIONS)
 @pytest.mark.parametrize("preset", ["overworld", "amplified", "large_biomes"])
 def test_raw_climate_bits_identical(version, preset):
     rng = np.random.default_rng(zlib.crc32(("raw/%s/%s" % (version, preset)).encode()))
@@ -40,7 +42,7 @@ def test_raw_climate_bits_identical(version, preset):
 NOISES = ["offset", "temperature", "vegetation", "continentalness", "erosion", "ridge"]
 
 
-@pytest.mark.parametrize("version", G.VERSIONS)
+@pytest.mark.parametrize("version", ALL_VERSIONS)
 def test_noise_values_bits_identical(version):
     """NormalNoise at double positions of every magnitude up to 2^36 (octave inputs x*f past 2^24 for
     every frequency of these noises), Overworld and legacy Nether noises."""

@@ -9,6 +9,8 @@ import pytest
 import golden_io as G
 import oracle as O
 
+ALL_VERSIONS = G.VERSIONS + ("26.4-snapshot-2",)
+
 pytestmark = pytest.mark.skipif(not O.RefEngine.available(), reason="oracle/_ref not built (make -C oracle ref)")
 
 
@@ -19,7 +21,7 @@ def _inputs(rng, ns, np_, span):
     return seeds, pts.astype(np.int32)
 
 
-@pytest.mark.parametrize("version", G.VERSIONS)
+@pytest.mark.parametrize("version", ALL_VERSIONS)
 @pytest.mark.parametrize("dim", ["overworld", "the_nether", "the_end"])
 def test_points_identical(version, dim):
     rng = np.random.default_rng(zlib.crc32(("%s/%s" % (version, dim)).encode()))
@@ -38,7 +40,7 @@ def test_points_identical(version, dim):
 def test_presets_identical(preset):
     rng = np.random.default_rng(3)
     seeds, pts = _inputs(rng, 16, 64, 2**18)
-    for version in G.VERSIONS:
+    for version in ALL_VERSIONS:
         a, ta = O.Oracle(version, "overworld", preset).points(seeds, pts, want_tg=True)
         ref = O.RefEngine(version, "overworld", preset)
         b, tb = ref.points(seeds, pts, want_tg=True)
@@ -46,7 +48,7 @@ def test_presets_identical(preset):
         assert np.array_equal(a, b) and np.array_equal(ta, tb)
 
 
-@pytest.mark.parametrize("version", G.VERSIONS)
+@pytest.mark.parametrize("version", ALL_VERSIONS)
 @pytest.mark.parametrize("dim", ["overworld", "the_nether", "the_end"])
 def test_far_coordinates_identical(version, dim):
     """Quart coordinates of magnitude 2^24 .. 2^28 (blocks up to 2^30, inside int32): the octave inputs
@@ -69,7 +71,7 @@ def test_far_coordinates_identical(version, dim):
     assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("version", G.VERSIONS)
+@pytest.mark.parametrize("version", ALL_VERSIONS)
 @pytest.mark.parametrize("dim", ["overworld", "the_nether"])
 def test_tie_mask_identical(version, dim):
     """or_tie_mask against the reference's mcg_ref_tie_mask (engine/cuda/mcg_host.c:85-96): targets of
