@@ -14,11 +14,14 @@
  *   climate sampler, spline, depth, quantize         : vectors/<V>/climate_overworld.tsv, df.tsv
  *   R-tree build                                     : data/rtree_<V>_*.json (the game's own tree)
  *   R-tree search                                    : climate_*.tsv biome column, brute force
- *   search history semantics                         : vectors/<V>/biome_grid_hashes.tsv
+ *   search history semantics                         : vectors/<V>/biome_grid_hashes.tsv (all 54 point rows)
  *   Nether / End                                     : climate_the_nether.tsv, climate_the_end.tsv
  *   SHA-256 hashed seed, zoom                        : misc_biome.jsonl (hashed, blockbiome)
- *   wrap_new outer branch (|x| >= 2^24)              : closed form only; no vector reaches it
- *                                                      (parity unpinned for that branch)
+ *   whole path incl. 26.4-snapshot-2, tie masks      : the reference engine built from its sources
+ *                                                      (oracle/_ref/libmcg_ref.so)
+ *   PerlinNoise.wrap outer branch (|x| >= 2^24),     : raw climate floats and NormalNoise values of the
+ *   raw climate, NormalNoise values                    reference's header-only engine (oracle/ref_shim.c),
+ *                                                      bit for bit at positions past 2^24
  */
 #include "mcoracle.h"
 #include <math.h>

@@ -119,12 +119,16 @@ def _hash_cases(version, dims, max_seeds):
 
 
 @pytest.mark.parametrize("version", G.VERSIONS)
-def test_grid_hashes_cpu_subset(version):
-    """biome_grid_hashes.tsv: 1024^2 quart grids (End 512^2), FNV-1a-64 over names in row-major order,
-    with the Java resolver's search history (each search starts from the previous leaf).  One seed
-    per dimension here; the GPU tests cover all of them."""
-    for dim, seed, qx0, qz0, n, qy, h in _hash_cases(version, ("overworld", "the_nether", "the_end"), 1):
-        w = O.Oracle(version, dim)
+@pytest.mark.parametrize("dim", ["overworld", "the_nether", "the_end"])
+def test_grid_hashes_all_point_rows(version, dim):
+    """biome_grid_hashes.tsv, every point-mode row (6 seeds per version and dimension, 54 grids): 1024^2
+    quart grids (End 512^2), FNV-1a-64 over names in row-major order, with the Java resolver's search
+    history (each search starts from the previous leaf).  The HIP engine's grids at one seed per version
+    and dimension are tied to these by tests/test_gpu_grid_hashes.py."""
+    cases = _hash_cases(version, (dim,), 99)
+    assert len(cases) == 6
+    w = O.Oracle(version, dim)
+    for _, seed, qx0, qz0, n, qy, h in cases:
         ids = w.grid(seed, qx0, qz0, n, n, qy, semantics=1)
         assert O.fnv1a_names(ids) == h, (version, dim, seed)
 
