@@ -234,7 +234,8 @@ def bench_batched(args, torch, mcbiome):
         outs[v] = torch.empty(plans[v].out_bytes, dtype=torch.uint8, device="cuda")
         if not args.tile_w:      # project autotuner: tile shape, columns per lane, persistent workgroups
             tw, ms = plans[v].autotune(outs[v].data_ptr(), sp)
-            tuned[v] = {"tile_w": tw & 0xFFFF, "cols_per_lane": tw >> 16, "grid_ms": round(ms, 3)}
+            tuned[v] = {"tile_w": tw & 0xFFFF, "cols_per_lane": tw >> 16, "blocks": plans[v].shape()[2],
+                        "wrap_free": bool(plans[v].shape()[3]), "grid_ms": round(ms, 3)}
     samples_per_step = sum(plans[v].samples for v in VERSIONS)
 
     # A step runs each version's plan; without --graph the stages are launched one by one so that HIP

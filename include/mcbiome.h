@@ -210,6 +210,11 @@ int mcb_plan_run_stage(McbPlan *p, int stage, uint8_t *d_out, void *stream);
 /* Capture the run into a hipGraph once and replay it (same d_out each time). */
 int mcb_plan_run_graph(McbPlan *p, uint8_t *d_out, void *stream);
 int mcb_plan_info(const McbPlan *p, int64_t *tiles, int64_t *columns, int64_t *samples, int64_t *seeds);
+/* Launch shape of world group `group` of a plan (0 for single-engine plans): tile width, columns per lane,
+ * persistent workgroups (0 = one per tile), and whether its quart-grid launch runs the 26.3 kernel form
+ * without PerlinNoise.wrap's range test (1: every octave input of its requests provably stays inside
+ * +-2^24, where wrap is the identity; chosen at plan creation). */
+int mcb_plan_shape(const McbPlan *p, int group, int *tile_w, int *cpl, int *blocks, int *wrap_free);
 void mcb_plan_free(McbPlan *p);
 /* Project-written autotuner: per world group of the plan, time the quart-grid launch for each tile
  * shape (tile_w in 8/16/32/64) and persistent-workgroup count (one per tile, or 2/4/8 per CU) and keep the

@@ -206,8 +206,11 @@ __device__ __forceinline__ float lerp_f(float t, float a, float b) { return a + 
 __device__ __forceinline__ double lerp_d(double t, double a, double b) { return a + t * (b - a); }
 
 // PerlinNoise.wrap: identity inside +-2^24, else x - floor(x/2^25 + 0.5)*2^25 (mc_noise.h:77,86-89)
+// WRAP = false: the caller has shown that no input reaches +-2^24 (wrap is then the identity), and the
+// per-coordinate test and its branch are left out.
+template <bool WRAP = true>
 __device__ __forceinline__ double wrapc(double x) {
-    if (__builtin_expect(!(fabs(x) < 16777216.0), 0)) x = x - floor(x * (1.0 / 33554432.0) + 0.5) * 33554432.0;
+    if (WRAP && __builtin_expect(!(fabs(x) < 16777216.0), 0)) x = x - floor(x * (1.0 / 33554432.0) + 0.5) * 33554432.0;
     return x;
 }
 
@@ -242,10 +245,10 @@ __device__ __forceinline__ typename R<MODE>::T corners(const typename GT<MODE>::
 }
 
 // Octave whose y coordinate is the octave offset (climate noises, ShiftA): x, z vary.
-template <int MODE, class PA>
+template <int MODE, class PA, bool WRAP = true>
 __device__ __forceinline__ typename R<MODE>::T oct_yc(PA P2, const typename GT<MODE>::T *G, const McbOct &o, double X, double Z) {
     typedef typename R<MODE>::T T;
-    const double x = wrapc(X) + o.xo, z = wrapc(Z) + o.zo;
+    const double x = wrapc<WRAP>(X) + o.xo, z = wrapc<WRAP>(Z) + o.zo;
     const double xf = floor(x), zf = floor(z);
     const int xi = (int)xf & 255, zi = (int)zf & 255, yi = o.yi & 255;
     const T fx = (T)(x - xf), fz = (T)(z - zf);
@@ -259,10 +262,10 @@ __device__ __forceinline__ typename R<MODE>::T oct_yc(PA P2, const typename GT<M
 }
 
 // Octave whose z coordinate is the octave offset (ShiftB): x, y vary.
-template <int MODE, class PA>
+template <int MODE, class PA, bool WRAP = true>
 __device__ __forceinline__ typename R<MODE>::T oct_zc(PA P2, const typename GT<MODE>::T *G, const McbOct &o, double X, double Y) {
     typedef typename R<MODE>::T T;
-    const double x = wrapc(X) + o.xo, y = wrapc(Y) + o.yo;
+    const double x = wrapc<WRAP>(X) + o.xo, y = wrapc<WRAP>(Y) + o.yo;
     const double xf = floor(x), yf = floor(y);
     const int xi = (int)xf & 255, yi = (int)yf & 255, zi = o.zi & 255;
     const T fx = (T)(x - xf), fy = (T)(y - yf);
@@ -281,12 +284,12 @@ __device__ __forceinline__ typename R<MODE>::T oct_zc(PA P2, const typename GT<M
 // reference's sum.  26.3: amp_f[k] * octave(X*f, Y*f) with f = freq[k], or freq[k]*1.0181... for the
 // second stack (NoiseStack.get, mc_noise.h:303-321); 26.1/26.2: amp_d[k] * octave(wrap(X'*f), wrap(Y'*f))
 // * vf_d[k] with X' = X, or X*1.0181... for the second stack (mc_noise.h:286-301).
-template <int MODE, bool ZC, class PA>
+template <int MODE, bool ZC, class PA, bool WRAP = true>
 __device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const typename GT<MODE>::T *G, const McbOct &o, const McbNoise &N, int k,
                                                            int st, double X, double Y) {
     if constexpr (MODE == 1) {
         const double f = st ? N.freq[k] * INPUT_FACTOR : N.freq[k];
-        const float v = ZC ? oct_zc<1>(P, G, o, X * f, Y * f) : oct_yc<1>(P, G, o, X * f, Y * f);
+        const float v = ZC ? oct_zc<1, PA, WRAP>(P, G, o, X * f, Y * f) : oct_yc<1, PA, WRAP>(P, G, o, X * f, Y * f);
         return N.amp_f[k] * v;
     } else {
         const double Xs = st ? X * INPUT_FACTOR : X, Ys = st ? Y * INPUT_FACTOR : Y, f = N.freq[k];
@@ -300,12 +303,12 @@ __device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const typename 
 // offset.  26.3 interleaves first/second layers and accumulates in float (NoiseStack.get,
 // mc_noise.h:303-312; legacy-nether order :313-321); 26.1/26.2 sums each stack in double, then
 // (v1+v2)*valueFactor (mc_noise.h:286-301).
-template <int MODE, bool ZC, int NL, int BASE, bool LEGACY, class PS>
+template <int MODE, bool ZC, int NL, int BASE, bool LEGACY, class PS, bool WRAP = true>
 __device__ __forceinline__ typename R<MODE>::T normal2(PS perm2, const typename GT<MODE>::T *G, const McbOct *oct,
                                                        const McbNoise &N, double X, double Y) {
     auto term = [&](int k, int st) {
         const int o = BASE + st * NL + k;
-        return octave_term<MODE, ZC>(perm_at(perm2, o), G, oct[o], N, k, st, X, Y);
+        return octave_term<MODE, ZC, decltype(perm_at(perm2, o)), WRAP>(perm_at(perm2, o), G, oct[o], N, k, st, X, Y);
     };
     if constexpr (MODE == 1) {
         float v = 0.0f;
@@ -622,22 +625,22 @@ __device__ __forceinline__ void shift_coords(typename R<MODE>::T a, typename R<M
     }
 }
 
-template <int MODE, class PS>
+template <int MODE, class PS, bool WRAP = true>
 __device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, PS perm, const typename GT<MODE>::T *G,
                                           const SplLds *SP, const McbOct *oct, int qx, int qz, Column &c) {
     typedef typename R<MODE>::T T_;
     const McbNoise *N = W->noise;
     const double X = (double)qx, Z = (double)qz;     // blockX*0.25 with blockX = 4*quartX
     // octave slots of the layout MCB_OW_LEVELS {3, 2, 2, 9, 4, 3}: bases 0, 6, 10, 14, 32, 40
-    const T_ a = normal2<MODE, false, 3, 0, false>(perm, G, oct, N[0], X, Z);
-    const T_ b = normal2<MODE, true, 3, 0, false>(perm, G, oct, N[0], Z, X);
+    const T_ a = normal2<MODE, false, 3, 0, false, PS, WRAP>(perm, G, oct, N[0], X, Z);
+    const T_ b = normal2<MODE, true, 3, 0, false, PS, WRAP>(perm, G, oct, N[0], Z, X);
     double x, z;
     shift_coords<MODE>(a, b, X, Z, x, z);
-    const T_ T = normal2<MODE, false, 2, 6, false>(perm, G, oct, N[1], x, z);
-    const T_ H = normal2<MODE, false, 2, 10, false>(perm, G, oct, N[2], x, z);
-    const T_ C = normal2<MODE, false, 9, 14, false>(perm, G, oct, N[3], x, z);
-    const T_ E = normal2<MODE, false, 4, 32, false>(perm, G, oct, N[4], x, z);
-    const T_ Wr = normal2<MODE, false, 3, 40, false>(perm, G, oct, N[5], x, z);
+    const T_ T = normal2<MODE, false, 2, 6, false, PS, WRAP>(perm, G, oct, N[1], x, z);
+    const T_ H = normal2<MODE, false, 2, 10, false, PS, WRAP>(perm, G, oct, N[2], x, z);
+    const T_ C = normal2<MODE, false, 9, 14, false, PS, WRAP>(perm, G, oct, N[3], x, z);
+    const T_ E = normal2<MODE, false, 4, 32, false, PS, WRAP>(perm, G, oct, N[4], x, z);
+    const T_ Wr = normal2<MODE, false, 3, 40, false, PS, WRAP>(perm, G, oct, N[5], x, z);
     column_finish<MODE>(W, SP, T, H, C, E, Wr, c);
 }
 
@@ -661,13 +664,13 @@ __device__ __forceinline__ int ow_depth(const McbWorld *__restrict__ W, const Co
 }
 
 // Nether climate: two legacy NormalNoises at (bx*0.25, by*0.0, bz*0.25) (engine/cuda/mcg_host.c:63-74)
-template <int MODE, class PS>
+template <int MODE, class PS, bool WRAP = true>
 __device__ __forceinline__ void ne_column(const McbWorld *__restrict__ W, PS perm, const typename GT<MODE>::T *G,
                                           const McbOct *oct, int qx, int qz, int *t) {
     const double X = (double)qx, Z = (double)qz;
     // layout MCB_NE_LEVELS {2, 2}, legacy order: bases 0 and 4
-    float T = (float)normal2<MODE, false, 2, 0, true>(perm, G, oct, W->noise[0], X, Z);
-    float H = (float)normal2<MODE, false, 2, 4, true>(perm, G, oct, W->noise[1], X, Z);
+    float T = (float)normal2<MODE, false, 2, 0, true, PS, WRAP>(perm, G, oct, W->noise[0], X, Z);
+    float H = (float)normal2<MODE, false, 2, 4, true, PS, WRAP>(perm, G, oct, W->noise[1], X, Z);
     t[0] = quantize(T); t[1] = quantize(H);
     t[2] = t[3] = t[4] = t[5] = 0;
 }

@@ -96,8 +96,11 @@ int dev_grow(T **p, size_t *cap, size_t n) {
 hipError_t launch_seed_init(const DevWorld &w, const int64_t *d_seeds, const int32_t *d_slot_of, int n,
                             McbSlot *d_slots, hipStream_t stream);
 // tile = tile_w x (256/tile_w)*cpl columns, cpl (1/2/4) columns per lane; max_ny = most layers of any request
+// nowrap: every octave input of these requests stays inside +-2^24 (wrap_free), so the 26.3 kernel may
+// leave out PerlinNoise.wrap's range test
 hipError_t launch_grid(const DevWorld &w, const McbSlot *d_slots, const McbRequest *d_reqs, const McbTile *d_tiles,
-                       int ntiles, uint8_t *d_out, int tile_w, int cpl, int max_ny, int grid_blocks, hipStream_t stream);
+                       int ntiles, uint8_t *d_out, int tile_w, int cpl, int max_ny, int grid_blocks, bool nowrap,
+                       hipStream_t stream);
 hipError_t launch_points(const DevWorld &w, const McbSlot *d_slots, const int32_t *d_slot_of, int ns,
                          const int32_t *d_pts, int np, uint8_t *d_out, int32_t *d_tg, hipStream_t stream);
 // biomes of n quantized climate targets (6 int32 each: t,h,c,e,d,w); Overworld and Nether only
@@ -125,6 +128,11 @@ struct Lowered {
     int max_ny_g = 1, max_ny_q = 1;
 };
 int check_region(const McbRegion &r);
+// True when no octave input of the quart-grid requests' climate columns can reach +-2^24 (so PerlinNoise.wrap,
+// engine/mc_noise.h:86-89, is the identity on every one): the largest quart coordinate, widened by the most
+// the offset noise can shift it (4 x twice the summed amplitudes of both stacks; every octave value is below
+// 2 in magnitude), times the largest frequency of any octave, stays below 2^23.
+bool wrap_free(const McbWorld &w, const std::vector<McbRequest> &reqs);
 void lower_region(const McbRegion &r, int slot, int tile_w, int cpl, Lowered &L);
 
 }  // namespace mcb

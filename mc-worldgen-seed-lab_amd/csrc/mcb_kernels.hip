@@ -117,7 +117,7 @@ __global__ __launch_bounds__(64) void k_seed_end(const McbWorld *__restrict__ W,
 // ================================================================================================
 // One workgroup = one tile of tile_w x (256/tile_w * cpl) columns; lane (lx, lz) evaluates the cpl
 // columns (lx, lz*cpl + c) in order, each column's search warm-started from the previous one.
-template <int MODE, int DIM, int MAXL>
+template <int MODE, int DIM, int MAXL, bool WRAP>
 __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
                                               const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
                                               const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
             uint8_t *o = out + R.out_off + (int64_t)iz * R.nx + ix;
             if (DIM == MCB_OVERWORLD) {
                 Column col;
-                ow_column<MODE>(W, perm, G, SP, oct, qx, qz, col);
+                ow_column<MODE, const uint8_t *, WRAP>(W, perm, G, SP, oct, qx, qz, col);
                 const int t5[5] = {col.t, col.h, col.c, col.e, col.w};
                 int dep[MAXL], b[MAXL];
                 bool in = W->cell_on && cell_in_range(t5[0]) && cell_in_range(t5[1]) && cell_in_range(t5[2]) &&
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                     if (j < ny) o[j * plane] = (uint8_t)b[j];
             } else {
                 int t[6];
-                ne_column<MODE>(W, perm, G, oct, qx, qz, t);
+                ne_column<MODE, const uint8_t *, WRAP>(W, perm, G, oct, qx, qz, t);
                 uint8_t b = (uint8_t)tree_search(tree, tree_n, t, cand[0]);
                 for (int j = 0; j < ny; j++) o[j * plane] = b;
             }
@@ -489,27 +489,38 @@ __global__ __launch_bounds__(256) void k_zoom(const McbRequest *__restrict__ zre
     }
 }
 
-template <int MODE, int DIM, int MAXL>
+template <int MODE, int DIM, int MAXL, bool WRAP>
 hipError_t grid_launch3(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
                         uint8_t *out, int tile_w, int cpl, int blocks, hipStream_t st) {
     if constexpr (DIM == MCB_END)
         hipLaunchKernelGGL((k_grid_end<MODE>), dim3(blocks), dim3(256), 0, st, w.d_world, slots, reqs, tiles, ntiles, tile_w,
                            cpl, out);
     else
-        hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
+        hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL, WRAP>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
                            w.d_cell_recs, slots, reqs, tiles, ntiles, tile_w, cpl, out);
     return hipGetLastError();
 }
+// The 26.3 (float) grid kernel has a variant without PerlinNoise.wrap's range test, for launches whose
+// octave inputs provably stay inside +-2^24 (wrap_free); the test and its branch cost about 9% of the
+// kernel (26.3 grid 2.84 -> 2.59 ms).  The 26.1/26.2 kernel keeps one form: without the test it measured
+// slower (3.74 -> 3.88 ms).
+template <int MODE, int DIM, int MAXL>
+hipError_t grid_launch2(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
+                        uint8_t *out, int tile_w, int cpl, int blocks, bool nowrap, hipStream_t st) {
+    if constexpr (MODE == 1 && DIM != MCB_END)
+        if (nowrap) return grid_launch3<MODE, DIM, MAXL, false>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+    return grid_launch3<MODE, DIM, MAXL, true>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+}
 template <int MODE, int DIM>
 hipError_t grid_launch(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
-                       uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, hipStream_t st) {
+                       uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, bool nowrap, hipStream_t st) {
     if constexpr (DIM != MCB_OVERWORLD) {
-        return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+        return grid_launch2<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, nowrap, st);
     } else {
-        if (max_ny <= 1) return grid_launch3<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
-        if (max_ny <= 2) return grid_launch3<MODE, DIM, 2>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
-        if (max_ny <= 4) return grid_launch3<MODE, DIM, 4>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
-        return grid_launch3<MODE, DIM, 8>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
+        if (max_ny <= 1) return grid_launch2<MODE, DIM, 1>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, nowrap, st);
+        if (max_ny <= 2) return grid_launch2<MODE, DIM, 2>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, nowrap, st);
+        if (max_ny <= 4) return grid_launch2<MODE, DIM, 4>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, nowrap, st);
+        return grid_launch2<MODE, DIM, 8>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, nowrap, st);
     }
 }
 // Points per seed from which staging the seed in LDS (k_points) beats reading its slot (k_points_g),
@@ -545,7 +556,7 @@ hipError_t launch_seed_init(const DevWorld &w, const int64_t *seeds, const int32
 }
 
 hipError_t launch_grid(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
-                       uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, hipStream_t st) {
+                       uint8_t *out, int tile_w, int cpl, int max_ny, int blocks, bool nowrap, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
     if (tile_w <= 0 || tile_w > 256 || 256 % tile_w || cpl < 1 || cpl > 64 || max_ny < 1 || max_ny > 8)
         return hipErrorInvalidValue;
@@ -553,14 +564,14 @@ hipError_t launch_grid(const DevWorld &w, const McbSlot *slots, const McbRequest
     const bool f = w.host.fmt_new != 0;
     switch (w.dim) {
         case MCB_OVERWORLD:
-            return f ? grid_launch<1, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st)
-                     : grid_launch<0, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st);
+            return f ? grid_launch<1, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, nowrap, st)
+                     : grid_launch<0, MCB_OVERWORLD>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, nowrap, st);
         case MCB_NETHER:
-            return f ? grid_launch<1, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st)
-                     : grid_launch<0, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st);
+            return f ? grid_launch<1, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, nowrap, st)
+                     : grid_launch<0, MCB_NETHER>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, nowrap, st);
         default:
-            return f ? grid_launch<1, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st)
-                     : grid_launch<0, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, st);
+            return f ? grid_launch<1, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, nowrap, st)
+                     : grid_launch<0, MCB_END>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, max_ny, blocks, nowrap, st);
     }
 }
 

@@ -27,6 +27,23 @@ int check_region(const McbRegion &r) {
     return 0;
 }
 
+bool wrap_free(const McbWorld &w, const std::vector<McbRequest> &reqs) {
+    double fmax = 0.0, shift = 0.0;
+    for (int i = 0; i < w.n_noise; i++)
+        for (int k = 0; k < w.noise[i].nlev; k++) {
+            fmax = std::max(fmax, w.noise[i].freq[k] * 1.0181268882175227 * 1.0181268882175227);
+            if (w.dim == MCB_OVERWORLD && i == 0) shift += 4.0 * 2.0 * 2.0 * std::fabs((double)w.noise[0].amp_f[k]);
+        }
+    double q = 0.0;
+    for (const McbRequest &r : reqs) {
+        q = std::max(q, std::fabs((double)r.x0));
+        q = std::max(q, std::fabs((double)r.x0 + r.nx - 1));
+        q = std::max(q, std::fabs((double)r.z0));
+        q = std::max(q, std::fabs((double)r.z0 + r.nz - 1));
+    }
+    return w.fmt_new && (q + shift + 1.0) * fmax < 8388608.0;
+}
+
 static void add_tiles(std::vector<McbTile> &tiles, int req, int nx, int nz, int tw, int th) {
     for (int tz = 0; tz < (nz + th - 1) / th; tz++)
         for (int tx = 0; tx < (nx + tw - 1) / tw; tx++) tiles.push_back(McbTile{req, (int16_t)tx, (int16_t)tz});
@@ -146,6 +163,7 @@ struct Group {
     int64_t quart_cap = 0;
     int tile_w = 16, cpl = 1, blocks_arg = 0, blocks_g = 0, blocks_q = 0;
     int max_ny_g = 1, max_ny_q = 1;
+    bool nowrap_g = false, nowrap_q = false;       // wrap_free over the quart-grid / zoom-layer requests
     int64_t columns = 0, samples = 0;
 
     void release() {
@@ -180,6 +198,8 @@ struct Group {
         cpl = cp;
         max_ny_g = L.max_ny_g;
         max_ny_q = L.max_ny_q;
+        nowrap_g = wrap_free(e->w.host, L.g);
+        nowrap_q = wrap_free(e->w.host, L.q);
         blocks_arg = blocks;
         blocks_g = blocks > 0 ? std::min(blocks, std::max(ngtiles, 1)) : ngtiles;
         blocks_q = blocks > 0 ? std::min(blocks, std::max(nqtiles, 1)) : nqtiles;
@@ -195,8 +215,8 @@ struct Group {
         const int bg = std::min(blocks_g, std::max(ng, 1)), bq = std::min(blocks_q, std::max(nq, 1));
         switch (s) {
             case 0: MCB_HIPCHK(launch_seed_init(w, d_seeds, d_slot_of, nseeds, d_slots, st)); break;
-            case 1: MCB_HIPCHK(launch_grid(w, d_slots, d_greqs, d_gtiles, ng, d_out, tile_w, cpl, max_ny_g, bg, st)); break;
-            case 2: MCB_HIPCHK(launch_grid(w, d_slots, d_qreqs, d_qtiles, nq, d_quart, tile_w, cpl, max_ny_q, bq, st)); break;
+            case 1: MCB_HIPCHK(launch_grid(w, d_slots, d_greqs, d_gtiles, ng, d_out, tile_w, cpl, max_ny_g, bg, nowrap_g, st)); break;
+            case 2: MCB_HIPCHK(launch_grid(w, d_slots, d_qreqs, d_qtiles, nq, d_quart, tile_w, cpl, max_ny_q, bq, nowrap_q, st)); break;
             default: MCB_HIPCHK(launch_zoom(d_zreqs, d_qreqs, d_ztiles, nztiles, d_quart, d_out, st)); break;
         }
         return 0;
@@ -351,6 +371,16 @@ int mcb_plan_info(const McbPlan *p, int64_t *tiles, int64_t *columns, int64_t *s
     return MCB_OK;
 }
 
+int mcb_plan_shape(const McbPlan *p, int group, int *tile_w, int *cpl, int *blocks, int *wrap_free_grid) {
+    if (!p || group < 0 || group >= (int)p->groups.size()) return fail(MCB_E_ARG, "no plan group %d", group);
+    const Group &G = p->groups[group];
+    if (tile_w) *tile_w = G.tile_w;
+    if (cpl) *cpl = G.cpl;
+    if (blocks) *blocks = G.blocks_arg;
+    if (wrap_free_grid) *wrap_free_grid = G.nowrap_g ? 1 : 0;
+    return MCB_OK;
+}
+
 void mcb_plan_free(McbPlan *p) {
     if (!p) return;
     p->drop_graph();
@@ -360,7 +390,9 @@ void mcb_plan_free(McbPlan *p) {
 
 // trial launches run a prefix of each tile list, long enough to fill the device many times over; the
 // whole-plan time is estimated as trial time x (tiles / trial tiles)
-static const int kTuneTiles = 8192;
+// Autotuner trials: a prefix of each grid launch holding about this many columns, the same work for every
+// tile shape (tiles hold 256 x cols_per_lane columns), timed and scaled to the whole launch.
+static const int64_t kTuneColumns = 1 << 24;
 
 int mcb_plan_autotune(McbPlan *p, uint8_t *d_out, void *stream, int *tile_w_out, double *best_ms_out) {
     if (!p || !d_out) return fail(MCB_E_ARG, "bad arguments");
@@ -388,17 +420,17 @@ int mcb_plan_autotune(McbPlan *p, uint8_t *d_out, void *stream, int *tile_w_out,
                 for (int k : per_cu) {
                     const int blocks = k ? k * cus : 0;
                     if (int rc = G.lower(tw, cp, blocks)) return rc;
+                    const int trial = (int)std::max<int64_t>(1, kTuneColumns / (256 * cp));
                     float ms = 0.f, acc = 0.f;
                     for (int rep = 0; rep < 3; rep++) {    // first rep warms caches, keep the best of the rest
                         MCB_HIPCHK(hipEventRecord(e0, st));
-                        if (int rc = G.stage(1, d_out, st, kTuneTiles)) return rc;
-                        if (int rc = G.stage(2, d_out, st, kTuneTiles)) return rc;
+                        if (int rc = G.stage(1, d_out, st, trial)) return rc;
+                        if (int rc = G.stage(2, d_out, st, trial)) return rc;
                         MCB_HIPCHK(hipEventRecord(e1, st));
                         MCB_HIPCHK(hipEventSynchronize(e1));
                         MCB_HIPCHK(hipEventElapsedTime(&ms, e0, e1));
                         if (rep == 1 || (rep > 1 && ms < acc)) acc = ms;
                     }
-                    const int trial = std::max(1, std::min(std::max(G.ngtiles, G.nqtiles), kTuneTiles));
                     acc *= (float)(G.ngtiles + G.nqtiles) / (float)std::max(1, std::min(G.ngtiles, trial) + std::min(G.nqtiles, trial));
                     if (acc < best) { best = acc; best_tw = tw; best_cpl = cp; best_blocks = blocks; }
                 }

@@ -260,9 +260,9 @@ int launch_batch(McbServer *S, Lane &L) {
     for (auto &B : wb) {
         const DevWorld &w = B.pool->e->w;
         MCB_HIPCHK(launch_grid(w, B.pool->d_slots, (const McbRequest *)dp(B.g), (const McbTile *)dp(B.gt), (int)B.gt.n,
-                               L.d_out, S->tile_w, S->cpl, B.L.max_ny_g, (int)B.gt.n, L.st));
+                               L.d_out, S->tile_w, S->cpl, B.L.max_ny_g, (int)B.gt.n, wrap_free(w.host, B.L.g), L.st));
         MCB_HIPCHK(launch_grid(w, B.pool->d_slots, (const McbRequest *)dp(B.q), (const McbTile *)dp(B.qt), (int)B.qt.n,
-                               L.d_quart, S->tile_w, S->cpl, B.L.max_ny_q, (int)B.qt.n, L.st));
+                               L.d_quart, S->tile_w, S->cpl, B.L.max_ny_q, (int)B.qt.n, wrap_free(w.host, B.L.q), L.st));
         MCB_HIPCHK(launch_zoom((const McbRequest *)dp(B.z), (const McbRequest *)dp(B.q), (const McbTile *)dp(B.zt),
                                (int)B.zt.n, L.d_quart, L.d_out, L.st));
     }

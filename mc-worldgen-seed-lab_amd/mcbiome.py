@@ -118,6 +118,7 @@ def lib():
         L.mcb_plan_run_graph.argtypes = [P, P, P]
         L.mcb_plan_run_stage.argtypes = [P, C.c_int, P, P]
         L.mcb_plan_info.argtypes = [P, i64p, i64p, i64p, i64p]
+        L.mcb_plan_shape.argtypes = [P, C.c_int, i32p, i32p, i32p, i32p]
         L.mcb_plan_free.argtypes = [P]
         L.mcb_plan_autotune.argtypes = [P, P, P, i32p, C.POINTER(C.c_double)]
         L.mcb_context_create.argtypes = [C.POINTER(P), C.c_char_p]
@@ -252,6 +253,12 @@ class Plan:
     def run(self, d_out_ptr, stream_ptr=0, graph=False):
         fn = lib().mcb_plan_run_graph if graph else lib().mcb_plan_run
         _check(fn(self.h, C.c_void_p(d_out_ptr), C.c_void_p(stream_ptr or None)))
+
+    def shape(self, group=0):
+        """(tile_w, cols_per_lane, blocks, wrap_free) of a world group's launches."""
+        v = [C.c_int32() for _ in range(4)]
+        _check(lib().mcb_plan_shape(self.h, group, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def autotune(self, d_out_ptr, stream_ptr=0):
         tw, ms = C.c_int32(), C.c_double()

@@ -98,3 +98,30 @@ def test_empty_and_bad_inputs():
         eng.plan([mcbiome.make_region(1, 0, 0, 0, 4, [16])])
     with pytest.raises(mcbiome.EngineError):
         eng.plan([mcbiome.make_region(1, 0, 0, 4, 4, list(range(9)))])
+
+
+@pytest.mark.parametrize("version", ["26.3", "26.1"])
+def test_wrap_free_kernel_choice_and_parity(version):
+    """The 26.3 grid runs without PerlinNoise.wrap's range test only when every octave input of the
+    launch provably stays inside +-2^24 (quart coordinates up to about 8e6 in 26.3); plans reaching
+    beyond keep the test.  Results equal the oracle on both sides of the threshold, in one plan and
+    mixed (one far request puts the whole launch on the tested form)."""
+    eng = mcbiome.Engine(version, "overworld")
+    orc = O.Oracle(version, "overworld")
+    near = [(11, -2**20, 2**20 - 40), (-5, 8_000_000 - 64, -8_000_000)]            # inside: no test needed
+    far = [(77, 2**26 - 30, -(2**24) - 7), (1, -(2**27) + 5, 2**25 + 3)]           # wrap decides rounding
+    for cases, expect_free in ((near, version == "26.3"), (far, False), (near + far, False)):
+        regs, off = [], 0
+        for seed, x0, z0 in cases:
+            regs.append(mcbiome.make_region(seed, x0, z0, 40, 24, [16, -4], False, off))
+            off += 40 * 24 * 2
+        plan = eng.plan(regs, tile_w=16 | 2 << 16)
+        assert plan.shape()[3] == (1 if expect_free else 0), cases
+        out = _dev_bytes(off)
+        plan.run(out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        host = out.cpu().numpy()
+        for r in regs:
+            got = host[r.out_off: r.out_off + 2 * 40 * 24].reshape(2, 24, 40)
+            for j, y in enumerate([16, -4]):
+                assert np.array_equal(got[j], orc.grid(r.seed, r.x0, r.z0, 40, 24, y, 0)), (r.seed, r.x0, j)
