@@ -293,7 +293,9 @@ __device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const typename 
         return N.amp_f[k] * v;
     } else {
         const double Xs = st ? X * INPUT_FACTOR : X, Ys = st ? Y * INPUT_FACTOR : Y, f = N.freq[k];
-        const double v = ZC ? oct_zc<0>(P, G, o, wrapc(Xs * f), wrapc(Ys * f)) : oct_yc<0>(P, G, o, wrapc(Xs * f), wrapc(Ys * f));
+        // the octave's own wrap is the identity on an already wrapped input (|x| <= 2^24): left out
+        const double v = ZC ? oct_zc<0, PA, false>(P, G, o, wrapc(Xs * f), wrapc(Ys * f))
+                            : oct_yc<0, PA, false>(P, G, o, wrapc(Xs * f), wrapc(Ys * f));
         return N.amp_d[k] * v * N.vf_d[k];
     }
 }
