@@ -22,6 +22,7 @@
 
 #include "mcb_internal.h"
 #include "mcg_plugin.h"
+#include "mcg_terrain.h"
 
 using namespace mcb::plug;
 
@@ -737,6 +738,56 @@ int mcb::plug::chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, u
     g->release();
     if (!ok && err && errlen) snprintf(err, errlen, "%s", mcb_last_error());
     return ok ? 0 : 1;
+}
+
+mcb::plug::PluginGate::PluginGate(bool needs_device) : lock(gate_mutex()), ready(!needs_device || binding().make_current()) {}
+
+bool mcb::plug::plugin_fail(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    mcb::failv(MCB_E_DEVICE, fmt, ap);
+    va_end(ap);
+    return false;
+}
+
+bool mcb::plug::point_program(const McgProg *p, PWorld &P, char **d_data, int *lds) {
+    *d_data = nullptr;
+    memset(&P, 0, sizeof P);
+    if (!p || p->old) return plugin_error("the point program must be a 26.3+ (float) program");
+    Compiled C;
+    if (!compile_prog(p, C)) return false;
+    *lds = grad_lds_bytes(true) + C.nslot * kWG * 4;
+    if (*lds > kMaxLdsBytes) return plugin_error("point program keeps %d values alive at once", C.nslot);
+    struct Part { const void *src; size_t bytes; size_t off; };
+    Part parts[] = {{C.ins.data(), C.ins.size() * sizeof(PIns), 0},        {p->noise, (size_t)p->nnoise * sizeof(McgNoise), 0},
+                    {p->layers, (size_t)p->nlayers * sizeof(McgLayer), 0}, {p->octs, (size_t)p->noct * sizeof(McgOct), 0},
+                    {p->sp, (size_t)p->nsp * sizeof(McgSpline), 0},        {p->splf, (size_t)p->nsplf * sizeof(float), 0},
+                    {p->spc, (size_t)p->nspc * sizeof(int32_t), 0},        {C.arr.data(), C.arr.size() * sizeof(int32_t), 0},
+                    {p->thr, (size_t)p->nthr * sizeof(double), 0}};
+    size_t total = 0;
+    for (auto &q : parts) { q.off = total; total += (q.bytes + 255) & ~(size_t)255; }
+    HCHK(hipMalloc((void **)d_data, std::max<size_t>(total, 256)));
+    for (auto &q : parts)
+        if (q.bytes) HCHK(hipMemcpy(*d_data + q.off, q.src, q.bytes, hipMemcpyHostToDevice));
+    auto at = [&](int i) { return (const void *)(*d_data + parts[i].off); };
+    P.ins = (const PIns *)at(0);
+    P.noise = (const McgNoise *)at(1);
+    P.layers = (const McgLayer *)at(2);
+    P.octs = (const McgOct *)at(3);
+    P.sp = (const McgSpline *)at(4);
+    P.splf = (const float *)at(5);
+    P.spc = (const int32_t *)at(6);
+    P.arr = (const int32_t *)at(7);
+    P.thr = (const double *)at(8);
+    P.nins = C.nins;
+    P.nydep = (int32_t)C.ins.size() - C.nins;
+    P.nnoise = p->nnoise;
+    P.nlayers = p->nlayers;
+    P.noct = p->noct;
+    P.nslot = C.nslot;
+    P.spl_depth = C.spl_depth;
+    for (int k = 0; k < 6; k++) P.root_slot[k] = C.root_slot[k];
+    return true;
 }
 
 namespace {

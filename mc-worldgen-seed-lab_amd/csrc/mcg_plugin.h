@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/mcgpu.h"
 
 namespace mcb {
@@ -105,6 +107,17 @@ int chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, uint8_t *out
 // The grid and points calls of the handle are mcgpu_biome_grid / mcgpu_biome_points.
 void *world_new(const McgBiomeWorld *W, int chain_cap, char *err, size_t errlen);
 int world_chunks(void *h, int n, const int32_t *cxz, uint8_t *out, int *nunres);
+
+// Entry points of the plugin ABI run inside a PluginGate: the process-wide lock (calls are serialised, as
+// the reference's are) and, when needs_device, the bound device made current (ready = false with the
+// error set when it cannot be).  plugin_fail sets the thread's error text (mcgpu_last_error) and returns
+// false.
+struct PluginGate {
+    std::unique_lock<std::mutex> lock;
+    bool ready;
+    explicit PluginGate(bool needs_device);
+};
+bool plugin_fail(const char *fmt, ...);
 
 // launchers (mcg_plugin.hip); lds = bytes of dynamic LDS, perm_lds = permutation tables staged in LDS
 hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st);

@@ -73,6 +73,27 @@ class McgTreeNode(C.Structure):
                 ("biome", C.c_int32), ("pad", C.c_int32)]
 
 
+MCG_MAX_VEINS = 4
+
+
+class McgVein(C.Structure):
+    _fields_ = [("density_root", C.c_int), ("richness_root", C.c_int), ("gap_root", C.c_int), ("ore", C.c_int),
+                ("raw", C.c_int), ("filler", C.c_int), ("raw_chance", C.c_float)]
+
+
+class McgTerrainWorld(C.Structure):
+    _fields_ = [("vol", C.POINTER(McgProg)), ("gap", C.POINTER(McgProg)), ("nmin", C.c_int), ("nh", C.c_int),
+                ("root_density", C.c_int), ("nveins", C.c_int), ("vein", McgVein * MCG_MAX_VEINS),
+                ("ore_lo", C.c_uint64), ("ore_hi", C.c_uint64), ("veins_on", C.c_int),
+                ("cache_ids", C.c_int * 32), ("ncache_ids", C.c_int), ("pre_root", C.c_int),
+                ("pre_sx", C.c_int), ("pre_sy", C.c_int), ("pre_sz", C.c_int), ("pre_dx", C.c_int), ("pre_dy", C.c_int),
+                ("pre_dz", C.c_int), ("pre_y0", C.c_int), ("pre_xoff", C.c_int), ("pre_zoff", C.c_int)]
+
+
+# Beardifier callback: (ud, chunk_index, vol[9], out) -> 0 when the chunk has no Beardifier
+McgBeardFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float))
+
+
 class McgBiomeWorld(C.Structure):
     _fields_ = [("source", C.c_int), ("fixed_biome", C.c_int), ("end_ids", C.c_int * 5), ("newf", C.c_int),
                 ("min_qy", C.c_int), ("qh", C.c_int), ("zoom_seed", C.c_int64), ("prog", C.POINTER(McgProg)),
@@ -100,6 +121,20 @@ def lib():
         L.mcgpu_biome_free.argtypes = [P]
         L.mcgpu_biome_grid.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p, ip]
         L.mcgpu_biome_points.argtypes = [P, C.c_int, ip, u8p, ip]
+        sp = C.POINTER(C.c_size_t)
+        L.mcgpu_terrain_new.restype = P
+        L.mcgpu_terrain_new.argtypes = [P, C.c_int, C.c_char_p, C.c_size_t]
+        L.mcgpu_terrain_free.argtypes = [P]
+        L.mcgpu_terrain_max_chunks.argtypes = [P]
+        L.mcgpu_terrain_cell_floats.restype = C.c_size_t
+        L.mcgpu_terrain_cell_floats.argtypes = [P]
+        L.mcgpu_terrain_cell_info.argtypes = [P, C.c_int, ip, ip, sp, sp]
+        L.mcgpu_terrain_cell_cand.argtypes = [P, C.c_int, C.c_int, ip, ip, ip, ip]
+        L.mcgpu_terrain_batch.argtypes = [P, C.c_int, ip, ip, McgBeardFn, P, C.POINTER(C.c_float), C.POINTER(C.c_uint16),
+                                          C.POINTER(C.c_float), C.POINTER(C.c_int8)]
+        L.mcgpu_host_alloc.restype = P
+        L.mcgpu_host_alloc.argtypes = [C.c_size_t]
+        L.mcgpu_host_free.argtypes = [P]
         _bound = L
     return L
 
@@ -144,6 +179,70 @@ class BiomeWorld:
     def close(self):
         if getattr(self, "h", None):
             lib().mcgpu_biome_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TerrainWorld:
+    """A terrain world created with mcgpu_terrain_new from a McgTerrainWorld (a ctypes instance)."""
+
+    def __init__(self, world, max_chunks=64):
+        L = lib()
+        err = C.create_string_buffer(512)
+        self.world = world
+        self.h = L.mcgpu_terrain_new(C.byref(world), max_chunks, err, len(err))
+        if not self.h:
+            raise PluginError(err.value.decode(errors="replace"))
+        self.max_chunks = L.mcgpu_terrain_max_chunks(self.h)
+        self.cell_floats = L.mcgpu_terrain_cell_floats(self.h)
+
+    def cells(self):
+        """[(cid, [(vol9, xrel, zrel, n), ...], cap, off)] per McgTerrainWorld.cache_ids entry"""
+        L = lib()
+        out = []
+        for k in range(self.world.ncache_ids):
+            cid, nc, cap, off = C.c_int(), C.c_int(), C.c_size_t(), C.c_size_t()
+            if L.mcgpu_terrain_cell_info(self.h, k, C.byref(cid), C.byref(nc), C.byref(cap), C.byref(off)):
+                raise PluginError("cell_info %d" % k)
+            cands = []
+            for j in range(nc.value):
+                vol = (C.c_int * 9)()
+                xr, zr, n = C.c_int(), C.c_int(), C.c_int()
+                L.mcgpu_terrain_cell_cand(self.h, k, j, vol, C.byref(xr), C.byref(zr), C.byref(n))
+                cands.append((tuple(vol), xr.value, zr.value, n.value))
+            out.append((cid.value, cands, cap.value, off.value))
+        return out
+
+    def batch(self, cx, cz, beard=None, veins=True, cells=False):
+        """(density (n, 16*nh*16) float32, veins (n, 16*nh*16) uint16 or None, cells (n, cell_floats) or None,
+        which (n, ncache) int8 or None)"""
+        cx = np.ascontiguousarray(cx, np.int32)
+        cz = np.ascontiguousarray(cz, np.int32)
+        n = len(cx)
+        nd = 256 * self.world.nh
+        dens = np.empty((n, nd), np.float32)
+        vb = np.empty((n, nd), np.uint16) if veins and self.world.veins_on else None
+        nc = self.world.ncache_ids
+        cb = np.empty((n, max(self.cell_floats, 1)), np.float32) if cells else None
+        wb = np.empty((n, max(nc, 1)), np.int8) if cells else None
+        fn = beard if beard is not None else McgBeardFn()
+        rc = lib().mcgpu_terrain_batch(self.h, n, cx.ctypes.data_as(C.POINTER(C.c_int)), cz.ctypes.data_as(C.POINTER(C.c_int)),
+                                       fn, None, dens.ctypes.data_as(C.POINTER(C.c_float)),
+                                       vb.ctypes.data_as(C.POINTER(C.c_uint16)) if vb is not None else None,
+                                       cb.ctypes.data_as(C.POINTER(C.c_float)) if cb is not None else None,
+                                       wb.ctypes.data_as(C.POINTER(C.c_int8)) if wb is not None else None)
+        if rc:
+            raise PluginError(last_error())
+        return dens, vb, cb, wb
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mcgpu_terrain_free(self.h)
             self.h = None
 
     def __del__(self):

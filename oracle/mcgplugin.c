@@ -329,6 +329,9 @@ static double vold(const McgProg *P, int i, int bx, int by, int bz) {           
 }
 static double vnode(const McgProg *P, int i, int x, int y, int z) { return P->old ? vold(P, i, x, y, z) : (double)vnew(P, i, x, y, z); }
 
+float or_mcg_point_f(const McgProg *P, int node, int x, int y, int z) { return vnew(P, node, x, y, z); }
+float or_mcg_stack_f(const McgProg *P, int ns, double x, double y, double z) { return stack_f(P, ns, x, y, z); }
+
 int or_mcg_climate(const McgBiomeWorld *W, int bx, int by, int bz, double out6[6]) {
     const McgProg *P = W->prog;
     for (int k = 0; k < 6; k++) out6[k] = P->root[k] >= 0 ? vnode(P, P->root[k], bx, by, bz) : 0.0;

@@ -29,6 +29,10 @@ int or_mcg_quart_history(const McgBiomeWorld *W, int qx0, int qz0, int nx, int n
  * chain followed (12: the plugin ABI; -1: none, libmcgen's CPU mcgen_biome_at) */
 int or_mcg_points(const McgBiomeWorld *W, int n, const int *xyz, uint8_t *out, int *nunres, int cap);
 int or_mcg_grid(const McgBiomeWorld *W, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out, int *nunres, int cap);
+/* the value of node `node` of a 26.3+ program at a block (point path, no caches), and a NoiseStack value
+ * (noise.c:339-347) at a position: the point pieces of the volume restatement in mcgterrain.c */
+float or_mcg_point_f(const McgProg *P, int node, int x, int y, int z);
+float or_mcg_stack_f(const McgProg *P, int ns, double x, double y, double z);
 
 #ifdef __cplusplus
 }
