@@ -415,6 +415,79 @@ def bench_endfield(args, torch, mcbiome):
             "checksum": float(np.float64(out.sum())), "cpu_baseline": cpu}
 
 
+def bench_terrain(args, torch, mcbiome):
+    """Terrain final_density through mcgpu_terrain_* (1024 seeds of splitmix64(4), 16x16 chunks each, the
+    density at the corners of 4x8x4-block cells over y in [-64, 320], interpolated to every block).  The
+    world is the engine's stand-in Overworld terrain (mcb_plugin_terrain_world: the vanilla 26.3 climate
+    router and offset spline under a final_density of the game's shape; the game's own graph is datapack
+    JSON).  Per seed the job builds the world (seed state on the device), plans it and evaluates its 256
+    chunks in one batch with the density left in HBM; the first args.terrain_seeds seeds are timed."""
+    import mcgpu
+    seeds = [signed64(v) for v in splitmix64_stream(4, 1024 + 2)]
+    origins = [(uniform(seeds[1024], 1 << 16), uniform(seeds[1025], 1 << 16))]
+    ox, oz = origins[0]
+    gx, gz = np.meshgrid(np.arange(16), np.arange(16), indexing="xy")
+    cx = (gx.ravel() + ox).astype(np.int32)
+    cz = (gz.ravel() + oz).astype(np.int32)
+    eng = mcbiome.Engine("26.3", "overworld")
+
+    def job(seed):
+        tw = eng.terrain_world(seed)
+        G = mcgpu.TerrainWorld(tw.world, max_chunks=256)
+        assert G.max_chunks == 256
+        G.batch_device(cx, cz)
+        G.close()
+        tw.close()
+
+    for k in range(max(args.warmup, 1)):
+        job(seeds[k])
+    n = min(args.terrain_seeds, 1024)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in seeds[:n]:
+        job(s)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / n
+    # one world: device batches alone, and with the density copied to pageable host memory
+    tw = eng.terrain_world(seeds[0])
+    G = mcgpu.TerrainWorld(tw.world, max_chunks=256)
+    G.batch_device(cx, cz)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        G.batch_device(cx, cz)
+    kt = (time.perf_counter() - t1) / args.steps
+    t2 = time.perf_counter()
+    dens, _, _, _ = G.batch(cx, cz, veins=False)
+    ht = time.perf_counter() - t2
+    nh = tw.world.nh
+    corners = 5 * (nh // 8 + 1) * 5
+    cpu = None
+    if not args.no_cpu:
+        import terrain as TR
+        t3 = time.perf_counter()
+        k = 0
+        while k < 256 and (k < 2 or time.perf_counter() - t3 < 10.0):
+            TR.chunk(tw.world, int(cx[k]), int(cz[k]))
+            k += 1
+        cpu = {"value": round(k / (time.perf_counter() - t3), 3), "unit": "chunks/s", "cores": 1, "kind": "port",
+               "sample": "%d chunks of seed 0 through the oracle's C restatement of libmcgen's sampleVolume "
+                         "(oracle/mcgterrain.c), one thread" % k}
+    G.close()
+    tw.close()
+    return {"metric": "terrain final_density: chunks/s (16 x 384 x 16 block densities from 5 x 49 x 5 cell corners), "
+                      "per-seed world build + plan + one 256-chunk batch, density resident in HBM",
+            "value": round(256 / dt, 1), "unit": "chunks/s", "n_gpus": 1, "steps": n, "warmup": max(args.warmup, 1),
+            "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (f64 coordinates)", "data": "synthetic (splitmix64(4) seeds and chunk origin)",
+            "config": {"workload": "terrain: %d of 1024 seeds x 16x16 chunks, y -64..320, 4x8x4 cells, "
+                                   "stand-in Overworld final_density over the vanilla 26.3 climate" % n,
+                       "chunk_origin": [int(ox), int(oz)]},
+            "batch_only_ms": round(kt * 1e3, 3), "batch_only_chunks_per_s": round(256 / kt, 1),
+            "corners_per_s": round(256 * corners / kt, 1), "block_densities_per_s": round(256 * 256 * nh / kt, 1),
+            "host_copy_batch_ms": round(ht * 1e3, 3),
+            "checksum": float(np.float64(dens.astype(np.float64).sum())), "cpu_baseline": cpu}
+
+
 def bench_plugin(args, torch, mcbiome):
     """libmcgen GPU plugin (mcgpu_biome_grid) on the engine's vanilla world for one seed: a 4096x4096
     block grid at y 64 (step 1) per call, origin from splitmix64(5); the call is synchronous and returns
@@ -749,7 +822,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="batched", choices=["batched", "plain", "mixed", "serve", "search", "endfield", "plugin", "chunks",
-                                                              "mcgen", "points"])
+                                                              "mcgen", "points", "terrain"])
     ap.add_argument("--search-candidates", type=int, default=1 << 28)
     ap.add_argument("--tile-w", type=int, default=0, help="fixed tile (width | cols_per_lane<<16); 0 = autotune")
     ap.add_argument("--graph", action="store_true", help="replay each plan as a hipGraph")
@@ -759,6 +832,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--mixed-requests", type=int, default=10000)
+    ap.add_argument("--terrain-seeds", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -778,7 +852,7 @@ def main():
         del src, dst
     fn = {"batched": bench_batched, "plain": bench_plain, "mixed": bench_mixed, "serve": bench_serve,
           "search": bench_search, "endfield": bench_endfield, "plugin": bench_plugin,
-          "chunks": bench_chunks, "mcgen": bench_mcgen, "points": bench_points}[args.workload]
+          "chunks": bench_chunks, "mcgen": bench_mcgen, "points": bench_points, "terrain": bench_terrain}[args.workload]
     print(json.dumps(fn(args, torch, mcbiome)))
 
 

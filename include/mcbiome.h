@@ -268,6 +268,20 @@ struct McgBiomeWorld;
 int mcb_plugin_world(McbEngine *e, int64_t seed, struct McgBiomeWorld **out);
 void mcb_plugin_world_free(struct McgBiomeWorld *w);
 
+/* A terrain world (McgTerrainWorld of include/mcgpu.h) over the engine's vanilla 26.3+ Overworld climate
+ * for one seed, shaped like the game's Overworld final_density: depth from the y gradient and the offset
+ * spline, a sloped body with a 3D noise, a cave noise, a top slide, interpolation over 4 x 8 x 4 cells and
+ * a final squeeze.  The game's own graph is datapack JSON the engine does not carry, so this is a stand-in
+ * of the same shape, not the vanilla function.  Feeds mcgpu_terrain_new without libmcgen (benchmarks,
+ * tests).  No counterpart in the reference.  Free with mcb_plugin_terrain_world_free. */
+struct McgTerrainWorld;
+int mcb_plugin_terrain_world(McbEngine *e, int64_t seed, struct McgTerrainWorld **out);
+void mcb_plugin_terrain_world_free(struct McgTerrainWorld *w);
+/* mcgpu_terrain_batch with the density left in device memory for device-side consumers (no host copy,
+ * no vein patches or cache cells): *dens = chunk i's 16 * nh * 16 floats at *dens + i * 16*nh*16, in
+ * mcgpu_terrain_batch's layout, valid until the handle's next batch.  No counterpart in the reference. */
+int mcb_terrain_batch_device(void *h, int n, const int *cx, const int *cz, const float **dens);
+
 /* Whole chunks as libmcgen stores them (world_chunk_biomes, libmcgen/src/biome.c:125-157): for each of n
  * chunks (cx, cz) in cxz, qh*16 bytes out[c*qh*16 + (qy*4 + qz)*4 + qx], qy counted from the world's lowest
  * quart (Overworld: 96 layers from quart -16; Nether, End: 64 from 0).  Each chunk's fitness ties are

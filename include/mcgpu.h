@@ -150,6 +150,34 @@ int mcgpu_biome_grid(void *h, int x0, int z0, int nx, int nz, int step, int y, u
 /* out[i] = biome at block (xyz[3i], xyz[3i+1], xyz[3i+2]) (mcgpu_biome.cu:252-280). */
 int mcgpu_biome_points(void *h, int n, const int *xyz, unsigned char *out, int *nunres);
 
+/* ---- terrain exports, optional for libmcgen (bound by gpu_bridge.c:141-144, used by gpu_bridge.c:503-656) ----
+ * The volume executor (mcgpu_terrain.cu:556-862): mcgpu_terrain_new plans the world's final_density (and
+ * vein roots) once; max_chunks caps the chunks of one batch (the result may be smaller: device memory).
+ * Node kinds log, pow, round and find_top_surface, and interpolation or slices over volume shapes other
+ * than the terrain's own, are refused at creation (the reference refuses the same).  NULL + err on
+ * failure.  Copies everything it needs from *world. */
+void *mcgpu_terrain_new(const McgTerrainWorld *world, int max_chunks, char *err, size_t errlen);
+void mcgpu_terrain_free(void *h);
+int mcgpu_terrain_max_chunks(void *h);
+/* floats of cache-cell values per chunk in out_cells, and cache k's (in world->cache_ids order) number,
+ * candidate volumes, their largest size and its offset in a chunk's cell floats; 0 = ok, -1 = no such k */
+size_t mcgpu_terrain_cell_floats(void *h);
+int mcgpu_terrain_cell_info(void *h, int k, int *cid, int *ncand, size_t *cap, size_t *off);
+/* candidate j of cache k: vol = {sx, sy, sz, x, y0, z, dx, dy, dz}, x (z) relative to the chunk's block
+ * origin when *xrel (*zrel); *n = its size */
+int mcgpu_terrain_cell_cand(void *h, int k, int j, int vol[9], int *xrel, int *zrel, int *n);
+/* pinned host memory for batch outputs (hipHostMalloc, portable) */
+void *mcgpu_host_alloc(size_t bytes);
+void mcgpu_host_free(void *p);
+/* n <= max_chunks chunks (cx[i], cz[i]): out_dens = n x (16 * nh * 16) floats, chunk i's block (x, y, z) at
+ * i * 16*nh*16 + (y - nmin) + (x + z*16) * nh; out_veins (or NULL) = n x (nh * 256) patches at
+ * ((y - nmin)*16 + z)*16 + x, 0xFFFF = none; out_cells / out_which (or NULL) = per chunk the cache cells'
+ * values and, per cache, which candidate volume the CPU would hold after the density (-1: the cell is
+ * unset or as the pre-density volume left it, -2: unknown, sample the chunk on the CPU).  beard_cb (or
+ * NULL) fills Beardifier volumes.  0 = ok, 1 = bad arguments, 3 = device error (mcgpu_last_error). */
+int mcgpu_terrain_batch(void *h, int n, const int *cx, const int *cz, McgBeardFn beard_cb, void *beard_ud, float *out_dens,
+                        unsigned short *out_veins, float *out_cells, signed char *out_which);
+
 #ifdef __cplusplus
 }
 #endif

@@ -867,4 +867,16 @@ int mcgpu_terrain_batch(void *h, int n, const int *cx, const int *cz, McgBeardFn
     return terrain_batch(g, n, cx, cz, beard_cb, beard_ud, out_dens, out_veins, out_cells, out_which) ? 0 : 3;
 }
 
+// The density of n chunks left in device memory (no host copies, no veins or cells): *dens = chunk i's
+// 16 * nh * 16 floats at *dens + i * 16*nh*16, valid until the next batch on the handle.
+int mcb_terrain_batch_device(void *h, int n, const int *cx, const int *cz, const float **dens) {
+    PluginGate gate(true);
+    Terrain *g = (Terrain *)h;
+    if (!g || n < 1 || n > g->Bmax || !cx || !cz || !dens) return plugin_fail("bad batch (n must be 1..%d)", g ? g->Bmax : 0) ? 0 : 1;
+    if (!gate.ready) return 1;
+    if (!terrain_batch(g, n, cx, cz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)) return 3;
+    *dens = g->d_arena + g->vals[g->density].off;
+    return 0;
+}
+
 }  // extern "C"

@@ -20,6 +20,29 @@ VERSIONS = {"26.1": 0, "26.2": 1, "26.3": 2, "26.4-snapshot-2": 3}
 DIMS = {"overworld": 0, "the_nether": 1, "nether": 1, "the_end": 2, "end": 2}
 
 
+class PluginTerrainWorld:
+    """Owner of a McgTerrainWorld made by mcb_plugin_terrain_world; ``.world`` is the mcgpu.McgTerrainWorld."""
+
+    def __init__(self, engine, seed):
+        import mcgpu
+        p = C.c_void_p()
+        _check(lib().mcb_plugin_terrain_world(engine.h, int(seed), C.byref(p)))
+        self.addr = p.value
+        self.world = C.cast(p, C.POINTER(mcgpu.McgTerrainWorld)).contents
+        self._engine = engine
+
+    def close(self):
+        if getattr(self, "addr", None):
+            lib().mcb_plugin_terrain_world_free(self.addr)
+            self.addr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PluginWorld:
     """Owner of a McgBiomeWorld made by mcb_plugin_world."""
 
@@ -104,6 +127,10 @@ def lib():
         L.mcb_end_obs_radius.argtypes = [P, C.POINTER(C.c_uint64)]
         L.mcb_plugin_world.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
         L.mcb_plugin_world_free.argtypes = [C.c_void_p]
+        L.mcb_plugin_terrain_world.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_void_p)]
+        L.mcb_plugin_terrain_world_free.argtypes = [C.c_void_p]
+        L.mcb_terrain_batch_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               C.POINTER(C.c_void_p)]
         L.mcb_chunk_biomes.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_uint8)]
         L.mcb_end_field.argtypes = [P, i64p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.POINTER(C.c_double)]
@@ -378,6 +405,11 @@ class Engine:
         """This world for `seed` in libmcgen GPU-plugin form (a McgBiomeWorld, include/mcgpu.h): pass
         ``.addr`` to mcgpu_biome_new (mcgpu.BiomeWorld)."""
         return PluginWorld(self, seed)
+
+    def terrain_world(self, seed):
+        """The stand-in Overworld terrain world (mcb_plugin_terrain_world) of a 26.3+ Overworld engine for
+        `seed`: ``.world`` feeds mcgpu_terrain_new (mcgpu.TerrainWorld)."""
+        return PluginTerrainWorld(self, seed)
 
     def end_field(self, seeds, bx0, bz0, nx, nz, step=1, kind=1):
         """End island field: kind 0 height value at section (bx/8, bz/8), kind 1 erosion at the block;

@@ -240,6 +240,18 @@ class TerrainWorld:
             raise PluginError(last_error())
         return dens, vb, cb, wb
 
+    def batch_device(self, cx, cz):
+        """The density of the chunks left in device memory (mcb_terrain_batch_device): its device address,
+        chunk i's 16*nh*16 floats at address + 4 * i * 16*nh*16, valid until the next batch."""
+        cx = np.ascontiguousarray(cx, np.int32)
+        cz = np.ascontiguousarray(cz, np.int32)
+        p = C.c_void_p()
+        rc = mcbiome.lib().mcb_terrain_batch_device(self.h, len(cx), cx.ctypes.data_as(C.POINTER(C.c_int)),
+                                                    cz.ctypes.data_as(C.POINTER(C.c_int)), C.byref(p))
+        if rc:
+            raise PluginError(last_error())
+        return p.value
+
     def close(self):
         if getattr(self, "h", None):
             lib().mcgpu_terrain_free(self.h)

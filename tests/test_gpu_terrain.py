@@ -123,3 +123,24 @@ def test_refusals():
     dens, _, _, _ = G.batch([3], [-4])
     x = 48 + np.arange(16, dtype=np.float32)
     assert np.array_equal(dens[0].reshape(16, 16, W2.nh)[0, :, 0], np.clip(x, 0, 16) / 16)
+
+
+def test_engine_terrain_world_matches_restatement():
+    """The engine's stand-in Overworld terrain world (mcb_plugin_terrain_world): GPU batches against the
+    restatement on the same program; a device-resident batch in between leaves the next host batch equal."""
+    import mcbiome
+    eng = mcbiome.Engine("26.3", "overworld")
+    chunks = CHUNKS[:6]
+    cx, cz = [c[0] for c in chunks], [c[1] for c in chunks]
+    for seed in (0, -1, SEED):
+        print("terrain: engine world, seed %d" % seed, flush=True)
+        tw = eng.terrain_world(seed)
+        G = mcgpu.TerrainWorld(tw.world, max_chunks=16)
+        dens, _, _, _ = G.batch(cx, cz)
+        for i, (x, z) in enumerate(chunks):
+            od, _, _ = T.chunk(tw.world, x, z)
+            assert same_bits(dens[i], od), (seed, x, z)
+        assert G.batch_device(cx, cz)
+        assert same_bits(G.batch(cx, cz)[0], dens)
+        G.close()
+        tw.close()

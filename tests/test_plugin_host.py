@@ -17,7 +17,11 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 def test_exports_every_declared_symbol():
     txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
     names = sorted(set(re.findall(r"\b(mcgpu_[a-z_]+)\s*\(", txt)))
-    assert len(names) == 11
+    # the reference plugin's exports (MCGPU_EXPORT in libmcgen/gpu/mcgpu_core.cu, mcgpu_biome.cu, mcgpu_terrain.cu)
+    assert names == sorted("""mcgpu_abi mcgpu_biome_free mcgpu_biome_grid mcgpu_biome_new mcgpu_biome_points mcgpu_device_count
+        mcgpu_device_info mcgpu_host_alloc mcgpu_host_free mcgpu_last_error mcgpu_mem_info mcgpu_select mcgpu_selected
+        mcgpu_terrain_batch mcgpu_terrain_cell_cand mcgpu_terrain_cell_floats mcgpu_terrain_cell_info mcgpu_terrain_free
+        mcgpu_terrain_max_chunks mcgpu_terrain_new""".split())
     L = mcgpu.lib()
     assert not [n for n in names if not hasattr(L, n)]
     assert L.mcgpu_abi() == mcgpu.MCGPU_ABI == 2
@@ -32,6 +36,7 @@ def test_struct_sizes_match_reference_layout():
     assert C.sizeof(mcgpu.McgNode) == 112
     assert C.sizeof(mcgpu.McgTreeNode) == 72
     assert C.sizeof(mcgpu.McgBiomeWorld) == 80
+    assert C.sizeof(mcgpu.McgTerrainWorld) == 336 and mcgpu.McgTerrainWorld.ore_lo.offset == 144
     assert mcgpu.K["FTS"] == 56 and mcgpu.O["PASS"] == 28
 
 
