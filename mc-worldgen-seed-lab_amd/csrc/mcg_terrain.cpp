@@ -105,6 +105,7 @@ struct Pass {
     std::vector<TIns> ins;
     std::vector<int32_t> arr;
     int nslot = 0;
+    int feat = 0, spd = 1;
     size_t ins_off = 0, arr_off = 0;
 };
 struct CellInfo {
@@ -373,6 +374,21 @@ bool build_passes(Terrain &g) {
     return true;
 }
 
+// levels of spline calls from spline spi down to its deepest leaf (constants count as a level)
+int spline_nesting(const McgProg *p, int spi) {
+    int best = 0;
+    std::vector<std::pair<int, int>> st{{spi, 1}};
+    while (!st.empty()) {
+        const auto [i, d] = st.back();
+        st.pop_back();
+        best = std::max(best, d);
+        const McgSpline &q = p->sp[i];
+        if (q.is_const) continue;
+        for (int k = 0; k < q.n; k++) st.push_back({p->spc[q.val0 + k], d + 1});
+    }
+    return best;
+}
+
 // arena offsets for a batch of Bmax chunks, then each pass's instruction list
 bool emit_passes(Terrain &g) {
     size_t total = 0;
@@ -460,6 +476,12 @@ bool emit_passes(Terrain &g) {
             q.i0 = s.i0; q.i1 = s.i1; q.i2 = s.i2; q.i3 = s.i3; q.oct = s.oct;
             q.f0 = s.f0; q.f1 = s.f1; q.f2 = s.f2; q.f3 = s.f3; q.d0 = s.d0; q.d1 = s.d1;
             q.flag = x.flag;
+            if (s.k == MCGK_NOISE || s.k == MCGK_NOISE_XZ || s.k == MCGK_NOISE_XYZ || s.k == MCGK_SHIFT_B || s.k == MCGK_END)
+                P.feat |= TF_NOISE;
+            if (s.k == MCGK_SPLINE) {
+                P.feat |= TF_NOISE | TF_SPLINE;
+                P.spd = std::max(P.spd, spline_nesting(g.P, s.sp));
+            }
             auto sl = [&](int o) { return slot[def[o]]; };
             size_t o = 0;
             switch (s.k) {
@@ -555,6 +577,7 @@ Terrain *terrain_new(const McgTerrainWorld *W, int max_chunks) {
     if (!check_prog(W->vol)) return nullptr;
     g->T = *W;
     const McgProg *p = W->vol;
+    g->P = p;
     g->nodes.assign(p->nodes, p->nodes + p->nnodes);
     g->parr.assign(p->arr, p->arr + p->narr);
     for (int i = 0; i < p->nroots; i++) g->proots[i] = p->root[i];
@@ -670,6 +693,7 @@ Terrain *terrain_new(const McgTerrainWorld *W, int max_chunks) {
             g->gap_slot[k] = g->gapw.root_slot[gr];
         }
     }
+    g->P = nullptr;                         // the caller's program is not kept
     return g.release();
 }
 
@@ -704,7 +728,7 @@ bool terrain_batch(Terrain *g, int n, const int *cx, const int *cz, McgBeardFn b
     const TBatch B{g->d_cx0, g->d_cz0, n, g->d_arena, want_cells ? g->d_flags : nullptr, g->Bmax};
     for (const Pass &P : g->passes) {
         const TPass A{(const TIns *)(g->d_plan + P.ins_off), (const int32_t *)(g->d_plan + P.arr_off), (int32_t)P.ins.size(),
-                      P.nslot, P.v, P.v.size()};
+                      P.nslot, P.v, P.v.size(), P.feat, P.spd};
         if (!hchk(launch_tpass(g->prog, A, B, g->st), "pass launch")) return false;
     }
     const size_t nd = (size_t)256 * (size_t)W.nh;

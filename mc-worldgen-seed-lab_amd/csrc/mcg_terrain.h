@@ -64,6 +64,10 @@ struct TProg {
     const double *thr;
 };
 
+// node kinds a pass holds, beyond arithmetic, fetches and coordinates: noises (and End islands), splines
+enum { TF_NOISE = 1, TF_SPLINE = 2 };
+constexpr int kShallowSpline = 6;           // spline nesting of the lean spline kernel
+
 // one pass of one batch
 struct TPass {
     const TIns *ins;
@@ -71,6 +75,7 @@ struct TPass {
     int32_t nins, nslot;
     TVol v;
     int32_t n;                              // elements per chunk
+    int32_t feat, spd;                      // TF_* of its instructions, deepest spline nesting
 };
 struct TBatch {
     const int32_t *cx0, *cz0;               // block origin of each chunk

@@ -20,6 +20,17 @@
 namespace mcb {
 namespace plug {
 
+// Records every lane of a wave reads at the same address (the instruction, a noise's layer list, an
+// octave's offsets) are read through the constant address space, i.e. with scalar loads, once per wave.
+template <typename T>
+__device__ __forceinline__ T cload(const T *p) {
+#if __HIP_DEVICE_COMPILE__
+    return *(const __attribute__((address_space(4))) T *)p;
+#else
+    return *p;                                   // host pass: never executed
+#endif
+}
+
 __device__ __forceinline__ int tvx0(const TVol &v, int cx0) { return v.xrel ? cx0 + v.xoff : v.xoff; }
 __device__ __forceinline__ int tvz0(const TVol &v, int cz0) { return v.zrel ? cz0 + v.zoff : v.zoff; }
 
@@ -27,14 +38,15 @@ __device__ __forceinline__ int tvz0(const TVol &v, int cz0) { return v.zrel ? cz
 __device__ __forceinline__ float vol_octave(const McgOct *__restrict__ o, const GradEnt *G, int smeared, double fs, int bx,
                                             int by, int bz, double sxz, double sy) {
     const uint8_t *P = o->p;
-    const double z = wrap_new(bz * sxz) + o->zo;
+    const double xo = cload(&o->xo), yo = cload(&o->yo), zo = cload(&o->zo);
+    const double z = wrap_new(bz * sxz) + zo;
     const int32_t fz = jfloor(z);
     const float rz = (float)(z - fz), az = smooth_f(rz);
-    const double x = wrap_new(bx * sxz) + o->xo;
+    const double x = wrap_new(bx * sxz) + xo;
     const int32_t fx = jfloor(x);
     const float rx = (float)(x - fx), ax = smooth_f(rx);
     const double oy = by * sy;
-    const double y = wrap_new(oy) + o->yo;
+    const double y = wrap_new(oy) + yo;
     const int32_t fy = jfloor(y);
     const double ryd = y - fy;
     const float ay = smooth_f((float)ryd);
@@ -62,11 +74,11 @@ __device__ __forceinline__ float vol_octave(const McgOct *__restrict__ o, const 
 }
 // NoiseStack.addToVolume into a zero buffer element (noise.c:359-366)
 __device__ float vol_stack(const TProg &P, const GradEnt *G, int nsi, int bx, int by, int bz, double sxz, double sy, float amp) {
-    const McgNoise mn = P.noise[nsi];
+    const McgNoise mn = cload(P.noise + nsi);
     float buf = 0.0f;
 #pragma unroll 1
     for (int l = 0; l < mn.n0; l++) {
-        const McgLayer L = P.layers[mn.l0 + l];
+        const McgLayer L = cload(P.layers + mn.l0 + l);
         const double f = L.a;
         buf += (amp * L.amp_f) * vol_octave(P.octs + L.oct, G, L.kind, L.b, bx, by, bz, sxz * f, sy * f);
     }
@@ -74,14 +86,14 @@ __device__ float vol_stack(const TProg &P, const GradEnt *G, int nsi, int bx, in
 }
 // NoiseStack.get (noise.c:339-347), the point path of shifted noises
 __device__ float pt_stack(const TProg &P, const GradEnt *G, int nsi, double x, double y, double z) {
-    const McgNoise mn = P.noise[nsi];
+    const McgNoise mn = cload(P.noise + nsi);
     float v = 0.0f;
 #pragma unroll 1
     for (int i = 0; i < mn.n0; i++) {
-        const McgLayer L = P.layers[mn.l0 + i];
+        const McgLayer L = cload(P.layers + mn.l0 + i);
         const McgOct *o = P.octs + L.oct;
         const double f = L.a;
-        const Off3 off{o->xo, o->yo, o->zo};
+        const Off3 off{cload(&o->xo), cload(&o->yo), cload(&o->zo)};
         v += L.amp_f * octave_f<false>(o->p, G, &off, L.kind, L.b, x * f, y * f, z * f);
     }
     return v;
@@ -152,7 +164,10 @@ __device__ __forceinline__ void or_mask(uint32_t *flags, int c, bool active, uin
     }
 }
 
-template <int SPD>
+// FEAT (TF_* of mcg_terrain.h) selects the node kinds a pass holds: a pass without noises or splines (the
+// block-level interpolation and arithmetic) compiles to a small kernel with few registers; SPD bounds the
+// spline nesting.
+template <int FEAT, int SPD>
 __global__ __launch_bounds__(kTWG) void k_tpass(TProg P, TPass A, TBatch B) {
     extern __shared__ __align__(16) uint8_t lds[];
     GradEnt *G = (GradEnt *)lds;
@@ -172,21 +187,34 @@ __global__ __launch_bounds__(kTWG) void k_tpass(TProg P, TPass A, TBatch B) {
 #define V(x) vals[(x) * kTWG]
 #pragma unroll 1
     for (int i = 0; i < A.nins; i++) {
-        const TIns s = A.ins[i];
+        const TIns s = cload(A.ins + i);
         float r;
         switch (s.op) {
         case MCGK_CONST: case MCGK_CTX_ALPHA: case MCGK_CTX_OFFSET: r = s.f0; break;
-        case MCGK_NOISE: r = vol_stack(P, G, s.ns, bx, by, bz, s.d0, s.d1, 1.0f); break;
-        case MCGK_SHIFT_B: r = vol_stack(P, G, s.ns, bz, bx, 0, s.d0, s.d0, 4.0f); break;   // transposed volume
+        case MCGK_NOISE:
+            if constexpr ((FEAT & TF_NOISE) != 0) r = vol_stack(P, G, s.ns, bx, by, bz, s.d0, s.d1, 1.0f);
+            else r = __builtin_nanf("");
+            break;
+        case MCGK_SHIFT_B:                                                             // transposed volume
+            if constexpr ((FEAT & TF_NOISE) != 0) r = vol_stack(P, G, s.ns, bz, bx, 0, s.d0, s.d0, 4.0f);
+            else r = __builtin_nanf("");
+            break;
         case MCGK_NOISE_XZ: case MCGK_NOISE_XYZ: {
-            const double nx = bx * s.d0 + (double)V(s.a);
-            double ny = by * s.d1;
-            if (s.op == MCGK_NOISE_XYZ) ny += (double)V(s.b);
-            const double nz = bz * s.d0 + (double)V(s.c);
-            r = pt_stack(P, G, s.ns, nx, ny, nz);
+            if constexpr ((FEAT & TF_NOISE) != 0) {
+                const double nx = bx * s.d0 + (double)V(s.a);
+                double ny = by * s.d1;
+                if (s.op == MCGK_NOISE_XYZ) ny += (double)V(s.b);
+                const double nz = bz * s.d0 + (double)V(s.c);
+                r = pt_stack(P, G, s.ns, nx, ny, nz);
+            } else {
+                r = __builtin_nanf("");
+            }
             break;
         }
-        case MCGK_END: r = (end_height_j<false>(P.octs[s.oct].p, bx / 8, bz / 8) - 8.0f) / 128.0f; break;
+        case MCGK_END:
+            if constexpr ((FEAT & TF_NOISE) != 0) r = (end_height_j<false>(P.octs[s.oct].p, bx / 8, bz / 8) - 8.0f) / 128.0f;
+            else r = __builtin_nanf("");
+            break;
         case MCGK_DIST: r = dist_value(s.i3, (float)(s.i0 - bx), (float)(s.i1 - by), (float)(s.i2 - bz)); break;
         case MCGK_GRAD_CLAMP: case MCGK_GRAD_REPEAT: case MCGK_GRAD_MIRROR:
             r = tgrad(s, s.i0 == 0 ? bx : (s.i0 == 1 ? by : bz));
@@ -232,9 +260,13 @@ __global__ __launch_bounds__(kTWG) void k_tpass(TProg P, TPass A, TBatch B) {
             break;
         }
         case MCGK_SPLINE: {
-            uint32_t used = 0;
-            r = tspline<SPD>(P, s.sp, vals, A.arr + s.arr0, used);
-            if (B.flags && s.flag >= 0) or_mask(B.flags + (int64_t)s.flag * B.flag_stride, c, active, used);
+            if constexpr ((FEAT & TF_SPLINE) != 0) {
+                uint32_t used = 0;
+                r = tspline<SPD>(P, s.sp, vals, A.arr + s.arr0, used);
+                if (B.flags && s.flag >= 0) or_mask(B.flags + (int64_t)s.flag * B.flag_stride, c, active, used);
+            } else {
+                r = __builtin_nanf("");
+            }
             break;
         }
         case TOP_LOAD: r = active ? B.arena[s.src + (int64_t)c * s.g0 + e] : 0.0f; break;
@@ -293,7 +325,15 @@ hipError_t launch_tpass(const TProg &P, const TPass &A, const TBatch &B, hipStre
     const int64_t items = (int64_t)A.n * B.nchunks;
     if (items <= 0) return hipSuccess;
     const int lds = 256 * (int)sizeof(GradEnt) + A.nslot * kTWG * 4;
-    hipLaunchKernelGGL(k_tpass<kSplineDepth>, dim3((unsigned)((items + kTWG - 1) / kTWG)), dim3(kTWG), lds, st, P, A, B);
+    const dim3 grid((unsigned)((items + kTWG - 1) / kTWG));
+    if (!(A.feat & TF_SPLINE)) {
+        if (A.feat & TF_NOISE) hipLaunchKernelGGL((k_tpass<TF_NOISE, 1>), grid, dim3(kTWG), lds, st, P, A, B);
+        else hipLaunchKernelGGL((k_tpass<0, 1>), grid, dim3(kTWG), lds, st, P, A, B);
+    } else if (A.spd <= kShallowSpline) {
+        hipLaunchKernelGGL((k_tpass<TF_NOISE | TF_SPLINE, kShallowSpline>), grid, dim3(kTWG), lds, st, P, A, B);
+    } else {
+        hipLaunchKernelGGL((k_tpass<TF_NOISE | TF_SPLINE, kSplineDepth>), grid, dim3(kTWG), lds, st, P, A, B);
+    }
     return hipGetLastError();
 }
 
