@@ -463,6 +463,9 @@ bool emit_passes(Terrain &g) {
                     q.op = TOP_INTERP;
                     q.i0 = s.i0; q.i1 = s.i1; q.f0 = s.f0; q.f1 = s.f1;
                     q.g1 = u.v.sx; q.g2 = u.v.sy; q.g3 = u.v.sz;
+                    auto log2_of = [](int v) { return (v > 0 && (v & (v - 1)) == 0) ? __builtin_ctz((unsigned)v) : -1; };
+                    const int lx = log2_of(s.i0), ly = log2_of(s.i1);
+                    q.g4 = (lx >= 0 && ly >= 0) ? (lx | ly << 8) : -1;
                 } else {
                     q.op = TOP_SLICE;
                     q.g1 = x.axis;
@@ -649,6 +652,10 @@ Terrain *terrain_new(const McgTerrainWorld *W, int max_chunks) {
     if (!hchk(hipMemGetInfo(&fr, &tot), "hipMemGetInfo")) return nullptr;
     const size_t budget = std::min<size_t>(fr / 4, (size_t)16 << 30);
     g->Bmax = (int)std::max<size_t>(1, std::min<size_t>((size_t)max_chunks, budget / chunk_bytes));
+    // the kernels index a pass's (chunk, element) pairs in 32 bits
+    size_t largest = nd;
+    for (const Pass &P : g->passes) largest = std::max<size_t>(largest, (size_t)P.v.size());
+    g->Bmax = (int)std::min<size_t>((size_t)g->Bmax, std::max<size_t>(1, ((size_t)1 << 31) / largest - 1));
     if (!emit_passes(*g)) { plugin_fail("execution plan: %s", g->fail.c_str()); return nullptr; }
     // device: program tables, pass instructions, arena, chunk origins, masks, vein patches, staging
     struct Part { const void *src; size_t bytes; size_t off; };

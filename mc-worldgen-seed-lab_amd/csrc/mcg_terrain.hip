@@ -174,13 +174,15 @@ __global__ __launch_bounds__(kTWG) void k_tpass(TProg P, TPass A, TBatch B) {
     dev::init_grad(G);
     __syncthreads();
     float *vals = (float *)(lds + 256 * sizeof(GradEnt)) + threadIdx.x;
-    const int64_t gid = (int64_t)blockIdx.x * kTWG + threadIdx.x;
+    // the host keeps n * nchunks below 2^31: 32-bit index arithmetic
+    const uint32_t gid = blockIdx.x * kTWG + threadIdx.x;
     const int n = A.n;
-    const bool active = gid < (int64_t)n * B.nchunks;
-    const int c = active ? (int)(gid / n) : 0;
-    const int e = active ? (int)(gid - (int64_t)c * n) : 0;
+    const bool active = gid < (uint32_t)n * (uint32_t)B.nchunks;
+    const int c = active ? (int)(gid / (uint32_t)n) : 0;
+    const int e = active ? (int)(gid - (uint32_t)c * (uint32_t)n) : 0;
     const TVol v = A.v;
-    const int iy = e % v.sy, t = e / v.sy, ix = t % v.sx, iz = t / v.sx;
+    const int t = (int)((uint32_t)e / (uint32_t)v.sy), iy = e - t * v.sy;
+    const int iz = (int)((uint32_t)t / (uint32_t)v.sx), ix = t - iz * v.sx;
     const int cx0 = B.cx0[c], cz0 = B.cz0[c];
     const int ox0 = tvx0(v, cx0), oz0 = tvz0(v, cz0);
     const int bx = ox0 + ix * v.dx, by = v.y0 + iy * v.dy, bz = oz0 + iz * v.dz;
@@ -274,8 +276,15 @@ __global__ __launch_bounds__(kTWG) void k_tpass(TProg P, TPass A, TBatch B) {
             // cell-step interpolation (df_new.c:867-915) at one element of a unit-step volume; the corner
             // volume (g1 x g2 x g3, step cxz = i0, cy = i1) sits at src, g0 floats per chunk
             const int cxz = s.i0, cy = s.i1;
-            const int mnx = jfloordiv(ox0, cxz), mny = jfloordiv(v.y0, cy), mnz = jfloordiv(oz0, cxz);
-            const int cellx = jfloordiv(bx, cxz) - mnx, celly = jfloordiv(by, cy) - mny, cellz = jfloordiv(bz, cxz) - mnz;
+            int mnx, mny, mnz, cellx, celly, cellz;
+            if (s.g4 >= 0) {                            // power-of-two cells: floor division by shifts
+                const int lx = s.g4 & 255, ly = s.g4 >> 8;
+                mnx = ox0 >> lx; mny = v.y0 >> ly; mnz = oz0 >> lx;
+                cellx = (bx >> lx) - mnx; celly = (by >> ly) - mny; cellz = (bz >> lx) - mnz;
+            } else {
+                mnx = jfloordiv(ox0, cxz); mny = jfloordiv(v.y0, cy); mnz = jfloordiv(oz0, cxz);
+                cellx = jfloordiv(bx, cxz) - mnx; celly = jfloordiv(by, cy) - mny; cellz = jfloordiv(bz, cxz) - mnz;
+            }
             const int cox = (mnx + cellx) * cxz - ox0, coy = (mny + celly) * cy - v.y0, coz = (mnz + cellz) * cxz - oz0;
             const int xx = ix - cox, yy = iy - coy, zz = iz - coz;
             const int ystart = coy < 0 ? -coy : 0;
