@@ -106,6 +106,8 @@ struct Pass {
     std::vector<int32_t> arr;
     int nslot = 0;
     int feat = 0, spd = 1;
+    bool fused = false;           // run as k_tinterp
+    TInterp fi{};
     size_t ins_off = 0, arr_off = 0;
 };
 struct CellInfo {
@@ -374,6 +376,45 @@ bool build_passes(Terrain &g) {
     return true;
 }
 
+bool is_unary(int k) {
+    switch (k) {
+    case MCGK_ABS: case MCGK_SQUARE: case MCGK_CUBE: case MCGK_SQRT: case MCGK_LEAKY: case MCGK_RECIP: case MCGK_NEG:
+    case MCGK_SQUEEZE: case MCGK_SIGN: case MCGK_CADD: case MCGK_CSUB: case MCGK_CMUL: case MCGK_CDIV: case MCGK_CMIN:
+    case MCGK_CMAX: case MCGK_CLAMP: case MCGK_RANGE_C:
+        return true;
+    default:
+        return false;
+    }
+}
+
+// A pass of one interpolation over a cell-aligned unit-step volume and a chain of unary nodes, with only
+// the chain's end written out, runs as k_tinterp (mcg_terrain.h).
+void fuse_interp(Pass &P) {
+    const std::vector<TIns> &I = P.ins;
+    if (I.empty() || I[0].op != TOP_INTERP || (int)I.size() - 1 > kMaxTail || I[0].g4 < 0) return;
+    const TVol &v = P.v;
+    const int cxz = I[0].i0, cy = I[0].i1;
+    if (v.dx != 1 || v.dy != 1 || v.dz != 1 || fl_mod(v.xoff, cxz) || fl_mod(v.zoff, cxz) || fl_mod(v.y0, cy) ||
+        v.sx % cxz || v.sz % cxz || v.sy % cy || (!v.xrel && fl_mod(v.xoff, cxz)) || (!v.zrel && fl_mod(v.zoff, cxz)))
+        return;
+    for (size_t k = 1; k < I.size(); k++)
+        if (!is_unary(I[k].op) || I[k].a != I[k - 1].dst || I[k - 1].store >= 0) return;
+    if (I.back().store < 0) return;
+    TInterp &f = P.fi;
+    f.src = I[0].src;
+    f.store = I.back().store;
+    f.ncv = I[0].g0;
+    f.cvx = I[0].g1; f.cvy = I[0].g2; f.cvz = I[0].g3;
+    f.lx = I[0].g4 & 255; f.ly = I[0].g4 >> 8;
+    f.f0 = I[0].f0; f.f1 = I[0].f1;
+    f.ntail = (int)I.size() - 1;
+    for (int k = 0; k < f.ntail; k++) {
+        const TIns &q = I[k + 1];
+        f.op[k] = q.op; f.k0[k] = q.f0; f.k1[k] = q.f1; f.k2[k] = q.f2; f.k3[k] = q.f3;
+    }
+    P.fused = true;
+}
+
 // levels of spline calls from spline spi down to its deepest leaf (constants count as a level)
 int spline_nesting(const McgProg *p, int spi) {
     int best = 0;
@@ -507,6 +548,7 @@ bool emit_passes(Terrain &g) {
             }
             P.ins.push_back(q);
         }
+        fuse_interp(P);
     }
     return true;
 }
@@ -736,7 +778,8 @@ bool terrain_batch(Terrain *g, int n, const int *cx, const int *cz, McgBeardFn b
     for (const Pass &P : g->passes) {
         const TPass A{(const TIns *)(g->d_plan + P.ins_off), (const int32_t *)(g->d_plan + P.arr_off), (int32_t)P.ins.size(),
                       P.nslot, P.v, P.v.size(), P.feat, P.spd};
-        if (!hchk(launch_tpass(g->prog, A, B, g->st), "pass launch")) return false;
+        const hipError_t e = P.fused ? launch_tinterp(P.fi, P.v, B, g->st) : launch_tpass(g->prog, A, B, g->st);
+        if (!hchk(e, "pass launch")) return false;
     }
     const size_t nd = (size_t)256 * (size_t)W.nh;
     const int nveins = W.veins_on ? W.nveins : 0;

@@ -32,7 +32,7 @@ struct TVol {
         return sx == o.sx && sy == o.sy && sz == o.sz && dx == o.dx && dy == o.dy && dz == o.dz && y0 == o.y0 &&
                xrel == o.xrel && xoff == o.xoff && zrel == o.zrel && zoff == o.zoff;
     }
-    int32_t size() const { return sx * sy * sz; }
+    __host__ __device__ int32_t size() const { return sx * sy * sz; }
 };
 
 // instruction kinds beyond the node kinds (MCGK_*): reads of values another pass wrote
@@ -102,6 +102,22 @@ struct TGather {
 };
 
 hipError_t launch_tpass(const TProg &P, const TPass &A, const TBatch &B, hipStream_t st);
+
+// A pass that is one interpolation over a cell-aligned unit-step volume followed by element-wise unary
+// nodes (the usual tail of a final_density): each lane takes one run of cy blocks up a cell column, reads
+// the cell's eight corners once and walks the run with the CPU's running y sum (df_new.c:867-888),
+// applying the tail to each block.  The values equal the general pass's, operation for operation.
+constexpr int kMaxTail = 8;
+struct TInterp {
+    int64_t src, store;                     // corner buffer, output buffer (arena offsets)
+    int32_t ncv, cvx, cvy, cvz;             // corner floats per chunk, corner volume size
+    int32_t lx, ly;                         // log2 of the cell size (x and z, y)
+    float f0, f1;                           // 1 / cell size (x and z, y)
+    int32_t ntail;
+    int32_t op[kMaxTail];
+    float k0[kMaxTail], k1[kMaxTail], k2[kMaxTail], k3[kMaxTail];
+};
+hipError_t launch_tinterp(const TInterp &I, const TVol &v, const TBatch &B, hipStream_t st);
 // gap program: a point world (PWorld with the program's instructions) evaluated per block where needed
 hipError_t launch_tveins(const PWorld &G, int lds, const TVeins &V, const TBatch &B, uint16_t *out, hipStream_t st);
 hipError_t launch_tgather(const TGather *items, int nitems, const float *arena, float *dst, hipStream_t st);

@@ -346,6 +346,70 @@ hipError_t launch_tpass(const TProg &P, const TPass &A, const TBatch &B, hipStre
     return hipGetLastError();
 }
 
+// element-wise unary node of the volume path (df_new.c:979-1033)
+__device__ __forceinline__ float tail_op(int op, float q, float f0, float f1, float f2, float f3) {
+    switch (op) {
+    case MCGK_ABS: return fabsf(q);
+    case MCGK_SQUARE: return q * q;
+    case MCGK_CUBE: return q * q * q;
+    case MCGK_SQRT: return (float)sqrt((double)q);
+    case MCGK_LEAKY: return q > 0.0f ? q : q * f0;
+    case MCGK_RECIP: return 1.0f / q;
+    case MCGK_NEG: return -q;
+    case MCGK_SQUEEZE: return squeeze_f(q);
+    case MCGK_SIGN: return jsignf(q);
+    case MCGK_CADD: return q + f0;
+    case MCGK_CSUB: return f0 - q;
+    case MCGK_CMUL: return q * f0;
+    case MCGK_CDIV: return f0 / q;
+    case MCGK_CMIN: return f0 < q ? f0 : q;
+    case MCGK_CMAX: return f0 > q ? f0 : q;
+    case MCGK_CLAMP: return jclampf(q, f0, f1);
+    default: return (q >= f0 && q < f1) ? f2 : f3;     // MCGK_RANGE_C
+    }
+}
+
+__global__ __launch_bounds__(kTWG) void k_tinterp(TInterp I, TVol v, TBatch B) {
+    const int cy = 1 << I.ly, ny = v.sy >> I.ly;
+    const uint32_t per = (uint32_t)(v.sx * v.sz * ny);
+    const uint32_t gid = blockIdx.x * kTWG + threadIdx.x;
+    if (gid >= per * (uint32_t)B.nchunks) return;      // no barrier in this kernel
+    const int c = (int)(gid / per), r = (int)(gid - (uint32_t)c * per);
+    const int run = r % ny, col = r / ny, ix = col % v.sx, iz = col / v.sx;
+    const int cellx = ix >> I.lx, cellz = iz >> I.lx, celly = run;
+    const int xx = ix - (cellx << I.lx), zz = iz - (cellz << I.lx);
+    const int nx2 = cellx + 1 < I.cvx - 1 ? cellx + 1 : I.cvx - 1;
+    const int ny2 = celly + 1 < I.cvy - 1 ? celly + 1 : I.cvy - 1;
+    const int nz2 = cellz + 1 < I.cvz - 1 ? cellz + 1 : I.cvz - 1;
+    const float *cb = B.arena + I.src + (int64_t)c * I.ncv;
+    auto at = [&](int x, int y, int z) { return cb[y + (x + z * I.cvx) * I.cvy]; };
+    const float v000 = at(cellx, celly, cellz), v100 = at(nx2, celly, cellz);
+    const float v001 = at(cellx, celly, nz2), v101 = at(nx2, celly, nz2);
+    const float v010 = at(cellx, ny2, cellz), v110 = at(nx2, ny2, cellz);
+    const float v011 = at(cellx, ny2, nz2), v111 = at(nx2, ny2, nz2);
+    const float az = (float)zz * I.f0;
+    const float v00 = jlerpf(az, v000, v001), v01 = jlerpf(az, v010, v011);
+    const float v10 = jlerpf(az, v100, v101), v11 = jlerpf(az, v110, v111);
+    const float ax = (float)xx * I.f0;
+    const float a0 = jlerpf(ax, v00, v10), a1 = jlerpf(ax, v01, v11);
+    const float step = (a1 - a0) * I.f1;
+    float value = a0 + step * 0.0f;
+    float *out = B.arena + I.store + (int64_t)c * v.size() + (int64_t)(ix + iz * v.sx) * v.sy + (int64_t)run * cy;
+    for (int k = 0; k < cy; k++) {
+        float q = value;
+        for (int t = 0; t < I.ntail; t++) q = tail_op(I.op[t], q, I.k0[t], I.k1[t], I.k2[t], I.k3[t]);
+        out[k] = q;
+        value += step;
+    }
+}
+
+hipError_t launch_tinterp(const TInterp &I, const TVol &v, const TBatch &B, hipStream_t st) {
+    const int64_t items = (int64_t)v.sx * v.sz * (v.sy >> I.ly) * B.nchunks;
+    if (items <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tinterp, dim3((unsigned)((items + kTWG - 1) / kTWG)), dim3(kTWG), 0, st, I, v, B);
+    return hipGetLastError();
+}
+
 // Mth.getSeed(x, y, z) (engine/mc_rng.h:124-128) and the positional Xoroshiro128++ floats
 __device__ __forceinline__ int64_t get_seed_xyz(int32_t x, int32_t y, int32_t z) {
     int64_t l = (int64_t)(int32_t)((uint32_t)x * 3129871u) ^ ((int64_t)z * 116129781LL) ^ (int64_t)y;
