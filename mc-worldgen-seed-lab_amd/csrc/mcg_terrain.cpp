@@ -394,7 +394,7 @@ void fuse_interp(Pass &P) {
     if (I.empty() || I[0].op != TOP_INTERP || (int)I.size() - 1 > kMaxTail || I[0].g4 < 0) return;
     const TVol &v = P.v;
     const int cxz = I[0].i0, cy = I[0].i1;
-    if (v.dx != 1 || v.dy != 1 || v.dz != 1 || fl_mod(v.xoff, cxz) || fl_mod(v.zoff, cxz) || fl_mod(v.y0, cy) ||
+    if (cy % 4 || v.dx != 1 || v.dy != 1 || v.dz != 1 || fl_mod(v.xoff, cxz) || fl_mod(v.zoff, cxz) || fl_mod(v.y0, cy) ||
         v.sx % cxz || v.sz % cxz || v.sy % cy || (!v.xrel && fl_mod(v.xoff, cxz)) || (!v.zrel && fl_mod(v.zoff, cxz)))
         return;
     for (size_t k = 1; k < I.size(); k++)

@@ -395,11 +395,18 @@ __global__ __launch_bounds__(kTWG) void k_tinterp(TInterp I, TVol v, TBatch B) {
     const float step = (a1 - a0) * I.f1;
     float value = a0 + step * 0.0f;
     float *out = B.arena + I.store + (int64_t)c * v.size() + (int64_t)(ix + iz * v.sx) * v.sy + (int64_t)run * cy;
-    for (int k = 0; k < cy; k++) {
-        float q = value;
-        for (int t = 0; t < I.ntail; t++) q = tail_op(I.op[t], q, I.k0[t], I.k1[t], I.k2[t], I.k3[t]);
-        out[k] = q;
-        value += step;
+    // runs of four blocks leave as one 16-byte store (the run starts 16-byte aligned: arena offsets are
+    // multiples of 64 floats and cy, sy of 4)
+    for (int k = 0; k < cy; k += 4) {
+        float q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            float w = value;
+            for (int t = 0; t < I.ntail; t++) w = tail_op(I.op[t], w, I.k0[t], I.k1[t], I.k2[t], I.k3[t]);
+            q[u] = w;
+            value += step;
+        }
+        *reinterpret_cast<float4 *>(out + k) = make_float4(q[0], q[1], q[2], q[3]);
     }
 }
 
