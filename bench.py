@@ -143,7 +143,9 @@ def run_serving(mcbiome, rate, n, lanes=4, batch_mb=8):
     srv = mcbiome.Server(ctx, batch_bytes=batch_mb << 20, lanes=lanes, slot_capacity=16384)
     reqs = serve_requests(n, rate)
     nbuf = 512
-    bufs = [np.empty(512 * 512 * 4, np.uint8) for _ in range(nbuf)]
+    # the client's result buffers, reused round robin and touched once up front (as a long-running client's
+    # are), so first-touch page faults of fresh pages do not land in the measured copies
+    bufs = [np.ones(512 * 512 * 4, np.uint8) for _ in range(nbuf)]
     warm = mcbiome.make_region(reqs[0][1], reqs[0][2], reqs[0][3], 512, 512, (-32, 64, 128, 256), True)
     warm.world = w
     srv.wait(srv.submit(warm, bufs[0]))
@@ -163,6 +165,7 @@ def run_serving(mcbiome, rate, n, lanes=4, batch_mb=8):
         srv.wait(t)
     dt = time.perf_counter() - t0
     st = srv.stats()
+    st["slowest_ms"] = srv.slowest(5)
     srv.close()
     ctx.close()
     st.update({"rate_req_s": rate, "requests": n, "achieved_req_s": round(n / dt, 1),
@@ -313,7 +316,16 @@ def bench_batched(args, torch, mcbiome):
             e.region(s, qx0, qz0, SIDE, SIDE, QYS)
         per_call = npc * SIDE * SIDE * len(QYS) / (time.perf_counter() - t1)
 
-    lat = run_serving(mcbiome, args.serve_rate, args.serve_requests) if args.serve_requests > 0 else None
+    # serving at each of the configured Poisson rates (512x512-block tiles x 4 y, Zipf seeds), and the uneven
+    # mixed batch (batched vs per-call), measured in the same run
+    rates = [float(r) for r in str(args.serve_rates).split(",") if r]
+    lat = [run_serving(mcbiome, r, args.serve_requests) for r in rates] if args.serve_requests > 0 else None
+    mixed = None
+    if args.mixed_requests > 0:
+        m = bench_mixed(args, torch, mcbiome)
+        mixed = {"requests": args.mixed_requests, "samples_per_s": m["value"], "ms_per_step": m["ms_per_step"],
+                 "per_call_samples_per_s": m["per_call_samples_per_s"], "batched_over_per_call": m["batched_over_per_call"],
+                 "samples": m["config"]["samples"]}
     cpu = None if args.no_cpu else cpu_baseline(work)
     return {
         "metric": "bit-exact biome samples/s batched vs per-call, 1 MI355X; p50/p99 tile latency",
@@ -330,6 +342,7 @@ def bench_batched(args, torch, mcbiome):
         "mismatched_seeds": mismatched,
         "per_version_kernel_samples_per_s": {v: round(kern[v]["samples_per_s"], 1) for v in VERSIONS},
         "tile_latency_ms": lat,
+        "mixed_batch": mixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -829,7 +842,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--percall-seeds", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=1000.0)
-    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-requests", type=int, default=5000, help="requests per serving rate")
+    ap.add_argument("--serve-rates", default="200,1000,5000", help="Poisson arrival rates (req/s) of the serving leg")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--mixed-requests", type=int, default=10000)
     ap.add_argument("--terrain-seeds", type=int, default=64)

@@ -254,6 +254,11 @@ int mcb_server_wait(McbServer *s, int64_t ticket, double *latency_ms);
 int mcb_server_drain(McbServer *s);
 int mcb_server_stats(McbServer *s, double *p50_ms, double *p99_ms, double *p999_ms, int64_t *completed,
                      double *slot_hit_rate, int64_t *batches);
+/* The n slowest requests since the last reset, slowest first: out[5i..5i+4] = total latency, then its
+ * phases: queued (submit -> admitted to a batch), launch (admitted -> the batch's work enqueued), device
+ * (enqueued -> the batch's results in pinned memory), copy (-> in the caller's buffer), all in ms.  Returns
+ * how many were written (the rest of out is zero). */
+int mcb_server_slowest(McbServer *s, int n, double *out);
 int mcb_server_reset_stats(McbServer *s);
 void mcb_server_destroy(McbServer *s);
 

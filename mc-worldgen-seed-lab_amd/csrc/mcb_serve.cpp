@@ -1,18 +1,20 @@
 // mcb_serve.cpp - serving loop: continuous batching of region requests on one device.
 //
-// Requests (McbRegion of any world of a context) are queued by mcb_server_submit.  A dispatcher thread
-// forms a batch whenever one of `lanes` in-flight batch slots is free: it admits queued requests in
+// Requests (McbRegion of any world of a context) are queued by mcb_server_submit.  Each of `lanes`
+// in-flight batch slots has a thread that forms a batch whenever its lane is free: it admits queued requests in
 // arrival order while their output and zoom-staging bytes fit the lane's device arena (memory-based
 // admission), looks their seeds up in a per-world device LRU of seed-state slots (only misses are
 // initialised), lowers the batch into kernel descriptors and launches it on the lane's stream
 // (seed state for misses, persistent quart-grid launch, zoom pass, copy-out).  Batches on different
-// lanes overlap on the device; each lane has its own completion thread, which copies the lane's
-// results to the callers' buffers and records per-request latency (submit -> result in the caller's
-// host buffer), so the host copies of different lanes run in parallel.
+// lanes overlap on the device; the lane's thread then waits for its batch and hands each
+// request's copy-out to a pool of copy threads, which move the bytes to the callers' buffers and record
+// per-request latency (submit -> result in the caller's host buffer) as each request's bytes land, so a
+// request does not wait for the copies of the requests batched before it.
 //
 // The nearest precedent in the reference is the pinned-slot terrain producer of
 // libmcgen/src/gpu_bridge.c:605-760; the reference has no request server for biomes.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -33,13 +35,18 @@ extern "C" int mcb_context_world_engine(McbContext *c, int world, McbEngine **e)
 
 namespace {
 using Clock = std::chrono::steady_clock;
+constexpr int kCopyThreads = 8;           // threads moving results from the lanes' pinned buffers to callers
 
 struct Req {
     McbRegion r;
     uint8_t *host_out;
     int64_t ticket;
-    Clock::time_point t_submit;
+    Clock::time_point t_submit, t_admit;
     int64_t out_bytes, stage_bytes;
+    // set at admission: the world's slot pool, the seed's pinned slot, whether this batch initialises it
+    struct SlotPool *pool = nullptr;
+    int slot = -1;
+    bool fresh = false;
 };
 
 // Device LRU of seed-state slots for one world.
@@ -123,9 +130,17 @@ struct Lane {
     std::vector<Req> reqs;
     std::vector<std::pair<int, int>> pinned;              // (world, slot)
     std::vector<std::pair<int, int>> fresh;               // (world, slot) acquired unmapped by this batch
+    std::vector<std::pair<int, int64_t>> wait_lanes;      // (lane, batch) whose seed-state kernels this batch reads
+    int64_t inited_batch = 0;                             // last batch whose seed-state event is recorded
     bool busy = false;
-    bool launched = false;                                // batch on the device, results not yet copied
-    std::thread completer;
+    Clock::time_point t_launched;                         // launch_batch returned
+    Clock::time_point t_done;                             // the batch's results reached pinned memory
+    int copies_left = 0;                                  // requests of the batch not yet copied out
+    std::thread worker;
+};
+struct CopyJob {
+    int lane;
+    size_t req;
 };
 }  // namespace
 
@@ -137,15 +152,17 @@ struct McbServer {
     std::unordered_map<int, SlotPool> pools;
     int slot_capacity = 0;
     std::mutex mu;
-    std::condition_variable cv_submit, cv_done, cv_lane;
+    std::condition_variable cv_submit, cv_done, cv_copy, cv_copied, cv_inited;
+    std::deque<CopyJob> copies;
+    std::vector<std::thread> copiers;
     std::deque<Req> pending;
     int inflight = 0;                                      // lanes with a launched batch
     int64_t completions = 0;                               // batches finished (slots unpinned)
     std::unordered_map<int64_t, std::pair<int, double>> results;   // ticket -> (status, latency ms)
     std::vector<double> latencies;
+    std::vector<std::array<double, 4>> phases;             // per latency: queue, launch, device, copy (ms)
     int64_t next_ticket = 1;
     bool stop = false;
-    std::thread dispatcher;
     std::string fatal;
     int64_t batches = 0, admitted = 0;
 };
@@ -179,37 +196,17 @@ int launch_batch(McbServer *S, Lane &L) {
         Seg seeds, slots, g, gt, q, qt, z, zt;
     };
     std::vector<WorldBatch> wb(worlds.size());
-    std::vector<int> wait_lanes;
-    const int my_lane = (int)(&L - &S->lanes[0]);
     for (size_t wi = 0; wi < worlds.size(); wi++) {
         WorldBatch &B = wb[wi];
         B.world = worlds[wi];
-        std::unique_lock<std::mutex> lk(S->mu);
-        B.pool = &S->pools[B.world];
-        if (!B.pool->e) {
-            McbEngine *e = nullptr;
-            if (int rc = mcb_context_world_engine(S->ctx, B.world, &e)) return rc;
-            if (int rc = B.pool->init(e, S->slot_capacity)) return rc;
-        }
         for (size_t i = 0; i < L.reqs.size(); i++) {
             Req &q = L.reqs[i];
             if (q.r.world != B.world) continue;
-            int fresh = 0;
-            int slot = B.pool->acquire(q.r.seed, &fresh);
-            if (slot < 0) return fail(MCB_E_LIMIT, "seed slot pool exhausted");
-            L.pinned.push_back({B.world, slot});
-            if (fresh) {
-                L.fresh.push_back({B.world, slot});
+            B.pool = q.pool;
+            const int slot = q.slot;
+            if (q.fresh) {
                 B.init_seeds.push_back(q.r.seed);
                 B.init_slots.push_back(slot);
-                B.pool->init_lane[slot] = my_lane;
-                B.pool->init_batch[slot] = L.batch_id;
-            } else {
-                // state written by a batch still in flight on another lane: order after its init kernels
-                const int il = B.pool->init_lane[slot];
-                if (il >= 0 && il != my_lane && S->lanes[il].busy && S->lanes[il].batch_id == B.pool->init_batch[slot] &&
-                    std::find(wait_lanes.begin(), wait_lanes.end(), il) == wait_lanes.end())
-                    wait_lanes.push_back(il);
             }
             McbRegion r = q.r;
             r.out_off = out_off;
@@ -230,7 +227,13 @@ int launch_batch(McbServer *S, Lane &L) {
             B.L.z.insert(B.L.z.end(), one.z.begin(), one.z.end());
         }
     }
-    for (int il : wait_lanes) MCB_HIPCHK(hipStreamWaitEvent(L.st, S->lanes[il].inited, 0));
+    // order after the seed-state kernels of batches on other lanes that initialise slots this batch reads:
+    // their lane records the event in its own launch, which may still be running
+    for (const auto &w8 : L.wait_lanes) {
+        std::unique_lock<std::mutex> lk(S->mu);
+        S->cv_inited.wait(lk, [&] { return S->lanes[w8.first].inited_batch >= w8.second; });
+        MCB_HIPCHK(hipStreamWaitEvent(L.st, S->lanes[w8.first].inited, 0));
+    }
     // pack descriptors into the lane's pinned staging, one H2D copy
     int64_t off = 0;
     auto put = [&](Seg &s, const void *src, int64_t bytes, int64_t n) {
@@ -257,6 +260,11 @@ int launch_batch(McbServer *S, Lane &L) {
         MCB_HIPCHK(launch_seed_init(B.pool->e->w, (const int64_t *)dp(B.seeds), (const int32_t *)dp(B.slots), (int)B.seeds.n,
                                     B.pool->d_slots, L.st));
     MCB_HIPCHK(hipEventRecord(L.inited, L.st));
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        L.inited_batch = L.batch_id;
+        S->cv_inited.notify_all();
+    }
     for (auto &B : wb) {
         const DevWorld &w = B.pool->e->w;
         MCB_HIPCHK(launch_grid(w, B.pool->d_slots, (const McbRequest *)dp(B.g), (const McbTile *)dp(B.gt), (int)B.gt.n,
@@ -280,109 +288,137 @@ void abandon_batch(McbServer *S, Lane &L) {
     L.pinned.clear();
 }
 
-void dispatcher_main(McbServer *S) {
+// Admission for lane L, with the server lock held: requests in arrival order while the lane's arena holds
+// their outputs and zoom staging (memory) and the world's unpinned seed slots hold the batch's new distinct
+// seeds.  Empty when the next request's seed finds every slot pinned by batches in flight.
+void admit(McbServer *S, Lane &L, int li) {
+    L.reqs.clear();
+    L.pinned.clear();
+    L.fresh.clear();
+    L.wait_lanes.clear();
+    L.batch_id = S->batches + 1;
+    int64_t out = 0, stage = 0;
+    // per world: the batch's distinct seeds, and how many of them need a slot that no batch in flight has
+    // pinned (an unmapped seed evicts one; a resident unpinned one pins its own)
+    std::unordered_map<int, std::unordered_set<int64_t>> seen;
+    std::unordered_map<int, int> need;
+    while (!S->pending.empty()) {
+        Req &q = S->pending.front();
+        if (!L.reqs.empty() && (out + q.out_bytes > S->batch_bytes || stage + q.stage_bytes > S->batch_bytes)) break;
+        const int w = q.r.world;
+        const bool new_seed = seen[w].count(q.r.seed) == 0;
+        bool takes_slot = false;
+        if (new_seed) {
+            auto pit = S->pools.find(w);
+            const SlotPool *P = (pit != S->pools.end() && pit->second.e) ? &pit->second : nullptr;
+            const int unpinned = P ? P->cap - P->npinned : S->slot_capacity;
+            takes_slot = !(P && P->pinned_seed(q.r.seed));
+            if (takes_slot && need[w] + 1 > unpinned) break;
+        }
+        if (new_seed) {
+            seen[w].insert(q.r.seed);
+            if (takes_slot) need[w]++;
+        }
+        // the world's slot pool (created with the world's engine on its first request)
+        SlotPool &pool = S->pools[w];
+        if (!pool.e) {
+            McbEngine *e = nullptr;
+            int rc = mcb_context_world_engine(S->ctx, w, &e);
+            if (!rc) rc = pool.init(e, S->slot_capacity);
+            if (rc) {
+                S->results[q.ticket] = {rc, 0.0};
+                S->fatal = mcb_last_error();
+                S->pending.pop_front();
+                S->cv_done.notify_all();
+                continue;
+            }
+        }
+        // pin the seed's slot now, so that other lanes' admissions see it taken
+        int fresh = 0;
+        const int slot = pool.acquire(q.r.seed, &fresh);
+        if (slot < 0) break;                              // every slot pinned (admission above prevents this)
+        L.pinned.push_back({w, slot});
+        if (fresh) {
+            L.fresh.push_back({w, slot});
+            pool.init_lane[slot] = li;
+            pool.init_batch[slot] = L.batch_id;
+        } else {
+            // state written by a batch still in flight on another lane: order after its seed-state kernels
+            const int il = pool.init_lane[slot];
+            const std::pair<int, int64_t> w8{il, pool.init_batch[slot]};
+            if (il >= 0 && il != li && S->lanes[il].busy && S->lanes[il].batch_id == w8.second &&
+                std::find(L.wait_lanes.begin(), L.wait_lanes.end(), w8) == L.wait_lanes.end())
+                L.wait_lanes.push_back(w8);
+        }
+        q.pool = &pool;
+        q.slot = slot;
+        q.fresh = fresh != 0;
+        out += q.out_bytes;
+        stage += q.stage_bytes;
+        q.t_admit = Clock::now();
+        L.reqs.push_back(q);
+        S->pending.pop_front();
+    }
+}
+
+// Thread of lane li: forms a batch whenever requests are queued and the lane is free, launches it,
+// waits for it, hands each request's copy-out to the copy threads and frees the lane once every
+// request's bytes have left its pinned buffer.  One thread per lane, so a batch whose host-side launch
+// stalls holds up only its own requests; the others keep admitting.
+void lane_main(McbServer *S, int li) {
+    Lane &L = S->lanes[li];
     int64_t blocked_at = -1;     // completions count when admission last found no free seed slot
     for (;;) {
-        int li = -1;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv_submit.wait(lk, [&] {
                 if (S->stop) return true;
                 if (S->pending.empty()) return false;
-                if (blocked_at >= 0 && S->completions == blocked_at) return false;
-                for (auto &L : S->lanes) if (!L.busy) return true;
-                return false;
+                return !(blocked_at >= 0 && S->completions == blocked_at);
             });
             if (S->stop) return;
             blocked_at = -1;
-            for (size_t i = 0; i < S->lanes.size(); i++) if (!S->lanes[i].busy) { li = (int)i; break; }
-            Lane &L = S->lanes[li];
-            L.reqs.clear();
-            L.pinned.clear();
-            L.fresh.clear();
-            int64_t out = 0, stage = 0;
-            // per world: the batch's distinct seeds, and how many of them need a slot that no batch in
-            // flight has pinned (an unmapped seed evicts one; a resident unpinned one pins its own)
-            std::unordered_map<int, std::unordered_set<int64_t>> seen;
-            std::unordered_map<int, int> need;
-            // admission in arrival order, while the lane arena holds the outputs and zoom staging
-            // (memory) and the world's unpinned seed slots hold the batch's new distinct seeds
-            while (!S->pending.empty()) {
-                Req &q = S->pending.front();
-                if (!L.reqs.empty() && (out + q.out_bytes > S->batch_bytes || stage + q.stage_bytes > S->batch_bytes)) break;
-                const int w = q.r.world;
-                const bool new_seed = seen[w].count(q.r.seed) == 0;
-                bool takes_slot = false;
-                if (new_seed) {
-                    auto pit = S->pools.find(w);
-                    const SlotPool *P = (pit != S->pools.end() && pit->second.e) ? &pit->second : nullptr;
-                    const int unpinned = P ? P->cap - P->npinned : S->slot_capacity;
-                    takes_slot = !(P && P->pinned_seed(q.r.seed));
-                    if (takes_slot && need[w] + 1 > unpinned) break;
-                }
-                if (new_seed) {
-                    seen[w].insert(q.r.seed);
-                    if (takes_slot) need[w]++;
-                }
-                out += q.out_bytes;
-                stage += q.stage_bytes;
-                L.reqs.push_back(q);
-                S->pending.pop_front();
-            }
+            admit(S, L, li);
             if (L.reqs.empty()) {
-                // every slot the next request could take is pinned by batches in flight: wait for one
                 blocked_at = S->completions;
                 continue;
             }
             L.busy = true;
-            L.batch_id = ++S->batches;
+            ++S->batches;
             S->admitted += (int64_t)L.reqs.size();
+            S->inflight++;
         }
-        Lane &L = S->lanes[li];
-        int rc = launch_batch(S, L);
-        std::lock_guard<std::mutex> lk(S->mu);
+        const int rc = launch_batch(S, L);
         if (rc) {
             (void)hipStreamSynchronize(L.st);
+            std::lock_guard<std::mutex> lk(S->mu);
+            L.inited_batch = std::max(L.inited_batch, L.batch_id);     // waiters on this batch proceed (fatal state)
+            S->cv_inited.notify_all();
             S->fatal = mcb_last_error();
             for (auto &q : L.reqs) S->results[q.ticket] = {rc, 0.0};
             abandon_batch(S, L);
+            S->inflight--;
             S->completions++;
             L.busy = false;
             S->cv_done.notify_all();
+            S->cv_submit.notify_all();
             continue;
         }
-        L.launched = true;
-        S->inflight++;
-        S->cv_lane.notify_all();
-    }
-}
-
-// completion thread of lane li: waits for the lane's batch, copies each request's result out of the
-// lane's pinned buffer and stamps its latency once its bytes are in the caller's buffer
-void completer_main(McbServer *S, int li) {
-    Lane &L = S->lanes[li];
-    std::vector<double> ms;
-    for (;;) {
-        {
-            std::unique_lock<std::mutex> lk(S->mu);
-            S->cv_lane.wait(lk, [&] { return S->stop || L.launched; });
-            if (!L.launched) return;
+        L.t_launched = Clock::now();
+        // poll rather than block: a blocking wait can sleep past the event by milliseconds
+        hipError_t e;
+        while ((e = hipEventQuery(L.done)) == hipErrorNotReady) std::this_thread::yield();
+        std::unique_lock<std::mutex> lk(S->mu);
+        L.t_done = Clock::now();
+        if (e == hipSuccess && !L.reqs.empty()) {
+            L.copies_left = (int)L.reqs.size();
+            for (size_t i = 0; i < L.reqs.size(); i++) S->copies.push_back({li, i});
+            S->cv_copy.notify_all();
+            S->cv_copied.wait(lk, [&] { return L.copies_left == 0; });
         }
-        hipError_t e = hipEventSynchronize(L.done);
-        ms.assign(L.reqs.size(), 0.0);
-        for (size_t i = 0; i < L.reqs.size(); i++) {
-            const Req &q = L.reqs[i];
-            if (e == hipSuccess) memcpy(q.host_out, L.h_out + q.r.out_off, (size_t)q.out_bytes);
-            ms[i] = std::chrono::duration<double, std::milli>(Clock::now() - q.t_submit).count();
-        }
-        std::lock_guard<std::mutex> lk(S->mu);
-        L.launched = false;
         S->inflight--;
-        for (size_t i = 0; i < L.reqs.size(); i++) {
-            S->results[L.reqs[i].ticket] = {e == hipSuccess ? 0 : (int)MCB_E_DEVICE, ms[i]};
-            if (e == hipSuccess) S->latencies.push_back(ms[i]);
-        }
         if (e != hipSuccess) {
+            for (auto &q : L.reqs) S->results[q.ticket] = {(int)MCB_E_DEVICE, 0.0};
             S->fatal = hipGetErrorString(e);
             abandon_batch(S, L);
         } else {
@@ -392,6 +428,34 @@ void completer_main(McbServer *S, int li) {
         L.busy = false;
         S->cv_done.notify_all();
         S->cv_submit.notify_all();
+    }
+}
+
+// copy thread: one request's bytes from its lane's pinned buffer to the caller, then its latency
+void copier_main(McbServer *S) {
+    auto span = [](Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+    for (;;) {
+        CopyJob j;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_copy.wait(lk, [&] { return S->stop || !S->copies.empty(); });
+            if (S->copies.empty()) return;
+            j = S->copies.front();
+            S->copies.pop_front();
+        }
+        Lane &L = S->lanes[j.lane];
+        const Req &q = L.reqs[j.req];
+        memcpy(q.host_out, L.h_out + q.r.out_off, (size_t)q.out_bytes);
+        const Clock::time_point t = Clock::now();
+        const double ms = span(q.t_submit, t);
+        const std::array<double, 4> ph = {span(q.t_submit, q.t_admit), span(q.t_admit, L.t_launched),
+                                          span(L.t_launched, L.t_done), span(L.t_done, t)};
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->results[q.ticket] = {0, ms};
+        S->latencies.push_back(ms);
+        S->phases.push_back(ph);
+        if (--L.copies_left == 0) S->cv_copied.notify_all();
+        S->cv_done.notify_all();
     }
 }
 
@@ -430,8 +494,8 @@ int mcb_server_create(McbServer **out, McbContext *ctx, int64_t batch_bytes, int
         MCB_HIPCHK(hipStreamSynchronize(L.st));
     }
     McbServer *s = S.release();
-    s->dispatcher = std::thread(dispatcher_main, s);
-    for (int i = 0; i < lanes; i++) s->lanes[i].completer = std::thread(completer_main, s, i);
+    for (int i = 0; i < lanes; i++) s->lanes[i].worker = std::thread(lane_main, s, i);
+    for (int i = 0; i < kCopyThreads; i++) s->copiers.emplace_back(copier_main, s);
     *out = s;
     return MCB_OK;
 }
@@ -500,10 +564,27 @@ int mcb_server_stats(McbServer *S, double *p50, double *p99, double *p999, int64
     return MCB_OK;
 }
 
+int mcb_server_slowest(McbServer *S, int n, double *out) {
+    if (!S || n < 0 || (n && !out)) return fail(MCB_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::vector<size_t> idx(S->latencies.size());
+    for (size_t i = 0; i < idx.size(); i++) idx[i] = i;
+    const size_t k = std::min<size_t>((size_t)n, idx.size());
+    std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](size_t a, size_t b) { return S->latencies[a] > S->latencies[b]; });
+    for (size_t i = 0; i < (size_t)n; i++) {
+        double *o = out + 5 * i;
+        if (i >= k) { for (int j = 0; j < 5; j++) o[j] = 0.0; continue; }
+        o[0] = S->latencies[idx[i]];
+        for (int j = 0; j < 4; j++) o[1 + j] = S->phases[idx[i]][j];
+    }
+    return (int)k;
+}
+
 int mcb_server_reset_stats(McbServer *S) {
     if (!S) return fail(MCB_E_ARG, "null server");
     std::lock_guard<std::mutex> lk(S->mu);
     S->latencies.clear();
+    S->phases.clear();
     for (auto &kv : S->pools) { kv.second.hits = 0; kv.second.misses = 0; }
     S->batches = 0;
     return MCB_OK;
@@ -517,9 +598,9 @@ void mcb_server_destroy(McbServer *S) {
         S->stop = true;
     }
     S->cv_submit.notify_all();
-    S->cv_lane.notify_all();
-    S->dispatcher.join();
-    for (auto &L : S->lanes) L.completer.join();
+    S->cv_copy.notify_all();
+    for (auto &L : S->lanes) L.worker.join();
+    for (auto &t : S->copiers) t.join();
     for (auto &L : S->lanes) {
         (void)hipStreamSynchronize(L.st);
         (void)hipStreamDestroy(L.st);

@@ -159,6 +159,7 @@ def lib():
         L.mcb_server_stats.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), i64p,
                                        C.POINTER(C.c_double), i64p]
         L.mcb_server_reset_stats.argtypes = [P]
+        L.mcb_server_slowest.argtypes = [P, C.c_int, C.POINTER(C.c_double)]
         L.mcb_server_destroy.argtypes = [P]
         L.mcg_host_load.argtypes = [C.POINTER(McgHost), C.c_int, C.c_int, C.c_char_p]
         L.mcg_host_free.argtypes = [C.POINTER(McgHost)]
@@ -507,6 +508,13 @@ class Server:
                                       C.byref(b)))
         return {"p50_ms": p50.value, "p99_ms": p99.value, "p999_ms": p999.value, "completed": n.value,
                 "slot_hit_rate": hit.value, "batches": b.value}
+
+    def slowest(self, n=5):
+        """The n slowest requests: [(total, queued, launch, device, copy) ms], slowest first."""
+        out = np.zeros(5 * n, np.float64)
+        k = lib().mcb_server_slowest(self.h, n, out.ctypes.data_as(C.POINTER(C.c_double)))
+        _check(0 if k >= 0 else k)
+        return [tuple(round(float(v), 3) for v in out[5 * i: 5 * i + 5]) for i in range(k)]
 
     def reset_stats(self):
         _check(lib().mcb_server_reset_stats(self.h))
