@@ -492,7 +492,57 @@ struct SharedTables {
     const int32_t *cbp = nullptr;
     const uint32_t *centry = nullptr;
     const uint16_t *clist = nullptr;
+    const uint32_t *clut = nullptr;      // null: the table has no locator
+    int32_t cstride[6] = {0, 0, 0, 0, 0, 0};
+    const PRec *crec = nullptr;
 };
+// The locator of targets within 16 bits: per dimension 256 buckets of 256 values, entry = (breakpoints at or
+// below the bucket's first value) * stride | (offset of the one breakpoint inside the bucket, 256: none) << 20,
+// so a dimension's cell offset is one load and a compare.  Usable only when no bucket holds two breakpoints
+// and every entry index fits 20 bits.  recs: the listed leaves copied in list order.
+bool cell_locator(const mcb::CellTable &ct, const std::vector<PNode> &T, std::vector<uint32_t> &lut, int32_t stride[6],
+                  std::vector<PRec> &recs) {
+    if (ct.entry.size() > (1u << 20)) return false;
+    int n[6];
+    for (int k = 0; k < 6; k++) n[k] = ct.nb[k] + 1;
+    // index = (((((cT*nH + cH)*nC + cC)*nE + cE)*nW + cW)*nD + cD)
+    stride[4] = 1;
+    stride[5] = n[4];
+    stride[3] = n[5] * stride[5];
+    stride[2] = n[3] * stride[3];
+    stride[1] = n[2] * stride[2];
+    stride[0] = n[1] * stride[1];
+    if ((size_t)n[0] * (size_t)stride[0] != ct.entry.size()) return false;
+    lut.assign(6 * 256, 0);
+    for (int k = 0; k < 6; k++) {
+        const int32_t *bp = ct.bp.data() + ct.off[k];
+        for (int b = 0; b < 256; b++) {
+            const int t0 = b * 256 - 32768;
+            int below = 0, inside = 0, off = 256;
+            for (int i = 0; i < ct.nb[k]; i++) {
+                if (bp[i] <= t0) below++;
+                else if (bp[i] <= t0 + 255) {
+                    inside++;
+                    off = bp[i] - t0;
+                }
+            }
+            if (inside > 1) return false;
+            lut[k * 256 + b] = (uint32_t)below * (uint32_t)stride[k] | (uint32_t)off << 20;
+        }
+    }
+    recs.resize(std::max<size_t>(ct.list.size(), 1));
+    for (size_t i = 0; i < ct.list.size(); i++) {
+        const PNode &p = T[ct.list[i]];
+        PRec &r = recs[i];
+        for (int k = 0; k < 6; k++) {
+            r.lo[k] = (int16_t)p.lo[k];
+            r.hi[k] = (int16_t)p.hi[k];
+        }
+        r.d6 = (uint32_t)p.d6;
+        r.leaf = (int32_t)ct.list[i];
+    }
+    return true;
+}
 // null with *failed false: not shared (the world uploads its own copy); *failed: a device error
 const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
     *failed = false;
@@ -501,18 +551,22 @@ const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
     auto it = cache.find(key);
     if (it != cache.end()) return it->second.ntree == H.tree.size() ? &it->second : nullptr;
     const mcb::CellTable &ct = *H.cells;
-    const size_t b[4] = {H.tree.size() * sizeof(PNode), ct.bp.size() * sizeof(int32_t), ct.entry.size() * sizeof(uint32_t),
-                         ct.list.size() * sizeof(uint16_t)};
-    const void *src[4] = {H.tree.data(), ct.bp.data(), ct.entry.data(), ct.list.data()};
-    size_t off[4], total = 0;
-    for (int i = 0; i < 4; i++) { off[i] = total; total += (b[i] + 255) & ~(size_t)255; }
     SharedTables s;
+    std::vector<uint32_t> lut;
+    std::vector<PRec> recs;
+    const bool located = cell_locator(ct, H.tree, lut, s.cstride, recs);
+    const size_t b[6] = {H.tree.size() * sizeof(PNode), ct.bp.size() * sizeof(int32_t), ct.entry.size() * sizeof(uint32_t),
+                         ct.list.size() * sizeof(uint16_t), located ? lut.size() * sizeof(uint32_t) : 0,
+                         located ? recs.size() * sizeof(PRec) : 0};
+    const void *src[6] = {H.tree.data(), ct.bp.data(), ct.entry.data(), ct.list.data(), lut.data(), recs.data()};
+    size_t off[6], total = 0;
+    for (int i = 0; i < 6; i++) { off[i] = total; total += (b[i] + 255) & ~(size_t)255; }
     if (hipMalloc((void **)&s.d, std::max<size_t>(total, 256)) != hipSuccess) {
         plugin_error("shared table allocation failed");
         *failed = true;
         return nullptr;
     }
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 6; i++)
         if (b[i] && hipMemcpy(s.d + off[i], src[i], b[i], hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipFree(s.d);
             plugin_error("shared table upload failed");
@@ -524,6 +578,10 @@ const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
     s.cbp = (const int32_t *)(s.d + off[1]);
     s.centry = (const uint32_t *)(s.d + off[2]);
     s.clist = (const uint16_t *)(s.d + off[3]);
+    if (located) {
+        s.clut = (const uint32_t *)(s.d + off[4]);
+        s.crec = (const PRec *)(s.d + off[5]);
+    }
     return &(cache[key] = s);
 }
 
@@ -592,6 +650,11 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
         P.centry = sh ? sh->centry : (const uint32_t *)at(11);
         P.clist = sh ? sh->clist : (const uint16_t *)at(12);
         for (int k = 0; k < 6; k++) { P.cnb[k] = H.cells->nb[k]; P.coff[k] = H.cells->off[k]; }
+        if (sh && sh->clut) {
+            P.clut = sh->clut;
+            P.crec = sh->crec;
+            for (int k = 0; k < 6; k++) P.cstride[k] = sh->cstride[k];
+        }
     }
     P.nins = (int32_t)C.nins;
     P.nydep = (int32_t)C.ins.size() - C.nins;

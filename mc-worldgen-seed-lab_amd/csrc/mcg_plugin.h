@@ -54,6 +54,15 @@ struct PNode {
 };
 static_assert(sizeof(PNode) == 72, "PNode layout");
 
+// A leaf listed in a climate cell, in the compact form the cell table needs (its bounds fit 16 bits and its
+// offset distance 32 when the table exists): the records follow the cell lists' order, one load each.
+struct PRec {
+    int16_t lo[6], hi[6];
+    uint32_t d6;
+    int32_t leaf;                  // the leaf's tree node
+};
+static_assert(sizeof(PRec) == 32, "PRec layout");
+
 // device view of one world
 struct PWorld {
     const PIns *ins;
@@ -72,6 +81,11 @@ struct PWorld {
     const uint32_t *centry;        // (first << 8) | count; count 0: walk the tree
     const uint16_t *clist;
     int32_t cnb[6], coff[6];
+    // cell locator of targets within 16 bits (null: binary searches over cbp): per tree dimension 256
+    // buckets of 256 values, entry = (cells below the bucket) * cstride | (offset of its one breakpoint) << 20
+    const uint32_t *clut;
+    int32_t cstride[6];
+    const PRec *crec;              // the listed leaves, entry by entry
     int32_t nins, noct, ntree, nslot, spl_depth;
     int32_t nydep;                 // ins[nins .. nins + nydep): the y-dependent instructions again
     int32_t nnoise, nlayers;

@@ -72,8 +72,51 @@ __device__ bool rt_tie(const PNode *__restrict__ T, int n, const int64_t *tg, ui
 // and the number of leaves at that distance is the tie count.  Within the table's +-2^17 range the
 // distances stay far below 2^63, so no sum wraps.  Returns false when the target is outside the table
 // or its cell is not listed (the caller walks the tree).
+__device__ __forceinline__ uint64_t rdist(const PRec &n, const int64_t *tg) {   // pdist of a listed leaf
+    uint64_t d = n.d6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int64_t above = tg[k] - (int64_t)n.hi[k], below = (int64_t)n.lo[k] - tg[k];
+        const int64_t x = above > 0 ? above : (below > 0 ? below : 0);
+        d += (uint64_t)x * (uint64_t)x;
+    }
+    return d;
+}
+// Targets within 16 bits: the cell offset of each dimension from the locator (one load per dimension
+// instead of a binary search), candidates from the entry's compact records.
+__device__ __forceinline__ bool cell_search_lut(const PWorld &W, const int64_t *tg, int &leaf, uint64_t &d, bool &tied) {
+    uint32_t idx = 0;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const uint32_t u = (uint32_t)((int32_t)tg[k] + 32768);
+        const uint32_t e = W.clut[k * 256 + (int)(u >> 8)];
+        idx += (e & 0xFFFFFu) + ((u & 255u) >= (e >> 20) ? (uint32_t)W.cstride[k] : 0u);
+    }
+    const uint32_t e = W.centry[idx];
+    const int cnt = (int)(e & 0xFF);
+    if (!cnt) return false;
+    const PRec *R = W.crec + (e >> 8);
+    uint64_t best = ~0ULL;
+    int bl = -1, nt = 0;
+    for (int q = 0; q < cnt; q++) {
+        const PRec r = R[q];
+        const uint64_t dq = rdist(r, tg);
+        if (dq < best) { best = dq; bl = r.leaf; nt = 1; }
+        else if (dq == best) nt++;
+    }
+    leaf = bl;
+    d = best;
+    tied = nt >= 2;
+    return true;
+}
 __device__ __forceinline__ bool cell_search(const PWorld &W, const int64_t *tg, int &leaf, uint64_t &d, bool &tied) {
     if (!W.centry) return false;
+    if (W.clut) {
+        bool in16 = true;
+#pragma unroll
+        for (int k = 0; k < 6; k++) in16 = in16 && tg[k] >= -32768 && tg[k] <= 32767;
+        if (in16) return cell_search_lut(W, tg, leaf, d, tied);
+    }
     int c[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
