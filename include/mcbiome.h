@@ -272,6 +272,15 @@ void mcb_server_destroy(McbServer *s);
 struct McgBiomeWorld;
 int mcb_plugin_world(McbEngine *e, int64_t seed, struct McgBiomeWorld **out);
 void mcb_plugin_world_free(struct McgBiomeWorld *w);
+/* How a handle of mcgpu_biome_new evaluates its climate: 1 = the program is the vanilla Overworld climate
+ * router (the graph mcb_plugin_world writes, in libmcgen's lowering) and runs through the engine's fused
+ * column code, 0 = the density-function interpreter; the biomes are the same either way.  With the
+ * environment variable MCB_PLUGIN_INTERPRET=1 set when the handle is made, every program is interpreted
+ * (to compare the two paths).  No counterpart in the reference. */
+int mcb_plugin_router(void *h);
+/* The same recognition for a world description, on the host only (no device call): 1 when
+ * mcgpu_biome_new would run it through the engine's column code. */
+int mcb_plugin_router_world(const struct McgBiomeWorld *w);
 
 /* A terrain world (McgTerrainWorld of include/mcgpu.h) over the engine's vanilla 26.3+ Overworld climate
  * for one seed, shaped like the game's Overworld final_density: depth from the y gradient and the offset

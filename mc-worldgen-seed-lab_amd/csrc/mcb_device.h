@@ -627,9 +627,12 @@ __device__ __forceinline__ void shift_coords(typename R<MODE>::T a, typename R<M
     }
 }
 
+// The five climate noises of a column: temperature, humidity, continentalness, erosion, weirdness at the
+// shifted coordinates (engine/mc_climate.h:113-164, before the spline and quantization)
+template <int MODE> struct Noises5 { typename R<MODE>::T t, h, c, e, w; };
 template <int MODE, class PS, bool WRAP = true>
-__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, PS perm, const typename GT<MODE>::T *G,
-                                          const SplLds *SP, const McbOct *oct, int qx, int qz, Column &c) {
+__device__ __forceinline__ Noises5<MODE> ow_noises(const McbWorld *__restrict__ W, PS perm, const typename GT<MODE>::T *G,
+                                                   const McbOct *oct, int qx, int qz) {
     typedef typename R<MODE>::T T_;
     const McbNoise *N = W->noise;
     const double X = (double)qx, Z = (double)qz;     // blockX*0.25 with blockX = 4*quartX
@@ -638,12 +641,19 @@ __device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, PS per
     const T_ b = normal2<MODE, true, 3, 0, false, PS, WRAP>(perm, G, oct, N[0], Z, X);
     double x, z;
     shift_coords<MODE>(a, b, X, Z, x, z);
-    const T_ T = normal2<MODE, false, 2, 6, false, PS, WRAP>(perm, G, oct, N[1], x, z);
-    const T_ H = normal2<MODE, false, 2, 10, false, PS, WRAP>(perm, G, oct, N[2], x, z);
-    const T_ C = normal2<MODE, false, 9, 14, false, PS, WRAP>(perm, G, oct, N[3], x, z);
-    const T_ E = normal2<MODE, false, 4, 32, false, PS, WRAP>(perm, G, oct, N[4], x, z);
-    const T_ Wr = normal2<MODE, false, 3, 40, false, PS, WRAP>(perm, G, oct, N[5], x, z);
-    column_finish<MODE>(W, SP, T, H, C, E, Wr, c);
+    Noises5<MODE> r;
+    r.t = normal2<MODE, false, 2, 6, false, PS, WRAP>(perm, G, oct, N[1], x, z);
+    r.h = normal2<MODE, false, 2, 10, false, PS, WRAP>(perm, G, oct, N[2], x, z);
+    r.c = normal2<MODE, false, 9, 14, false, PS, WRAP>(perm, G, oct, N[3], x, z);
+    r.e = normal2<MODE, false, 4, 32, false, PS, WRAP>(perm, G, oct, N[4], x, z);
+    r.w = normal2<MODE, false, 3, 40, false, PS, WRAP>(perm, G, oct, N[5], x, z);
+    return r;
+}
+template <int MODE, class PS, bool WRAP = true>
+__device__ __forceinline__ void ow_column(const McbWorld *__restrict__ W, PS perm, const typename GT<MODE>::T *G,
+                                          const SplLds *SP, const McbOct *oct, int qx, int qz, Column &c) {
+    const Noises5<MODE> n = ow_noises<MODE, PS, WRAP>(W, perm, G, oct, qx, qz);
+    column_finish<MODE>(W, SP, n.t, n.h, n.c, n.e, n.w, c);
 }
 
 // depth = clamped y gradient at block y + offset, as the float the sampler returns

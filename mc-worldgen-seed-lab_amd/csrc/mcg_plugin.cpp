@@ -14,6 +14,7 @@
 #include <map>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -388,17 +389,22 @@ bool convert_tree(const McgTreeNode *T, int n, int root, std::vector<PNode> &out
 struct GBiome {
     PWorld W;
     bool newf = false, perm_lds = false;
+    bool router = false;                    // the vanilla router runs through the engine's column code
     int lds = 0;
     int source = 0;
     hipStream_t st = nullptr;
     char *d_data = nullptr;                 // program and tree, one allocation
     uint8_t *d_out = nullptr, *d_lat = nullptr;
     int32_t *d_xyz = nullptr, *d_bad = nullptr;
-    size_t out_cap = 0, lat_cap = 0, xyz_cap = 0;
+    int32_t *d_tgt = nullptr, *d_fix = nullptr;   // the router's lattice: column targets, cells to follow along their chains
+    WinCell *d_win = nullptr;                     // and their chain windows
+    size_t out_cap = 0, lat_cap = 0, xyz_cap = 0, tgt_cap = 0, fix_cap = 0, win_cap = 0;
     void release() {
         if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
         (void)hipFree(d_data); (void)hipFree(d_out); (void)hipFree(d_lat); (void)hipFree(d_xyz); (void)hipFree(d_bad);
-        st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr;
+        (void)hipFree(d_tgt); (void)hipFree(d_fix); (void)hipFree(d_win);
+        st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr; d_tgt = d_fix = nullptr;
+        d_win = nullptr;
     }
 };
 
@@ -420,6 +426,9 @@ struct HostWorld {
     std::vector<PNode> tree;
     std::shared_ptr<const mcb::CellTable> cells;   // strict cell table, when the tree admits one
     const McgProg *p = nullptr;
+    // the vanilla Overworld router: the engine's constants and seed layout (null: interpret)
+    std::unique_ptr<McbWorld> vw;
+    std::unique_ptr<McbSlot> vslot;
 };
 
 // The cell table is built over the engine's compact node form, so it is used only when that form holds
@@ -471,6 +480,13 @@ bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
     if (W->source == MCG_BS_MULTI) {
         if (!convert_tree(W->tree, W->ntree, W->tree_root, H.tree)) return false;
         H.cells = plugin_cells(H.tree);
+        // MCB_PLUGIN_INTERPRET=1: interpret every program (comparisons of the two paths)
+        const char *force = getenv("MCB_PLUGIN_INTERPRET");
+        if (!(force && force[0] == '1')) {
+            H.vw.reset(new McbWorld());
+            H.vslot.reset(new McbSlot());
+            if (!vanilla_router(W, *H.vw, *H.vslot)) { H.vw.reset(); H.vslot.reset(); }
+        }
     }
     const int vbytes = H.C.nslot * kWG * (W->newf ? 4 : 8);
     if (grad_lds_bytes(W->newf != 0) + vbytes > kMaxLdsBytes) {
@@ -495,14 +511,15 @@ struct SharedTables {
     const uint32_t *clut = nullptr;      // null: the table has no locator
     int32_t cstride[6] = {0, 0, 0, 0, 0, 0};
     const PRec *crec = nullptr;
+    const uint32_t *cent1 = nullptr;
 };
 // The locator of targets within 16 bits: per dimension 256 buckets of 256 values, entry = (breakpoints at or
 // below the bucket's first value) * stride | (offset of the one breakpoint inside the bucket, 256: none) << 20,
 // so a dimension's cell offset is one load and a compare.  Usable only when no bucket holds two breakpoints
 // and every entry index fits 20 bits.  recs: the listed leaves copied in list order.
 bool cell_locator(const mcb::CellTable &ct, const std::vector<PNode> &T, std::vector<uint32_t> &lut, int32_t stride[6],
-                  std::vector<PRec> &recs) {
-    if (ct.entry.size() > (1u << 20)) return false;
+                  std::vector<PRec> &recs, std::vector<uint32_t> &ent1) {
+    if (ct.entry.size() > (1u << 20) || ct.list.size() >= (1u << 24)) return false;
     int n[6];
     for (int k = 0; k < 6; k++) n[k] = ct.nb[k] + 1;
     // index = (((((cT*nH + cH)*nC + cC)*nE + cE)*nW + cW)*nD + cD)
@@ -539,7 +556,12 @@ bool cell_locator(const mcb::CellTable &ct, const std::vector<PNode> &T, std::ve
             r.hi[k] = (int16_t)p.hi[k];
         }
         r.d6 = (uint32_t)p.d6;
-        r.leaf = (int32_t)ct.list[i];
+        r.leaf = (int32_t)ct.list[i] | (p.biome & 0xFF) << 16;
+    }
+    ent1.resize(ct.entry.size());
+    for (size_t i = 0; i < ct.entry.size(); i++) {
+        const uint32_t e = ct.entry[i], cnt = e & 255u;
+        ent1[i] = cnt == 1 ? 1u | (uint32_t)ct.list[e >> 8] << 8 | (uint32_t)(T[ct.list[e >> 8]].biome & 0xFF) << 24 : e;
     }
     return true;
 }
@@ -552,21 +574,21 @@ const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
     if (it != cache.end()) return it->second.ntree == H.tree.size() ? &it->second : nullptr;
     const mcb::CellTable &ct = *H.cells;
     SharedTables s;
-    std::vector<uint32_t> lut;
+    std::vector<uint32_t> lut, ent1;
     std::vector<PRec> recs;
-    const bool located = cell_locator(ct, H.tree, lut, s.cstride, recs);
-    const size_t b[6] = {H.tree.size() * sizeof(PNode), ct.bp.size() * sizeof(int32_t), ct.entry.size() * sizeof(uint32_t),
+    const bool located = cell_locator(ct, H.tree, lut, s.cstride, recs, ent1);
+    const size_t b[7] = {H.tree.size() * sizeof(PNode), ct.bp.size() * sizeof(int32_t), ct.entry.size() * sizeof(uint32_t),
                          ct.list.size() * sizeof(uint16_t), located ? lut.size() * sizeof(uint32_t) : 0,
-                         located ? recs.size() * sizeof(PRec) : 0};
-    const void *src[6] = {H.tree.data(), ct.bp.data(), ct.entry.data(), ct.list.data(), lut.data(), recs.data()};
-    size_t off[6], total = 0;
-    for (int i = 0; i < 6; i++) { off[i] = total; total += (b[i] + 255) & ~(size_t)255; }
+                         located ? recs.size() * sizeof(PRec) : 0, located ? ent1.size() * sizeof(uint32_t) : 0};
+    const void *src[7] = {H.tree.data(), ct.bp.data(), ct.entry.data(), ct.list.data(), lut.data(), recs.data(), ent1.data()};
+    size_t off[7], total = 0;
+    for (int i = 0; i < 7; i++) { off[i] = total; total += (b[i] + 255) & ~(size_t)255; }
     if (hipMalloc((void **)&s.d, std::max<size_t>(total, 256)) != hipSuccess) {
         plugin_error("shared table allocation failed");
         *failed = true;
         return nullptr;
     }
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 7; i++)
         if (b[i] && hipMemcpy(s.d + off[i], src[i], b[i], hipMemcpyHostToDevice) != hipSuccess) {
             (void)hipFree(s.d);
             plugin_error("shared table upload failed");
@@ -581,6 +603,7 @@ const SharedTables *shared_tables(const HostWorld &H, bool *failed) {
     if (located) {
         s.clut = (const uint32_t *)(s.d + off[4]);
         s.crec = (const PRec *)(s.d + off[5]);
+        s.cent1 = (const uint32_t *)(s.d + off[6]);
     }
     return &(cache[key] = s);
 }
@@ -600,7 +623,7 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     g->source = W->source;
     g->newf = W->newf != 0;
     HCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
-    HCHK(hipMalloc((void **)&g->d_bad, sizeof(int32_t)));
+    HCHK(hipMalloc((void **)&g->d_bad, 4 * sizeof(int32_t)));      // [0] unresolved count, [1..2] the router's list lengths
     if (W->source == MCG_BS_FIXED) return true;
     const McgProg *p = H.p;
     const Compiled &C = H.C;
@@ -626,7 +649,8 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
         {p->thr, (size_t)p->nthr * sizeof(double), 0},           {H.tree.data(), H.tree.size() * sizeof(PNode), 0},
         {H.cells ? H.cells->bp.data() : nullptr, H.cells ? H.cells->bp.size() * sizeof(int32_t) : 0, 0},
         {H.cells ? H.cells->entry.data() : nullptr, H.cells ? H.cells->entry.size() * sizeof(uint32_t) : 0, 0},
-        {H.cells ? H.cells->list.data() : nullptr, H.cells ? H.cells->list.size() * sizeof(uint16_t) : 0, 0}};
+        {H.cells ? H.cells->list.data() : nullptr, H.cells ? H.cells->list.size() * sizeof(uint16_t) : 0, 0},
+        {H.vw.get(), H.vw ? sizeof(McbWorld) : 0, 0},             {H.vslot.get(), H.vslot ? sizeof(McbSlot) : 0, 0}};
     size_t total = 0;
     if (sh)
         for (int i = 9; i < 13; i++) parts[i].bytes = 0;
@@ -653,8 +677,16 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
         if (sh && sh->clut) {
             P.clut = sh->clut;
             P.crec = sh->crec;
+            P.cent1 = sh->cent1;
             for (int k = 0; k < 6; k++) P.cstride[k] = sh->cstride[k];
         }
+    }
+    if (H.vw) {
+        P.vw = (const McbWorld *)at(13);
+        P.vslot = (const McbSlot *)at(14);
+        HCHK(finish_router_slot((McbSlot *)at(14), g->st));
+        HCHK(hipStreamSynchronize(g->st));
+        g->router = true;
     }
     P.nins = (int32_t)C.nins;
     P.nydep = (int32_t)C.ins.size() - C.nins;
@@ -696,7 +728,17 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
         J.qys[1] = clampi(py + 1, lo, hi);
         J.out = g->d_lat;
         J.nbad = nullptr;                    // counted per output point by the zoom pass
-        HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        // the router's three launches keep 36 bytes per column (targets and fix list): up to 2^27 columns
+        if (g->router && g->W.clut && g->W.cent1 && nax * naz <= ((int64_t)1 << 27)) {
+            const size_t per = (size_t)(nax * naz);
+            if (!grow(&g->d_tgt, &g->tgt_cap, 7 * per) || !grow(&g->d_fix, &g->fix_cap, 4 * per) ||
+                !grow(&g->d_win, &g->win_cap, (size_t)kWinCells * kWin))
+                return false;
+            HCHK(hipMemsetAsync(g->d_bad + 1, 0, 2 * sizeof(int32_t), g->st));
+            HCHK(launch_van_lattice(g->W, J, g->newf, g->d_tgt, g->d_fix, g->d_win, g->d_bad + 1, g->st));
+        } else {
+            HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        }
         HCHK(launch_zoom_lattice(g->W, J, x0, z0, nx, nz, step, y, g->d_out, g->d_bad, g->st));
     } else {
         J.gen = GEN_GRIDPTS;
@@ -915,6 +957,17 @@ void *mcgpu_biome_new(const McgBiomeWorld *W, char *err, size_t errlen) {
     g->release();
     if (err && errlen) snprintf(err, errlen, "%s", mcb_last_error());
     return nullptr;
+}
+
+int mcb_plugin_router(void *h) {
+    Gate gate(false);
+    return h && ((GBiome *)h)->router ? 1 : 0;
+}
+
+int mcb_plugin_router_world(const McgBiomeWorld *W) {
+    std::unique_ptr<McbWorld> w(new McbWorld());
+    std::unique_ptr<McbSlot> slot(new McbSlot());
+    return W && vanilla_router(W, *w, *slot) ? 1 : 0;
 }
 
 void mcgpu_biome_free(void *h) {

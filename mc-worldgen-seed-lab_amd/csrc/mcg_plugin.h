@@ -15,6 +15,7 @@
 #include <mutex>
 
 #include "../../include/mcgpu.h"
+#include "mcb_common.h"
 
 namespace mcb {
 namespace plug {
@@ -59,7 +60,7 @@ static_assert(sizeof(PNode) == 72, "PNode layout");
 struct PRec {
     int16_t lo[6], hi[6];
     uint32_t d6;
-    int32_t leaf;                  // the leaf's tree node
+    int32_t leaf;                  // the leaf's tree node | its biome << 16
 };
 static_assert(sizeof(PRec) == 32, "PRec layout");
 
@@ -86,6 +87,10 @@ struct PWorld {
     const uint32_t *clut;
     int32_t cstride[6];
     const PRec *crec;              // the listed leaves, entry by entry
+    // the entries again for the locator's path: count in bits 0-7; count 1: the one leaf that can be at
+    // the least distance anywhere in the cell (tree node in bits 8-23, biome in bits 24-31); count > 1:
+    // first record in crec (bits 8-31); count 0: walk the tree
+    const uint32_t *cent1;
     int32_t nins, noct, ntree, nslot, spl_depth;
     int32_t nydep;                 // ins[nins .. nins + nydep): the y-dependent instructions again
     int32_t nnoise, nlayers;
@@ -93,10 +98,14 @@ struct PWorld {
     int32_t root_slot[6];          // -1: field absent
     int32_t source, fixed_biome, end_ids[5], min_qy, qh;
     int64_t zoom_seed;
+    // the vanilla Overworld router (mcg_router.cpp): its constants in the engine's world form and its
+    // octaves in the engine's seed layout; null: the program is interpreted
+    const McbWorld *vw;
+    const McbSlot *vslot;
 };
 
 // work description of one launch
-enum { GEN_LATTICE = 0, GEN_POINTS = 1, GEN_GRIDPTS = 2, GEN_CHUNKS = 3 };
+enum { GEN_LATTICE = 0, GEN_POINTS = 1, GEN_GRIDPTS = 2, GEN_CHUNKS = 3, GEN_LIST = 4 };
 struct PJob {
     int32_t gen;
     int64_t n;                     // items
@@ -109,6 +118,10 @@ struct PJob {
     int32_t cells;                 // chunks: quart cells per chunk (16 per quart layer), item = chunk * cells + cell
     uint8_t *out;
     int32_t *nbad;
+    // list: the lattice cells list[0 .. *nlist) (cell i = layer * nqx * nqz + column, as the lattice numbers
+    // them), each written to out[i]
+    const int32_t *list;
+    const int32_t *nlist;
 };
 
 // Whole chunks of a world description in libmcgen's stored form (world_chunk_biomes,
@@ -133,8 +146,34 @@ struct PluginGate {
 };
 bool plugin_fail(const char *fmt, ...);
 
+// The program of W is the vanilla Overworld climate router (mcg_router.cpp): w = its constants in the
+// engine's world form, slot = its octaves in the engine's seed layout (permutations and offsets; the
+// derived McbOct fields are completed on the device by finish_router_slot).  false: interpret the program.
+bool vanilla_router(const McgBiomeWorld *W, McbWorld &w, McbSlot &slot);
+hipError_t finish_router_slot(McbSlot *d_slot, hipStream_t st);
+
 // launchers (mcg_plugin.hip); lds = bytes of dynamic LDS, perm_lds = permutation tables staged in LDS
 hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st);
+// The lattice of a world with the vanilla router (W.vw): the quantized climate of every column into tgt
+// (7 * columns int32); the table search of every cell at high occupancy, listing the cells it cannot settle
+// (ties to follow along the fill order, targets beyond the table) in fix / cnt[0]; for the first
+// kWinCells of those, every cell of their chain window (the cell and kWin - 1 cells back in fill order)
+// evaluated in parallel into win, then each chain resolved from its window; chains the window cannot
+// settle (uncapped ones that run past it) and listed cells beyond kWinCells through the per-lane state
+// machine (list fix + 2 * columns, length cnt[1]).  tgt holds 7 * nqx * nqz values, fix 4 * nqx * nqz, win
+// kWinCells * kWin records; cnt[0..1] must be zero on entry.
+constexpr int kWin = kChainCap + 1;
+constexpr int kWinCells = 32768;
+struct WinCell {
+    int64_t tg[6];                 // the cell's targets
+    uint64_t d;                    // its least distance (unless single)
+    int32_t leaf, biome;           // first leaf at the least distance, its biome
+    uint8_t ok, tied, single, pad; // ok: finite targets and a leaf found
+    int32_t pad2;
+};
+static_assert(sizeof(WinCell) == 72, "WinCell layout");
+hipError_t launch_van_lattice(const PWorld &W, const PJob &J, bool newf, int32_t *tgt, int32_t *fix, WinCell *win,
+                              int32_t *cnt, hipStream_t st);
 // block grid from a lattice of cells (MULTI) or of chunks (END): out[iz*nx+ix]
 hipError_t launch_zoom_lattice(const PWorld &W, const PJob &L, int x0, int z0, int nx, int nz, int step, int y,
                                uint8_t *out, int32_t *nbad, hipStream_t st);

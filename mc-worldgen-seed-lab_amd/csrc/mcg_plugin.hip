@@ -72,50 +72,71 @@ __device__ bool rt_tie(const PNode *__restrict__ T, int n, const int64_t *tg, ui
 // and the number of leaves at that distance is the tie count.  Within the table's +-2^17 range the
 // distances stay far below 2^63, so no sum wraps.  Returns false when the target is outside the table
 // or its cell is not listed (the caller walks the tree).
-__device__ __forceinline__ uint64_t rdist(const PRec &n, const int64_t *tg) {   // pdist of a listed leaf
-    uint64_t d = n.d6;
+struct Found {
+    int leaf, biome;               // the first leaf at the least distance and its biome
+    uint64_t d;                    // the least distance (unset when single)
+    bool tied;                     // two or more leaves at the least distance
+    bool single;                   // the cell lists one leaf: nothing else can be at the least distance
+};
+// Targets within 16 bits: the cell offset of each dimension from the locator (one load per dimension
+// instead of a binary search).  A cell with one listed leaf answers from its entry; otherwise the
+// listed leaves' compact records are scanned with distances in doubles: every term is the square of an
+// integer below 2^17 and there are seven, so each sum is an exact integer (the u64 sum's value).
+__device__ __forceinline__ double rdist_d(const PRec &r, const int *t) {
+    double d = (double)r.d6;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        const int64_t above = tg[k] - (int64_t)n.hi[k], below = (int64_t)n.lo[k] - tg[k];
-        const int64_t x = above > 0 ? above : (below > 0 ? below : 0);
-        d += (uint64_t)x * (uint64_t)x;
+        const int x = max(max(t[k] - (int)r.hi[k], (int)r.lo[k] - t[k]), 0);
+        d = fma((double)x, (double)x, d);
     }
     return d;
 }
-// Targets within 16 bits: the cell offset of each dimension from the locator (one load per dimension
-// instead of a binary search), candidates from the entry's compact records.
-__device__ __forceinline__ bool cell_search_lut(const PWorld &W, const int64_t *tg, int &leaf, uint64_t &d, bool &tied) {
+__device__ __forceinline__ bool cell_search_lut(const PWorld &W, const int *t, Found &f) {
     uint32_t idx = 0;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        const uint32_t u = (uint32_t)((int32_t)tg[k] + 32768);
+        const uint32_t u = (uint32_t)(t[k] + 32768);
         const uint32_t e = W.clut[k * 256 + (int)(u >> 8)];
         idx += (e & 0xFFFFFu) + ((u & 255u) >= (e >> 20) ? (uint32_t)W.cstride[k] : 0u);
     }
-    const uint32_t e = W.centry[idx];
+    const uint32_t e = W.cent1[idx];
     const int cnt = (int)(e & 0xFF);
     if (!cnt) return false;
+    f.tied = false;
+    if (cnt == 1) {
+        f.single = true;
+        f.leaf = (int)((e >> 8) & 0xFFFFu);
+        f.biome = (int)(e >> 24);
+        return true;
+    }
+    f.single = false;
     const PRec *R = W.crec + (e >> 8);
-    uint64_t best = ~0ULL;
-    int bl = -1, nt = 0;
+    double best = __builtin_inf();
+    int bl = 0, nt = 0;
     for (int q = 0; q < cnt; q++) {
         const PRec r = R[q];
-        const uint64_t dq = rdist(r, tg);
+        const double dq = rdist_d(r, t);
         if (dq < best) { best = dq; bl = r.leaf; nt = 1; }
         else if (dq == best) nt++;
     }
-    leaf = bl;
-    d = best;
-    tied = nt >= 2;
+    f.leaf = bl & 0xFFFF;
+    f.biome = (bl >> 16) & 0xFF;
+    f.d = (uint64_t)best;
+    f.tied = nt >= 2;
     return true;
 }
-__device__ __forceinline__ bool cell_search(const PWorld &W, const int64_t *tg, int &leaf, uint64_t &d, bool &tied) {
+__device__ __forceinline__ bool cell_search(const PWorld &W, const int64_t *tg, Found &f) {
     if (!W.centry) return false;
     if (W.clut) {
         bool in16 = true;
 #pragma unroll
         for (int k = 0; k < 6; k++) in16 = in16 && tg[k] >= -32768 && tg[k] <= 32767;
-        if (in16) return cell_search_lut(W, tg, leaf, d, tied);
+        if (in16) {
+            int t[6];
+#pragma unroll
+            for (int k = 0; k < 6; k++) t[k] = (int)tg[k];
+            return cell_search_lut(W, t, f);
+        }
     }
     int c[6];
 #pragma unroll
@@ -149,9 +170,11 @@ __device__ __forceinline__ bool cell_search(const PWorld &W, const int64_t *tg, 
         if (dq < best) { best = dq; bl = l; nt = 1; }
         else if (dq == best) nt++;
     }
-    leaf = bl;
-    d = best;
-    tied = nt >= 2;
+    f.leaf = bl;
+    f.biome = W.tree[bl].biome;
+    f.d = best;
+    f.tied = nt >= 2;
+    f.single = false;
     return true;
 }
 
@@ -200,39 +223,36 @@ __device__ __forceinline__ void cell_point(const PWorld &W, const Cell &c, int &
     y = (W.min_qy + s * 4 + ((r >> 2) & 3)) * 4;
     z = (c.bzq + (r & 3)) * 4;
 }
-// consume the program's values at the point cell_point gave
-template <typename VT>
-__device__ __forceinline__ void cell_step(const PWorld &W, const VT *vals, Cell &c) {
-    if (c.st == ST_END) {                          // TheEndBiomeSource (biome.c:26-45)
-        const double h = (double)vals[W.root_slot[3] * kWG];
-        c.result = h > 0.25 ? W.end_ids[1] : (h >= -0.0625 ? W.end_ids[2] : (h < -0.21875 ? W.end_ids[3] : W.end_ids[4]));
-        c.st = ST_DONE;
-        return;
-    }
-    int64_t tg[6];
+// Climate.target's quantization of the six root values (false: a value is non-finite or beyond the exact
+// range of the cast; the cell is unresolved)
+template <class Val>
+__device__ __forceinline__ bool cell_targets(Val val, int64_t *tg) {
     bool ok = true;
 #pragma unroll
     for (int q = 0; q < 6; q++) {
-        const float f = (float)vals[W.root_slot[q] * kWG] * 10000.0f;
+        const float f = (float)val(q) * 10000.0f;
         if (!(fabsf(f) < 9.0e18f)) ok = false;     // NaN / inf / beyond the exact range of the cast
         tg[q] = ok ? (int64_t)f : 0;
     }
-    if (!ok) { c.st = ST_DONE; c.result = -1; return; }
-    int l0;
-    uint64_t d;
-    bool tied = false;
-    const bool listed = cell_search(W, tg, l0, d, tied);
+    return ok;
+}
+// advance the lane's state machine with the search result of its current cell (listed: from the cell
+// table; else the tree is walked here)
+__device__ __forceinline__ void cell_advance(const PWorld &W, Cell &c, const int64_t *tg, bool listed, Found f) {
     if (!listed) {
-        l0 = rt_find(W.tree, W.ntree, tg);
-        if (l0 < 0) { c.st = ST_DONE; c.result = -1; return; }
-        d = pdist(W.tree[l0], tg);
+        f.leaf = rt_find(W.tree, W.ntree, tg);
+        if (f.leaf < 0) { c.st = ST_DONE; c.result = -1; return; }
+        f.d = pdist(W.tree[f.leaf], tg);
+        f.biome = W.tree[f.leaf].biome;
+        f.single = false;
     }
+    const int l0 = f.leaf;
     if (c.st == ST_BACK) {
         const int idx = c.idx0 - c.k;
-        if (idx == 0 || !(listed ? tied : rt_tie(W.tree, W.ntree, tg, d))) {
+        if (idx == 0 || !(listed ? f.tied : rt_tie(W.tree, W.ntree, tg, f.d))) {
             c.leaf = l0;
             c.n = c.k;
-            if (c.n == 0) { c.st = ST_DONE; c.result = W.tree[l0].biome; return; }
+            if (c.n == 0) { c.st = ST_DONE; c.result = f.biome; return; }
             c.st = ST_FWD;
             c.k = c.n - 1;                         // re-evaluate the tied cells forward
             return;
@@ -241,10 +261,27 @@ __device__ __forceinline__ void cell_step(const PWorld &W, const VT *vals, Cell 
         c.k++;
         return;
     }
-    // forward: a tied cell keeps the previous cell's leaf when it is at the least distance
-    c.leaf = pdist(W.tree[c.leaf], tg) == d ? c.leaf : l0;
+    // forward: a tied cell keeps the previous cell's leaf when it is at the least distance (a cell with
+    // one listed leaf has no other leaf at its least distance)
+    c.leaf = !f.single && pdist(W.tree[c.leaf], tg) == f.d ? c.leaf : l0;
     if (c.k == 0) { c.st = ST_DONE; c.result = W.tree[c.leaf].biome; return; }
     c.k--;
+}
+// consume the program's values at the point cell_point gave; val(q) = root q's value (0 temperature,
+// 1 vegetation, 2 continents, 3 erosion, 4 depth, 5 ridges)
+template <class Val>
+__device__ __forceinline__ void cell_step(const PWorld &W, Val val, Cell &c) {
+    if (c.st == ST_END) {                          // TheEndBiomeSource (biome.c:26-45)
+        const double h = (double)val(3);
+        c.result = h > 0.25 ? W.end_ids[1] : (h >= -0.0625 ? W.end_ids[2] : (h < -0.21875 ? W.end_ids[3] : W.end_ids[4]));
+        c.st = ST_DONE;
+        return;
+    }
+    int64_t tg[6];
+    if (!cell_targets(val, tg)) { c.st = ST_DONE; c.result = -1; return; }
+    Found f;
+    const bool listed = cell_search(W, tg, f);
+    cell_advance(W, c, tg, listed, f);
 }
 
 __device__ __forceinline__ void count_bad(int32_t *nbad, bool bad) {
@@ -252,30 +289,12 @@ __device__ __forceinline__ void count_bad(int32_t *nbad, bool bad) {
     if (m && nbad && (int)(threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) atomicAdd(nbad, (int)__popcll(m));
 }
 
-// cells of a lattice, points, or grid points.  Each lane walks its items (stride gridDim*256) and runs
-// one program evaluation per loop trip; a lane whose item finishes starts its next item in the same trip.
-template <bool NEWF, bool PL, int SPD>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NEWF && SPD <= 6 ? 5 : 1))) void k_cells(PWorld W, PJob J) {
-    typedef typename VT_of<NEWF>::T VT;
-    extern __shared__ __align__(16) uint8_t lds[];
-    constexpr int gbytes = grad_bytes<NEWF>();
-    const int pbytes = PL ? W.noct * kPermStride + rec_bytes(W) : 0;
-    dev::init_grad((typename GradOf<NEWF>::T *)lds);
-    if (PL) {
-        uint32_t *dst = (uint32_t *)(lds + gbytes);     // word w of octave o: table bytes 4(w & 63).., twice
-        const int wpo = kPermStride / 4;                 // words per staged octave
-        for (int i = threadIdx.x; i < W.noct * wpo; i += kWG) dst[i] = ((const uint32_t *)W.octs[i / wpo].p)[i & 63];
-        // records: noise and layers copied word by word, octave offsets from the head of each McgOct
-        uint32_t *rn = (uint32_t *)noise_of<NEWF, true>(W, lds, 0), *rl = (uint32_t *)layer_of<NEWF, true>(W, lds, 0);
-        const int nn = W.nnoise * (int)sizeof(McgNoise) / 4, nl = W.nlayers * (int)sizeof(McgLayer) / 4;
-        for (int i = threadIdx.x; i < nn; i += kWG) rn[i] = ((const uint32_t *)W.noise)[i];
-        for (int i = threadIdx.x; i < nl; i += kWG) rl[i] = ((const uint32_t *)W.layers)[i];
-        Off3 *ro = (Off3 *)off_of<NEWF, true>(W, lds, 0);
-        for (int i = threadIdx.x; i < W.noct; i += kWG) ro[i] = Off3{W.octs[i].xo, W.octs[i].yo, W.octs[i].zo};
-    }
-    __syncthreads();
-    VT *vals = (VT *)(lds + gbytes + pbytes) + threadIdx.x;
-    const uint8_t *lp = lds;
+// The item loop of the cell kernels: each lane walks its items (cells of a lattice, points, grid points or
+// chunk cells; stride gridDim*256) and runs one climate evaluation per loop trip, eval(c, x, y, z, same)
+// (same: the lane's previous evaluation had this x and z), which also advances the lane's state machine; a
+// lane whose item finishes starts its next item in the same trip.
+template <class Eval>
+__device__ __forceinline__ void cells_items(const PWorld &W, const PJob &J, Eval eval) {
     const int64_t stride = (int64_t)gridDim.x * kWG;
     int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x;
     Cell c;
@@ -284,8 +303,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NEWF && SPD
     bool have = false;
     // a lattice item is a column: its two quart y layers run back to back in the lane (layer 1 reuses the
     // y-independent values of layer 0); out[layer * per + column]
-    const bool lat = J.gen == GEN_LATTICE;
-    const int64_t per = (int64_t)J.nqx * J.nqz, nitems = lat ? per : J.n;
+    const bool lat = J.gen == GEN_LATTICE, lst = J.gen == GEN_LIST;
+    const int64_t per = (int64_t)J.nqx * J.nqz, nitems = lat ? per : (lst ? (int64_t)*J.nlist : J.n);
     int layer = 0;
     int lx = 0, lz = 0;                                  // x, z of the lane's last evaluation
     bool lv = false;
@@ -293,7 +312,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NEWF && SPD
         // finish the current item, start the next one (items whose biome needs no evaluation finish here)
         while (c.st == ST_DONE) {
             if (have) {
-                J.out[lat ? it + layer * per : it] = c.result < 0 ? (uint8_t)255 : (uint8_t)c.result;
+                J.out[lat ? it + layer * per : (lst ? (int64_t)J.list[it] : it)] = c.result < 0 ? (uint8_t)255 : (uint8_t)c.result;
                 if (J.nbad && c.result < 0) atomicAdd(J.nbad, 1);
                 if (lat && layer == 0) {
                     layer = 1;
@@ -304,7 +323,10 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NEWF && SPD
             }
             have = it < nitems;
             if (!have) break;
-            if (J.gen == GEN_CHUNKS) {
+            if (lst) {
+                const int64_t i = J.list[it], ly = i >= per ? 1 : 0, col = i - ly * per;
+                cell_start(W, J.q0x + (int)(col % J.nqx), J.qys[ly], J.q0z + (int)(col / J.nqx), c);
+            } else if (J.gen == GEN_CHUNKS) {
                 const int ch = (int)(it / J.cells), r = (int)(it % J.cells);
                 cell_start(W, J.xyz[2 * ch] * 4 + (r & 3), W.min_qy + (r >> 4), J.xyz[2 * ch + 1] * 4 + ((r >> 2) & 3), c);
             } else if (lat) {
@@ -327,10 +349,264 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NEWF && SPD
         if (!have) break;
         int x, y, z;
         cell_point(W, c, x, y, z);
-        eval_prog<NEWF, PL, SPD>(W, lp, vals, x, y, z, lv && x == lx && z == lz);
+        eval(c, x, y, z, lv && x == lx && z == lz);
         lx = x; lz = z; lv = true;
-        cell_step<VT>(W, vals, c);
     }
+}
+
+// The program interpreter over the cell items.
+template <bool NEWF, bool PL, int SPD>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(NEWF && SPD <= 6 ? 5 : 1))) void k_cells(PWorld W, PJob J) {
+    typedef typename VT_of<NEWF>::T VT;
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr int gbytes = grad_bytes<NEWF>();
+    const int pbytes = PL ? W.noct * kPermStride + rec_bytes(W) : 0;
+    dev::init_grad((typename GradOf<NEWF>::T *)lds);
+    if (PL) {
+        uint32_t *dst = (uint32_t *)(lds + gbytes);     // word w of octave o: table bytes 4(w & 63).., twice
+        const int wpo = kPermStride / 4;                 // words per staged octave
+        for (int i = threadIdx.x; i < W.noct * wpo; i += kWG) dst[i] = ((const uint32_t *)W.octs[i / wpo].p)[i & 63];
+        // records: noise and layers copied word by word, octave offsets from the head of each McgOct
+        uint32_t *rn = (uint32_t *)noise_of<NEWF, true>(W, lds, 0), *rl = (uint32_t *)layer_of<NEWF, true>(W, lds, 0);
+        const int nn = W.nnoise * (int)sizeof(McgNoise) / 4, nl = W.nlayers * (int)sizeof(McgLayer) / 4;
+        for (int i = threadIdx.x; i < nn; i += kWG) rn[i] = ((const uint32_t *)W.noise)[i];
+        for (int i = threadIdx.x; i < nl; i += kWG) rl[i] = ((const uint32_t *)W.layers)[i];
+        Off3 *ro = (Off3 *)off_of<NEWF, true>(W, lds, 0);
+        for (int i = threadIdx.x; i < W.noct; i += kWG) ro[i] = Off3{W.octs[i].xo, W.octs[i].yo, W.octs[i].zo};
+    }
+    __syncthreads();
+    VT *vals = (VT *)(lds + gbytes + pbytes) + threadIdx.x;
+    const uint8_t *lp = lds;
+    cells_items(W, J, [&](Cell &c, int x, int y, int z, bool same) {
+        eval_prog<NEWF, PL, SPD>(W, lp, vals, x, y, z, same);
+        cell_step(W, [&](int q) { return vals[W.root_slot[q] * kWG]; }, c);
+    });
+}
+
+// The vanilla Overworld router (mcg_router.cpp) through the engine's fused column code: the seed's 46
+// permutation tables doubled in LDS, the engine's gradient table and offset spline beside them, the five
+// noises and the offset computed once per (x, z) and kept in registers while the lane stays on that
+// column; depth per y.  MODE 1: 26.3+ (float noise), 0: 26.1/26.2 (double).  The six fields are the values
+// the program's roots would hold (as floats: the search reads them as floats), so the search and the tie
+// chain are the interpreter's.
+// vw and vslot are W.vw and W.vslot passed again as read-only, non-aliasing kernel arguments: the
+// constants and octave records are then read through the scalar cache (from the struct fields the
+// compiler cannot rule out stores to them and reads them per lane).
+template <int MODE, bool WRAP>
+__global__ __launch_bounds__(kWG) void k_cells_van(PWorld W, PJob J, const McbWorld *__restrict__ vw,
+                                                   const McbSlot *__restrict__ vslot) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_OW_OCT * OCT_BYTES];
+    __shared__ __attribute__((aligned(16))) typename dev::GT<MODE>::T G[256];
+    __shared__ dev::SplLds spl[1];
+    dev::init_grad(G);
+    dev::stage_spline(spl, vw->spline);
+    dev::stage_perm2(perm, vslot, MCB_OW_OCT);
+    __syncthreads();
+    const McbOct *oct = vslot->oct;
+    dev::Noises5<MODE> n{};
+    dev::Column col{};
+    cells_items(W, J, [&](Cell &c, int x, int y, int z, bool same) {
+        if (!same) {
+            n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, perm, G, oct, x >> 2, z >> 2);
+            dev::column_finish<MODE>(vw, spl, n.t, n.h, n.c, n.e, n.w, col);
+        }
+        const float v[6] = {(float)n.t, (float)n.h, (float)n.c, (float)n.e, dev::ow_depth_raw<MODE>(vw, col, y), (float)n.w};
+        cell_step(W, [&](int q) { return v[q]; }, c);
+    });
+}
+
+// The router's lattice, first launch: per column the climate of both quart layers, quantized as
+// Climate.target does ((long)(v * 10000.0f)), as int32 in tgt[k * per + column] for k = temperature,
+// humidity, continentalness, erosion, weirdness, depth of layer 0, depth of layer 1.  A value whose target
+// is non-finite or beyond +-32767 (outside the cell locator) is stored as INT32_MIN: the column (or that
+// layer) goes to the state machine, which applies the plugin's rules to it.
+constexpr int32_t kNoTarget = INT32_MIN;
+__device__ __forceinline__ int32_t target16(float v) {
+    const float f = v * 10000.0f;
+    return f > -32769.0f && f < 32768.0f ? (int32_t)f : kNoTarget;   // false for NaN
+}
+template <int MODE, bool WRAP>
+__global__ __launch_bounds__(kWG) void k_van_columns(PJob J, const McbWorld *__restrict__ vw, const McbSlot *__restrict__ vslot,
+                                                     int32_t *__restrict__ tgt) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_OW_OCT * OCT_BYTES];
+    __shared__ __attribute__((aligned(16))) typename dev::GT<MODE>::T G[256];
+    __shared__ dev::SplLds spl[1];
+    dev::init_grad(G);
+    dev::stage_spline(spl, vw->spline);
+    dev::stage_perm2(perm, vslot, MCB_OW_OCT);
+    __syncthreads();
+    const McbOct *oct = vslot->oct;
+    const int per = J.nqx * J.nqz;
+    for (int col = blockIdx.x * kWG + threadIdx.x; col < per; col += gridDim.x * kWG) {
+        const int qx = J.q0x + col % J.nqx, qz = J.q0z + col / J.nqx;
+        const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, perm, G, oct, qx, qz);
+        dev::Column c;
+        dev::column_finish<MODE>(vw, spl, n.t, n.h, n.c, n.e, n.w, c);
+        const int32_t t5[5] = {target16((float)n.t), target16((float)n.h), target16((float)n.c), target16((float)n.e),
+                               target16((float)n.w)};
+        const bool ok = t5[0] != kNoTarget && t5[1] != kNoTarget && t5[2] != kNoTarget && t5[3] != kNoTarget &&
+                        t5[4] != kNoTarget;
+#pragma unroll
+        for (int k = 0; k < 5; k++) tgt[(int64_t)k * per + col] = ok ? t5[k] : kNoTarget;
+        tgt[(int64_t)5 * per + col] = target16(dev::ow_depth_raw<MODE>(vw, c, J.qys[0] * 4));
+        tgt[(int64_t)6 * per + col] = target16(dev::ow_depth_raw<MODE>(vw, c, J.qys[1] * 4));
+    }
+}
+
+// Second launch: one lane per cell (i = layer * per + column), the locator staged in LDS.  A cell with one
+// listed leaf, or an untied least distance, or a tie at its chunk's first fill cell (which takes the first
+// leaf), is settled here; the others (ties to follow along the fill order, unlisted cells, targets beyond
+// the locator) are written to fix for the state machine.  Small and memory-bound: many waves per SIMD
+// hide the entry and record latencies.
+__global__ __launch_bounds__(kWG) void k_van_search(PWorld W, PJob J, const int32_t *__restrict__ tgt,
+                                                    int32_t *__restrict__ fix, int32_t *__restrict__ nfix) {
+    __shared__ uint32_t lut[6 * 256];
+    for (int i = threadIdx.x; i < 6 * 256; i += kWG) lut[i] = W.clut[i];
+    __syncthreads();
+    const int per = J.nqx * J.nqz;
+    for (int i = blockIdx.x * kWG + threadIdx.x; i < 2 * per; i += gridDim.x * kWG) {
+        const int ly = i >= per ? 1 : 0, col = i - ly * per;
+        // tree dimensions: temperature, humidity, continentalness, erosion, depth, weirdness
+        const int64_t P = per;
+        const int t[6] = {tgt[col], tgt[P + col], tgt[2 * P + col], tgt[3 * P + col], tgt[(5 + ly) * P + col], tgt[4 * P + col]};
+        int b = -1;
+        if (t[0] != kNoTarget && t[4] != kNoTarget) {
+            uint32_t idx = 0;
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                const uint32_t u = (uint32_t)(t[k] + 32768);
+                const uint32_t e = lut[k * 256 + (int)(u >> 8)];
+                idx += (e & 0xFFFFFu) + ((u & 255u) >= (e >> 20) ? (uint32_t)W.cstride[k] : 0u);
+            }
+            const uint32_t e = W.cent1[idx];
+            const int cnt = (int)(e & 0xFF);
+            if (cnt == 1) {
+                b = (int)(e >> 24);
+            } else if (cnt > 1) {
+                const PRec *R = W.crec + (e >> 8);
+                double best = __builtin_inf();
+                int bl = 0, nt = 0;
+                for (int q = 0; q < cnt; q++) {
+                    const PRec r = R[q];
+                    const double dq = rdist_d(r, t);
+                    if (dq < best) { best = dq; bl = r.leaf; nt = 1; }
+                    else if (dq == best) nt++;
+                }
+                // a tie is settled here only at the chunk's first fill cell (index 0: lowest quart layer,
+                // x and z multiples of 4), where the chain does not walk back
+                const int qx = J.q0x + col % J.nqx, qz = J.q0z + col / J.nqx;
+                if (nt < 2 || (J.qys[ly] == W.min_qy && (qx & 3) == 0 && (qz & 3) == 0)) b = (bl >> 16) & 0xFF;
+            }
+        }
+        if (b >= 0) {
+            J.out[i] = (uint8_t)b;
+        } else {
+            J.out[i] = 255;
+            fix[atomicAdd(nfix, 1)] = i;
+        }
+    }
+}
+
+// Third launch: the chain windows of the first kWinCells listed cells, one lane per (cell, step back j):
+// the cell j places before it in its chunk's fill order evaluated and searched as the state machine's trip
+// at chain position j would (targets, least distance, tie by the table or the tree).
+template <int MODE, bool WRAP>
+__global__ __launch_bounds__(kWG) void k_van_window(PWorld W, PJob J, const McbWorld *__restrict__ vw,
+                                                    const McbSlot *__restrict__ vslot, const int32_t *__restrict__ fix,
+                                                    const int32_t *__restrict__ cnt, WinCell *__restrict__ win) {
+    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_OW_OCT * OCT_BYTES];
+    __shared__ __attribute__((aligned(16))) typename dev::GT<MODE>::T G[256];
+    __shared__ dev::SplLds spl[1];
+    const int nf = min(cnt[0], kWinCells);
+    if ((int64_t)blockIdx.x * kWG >= (int64_t)nf * kWin) return;        // the whole workgroup has nothing to do
+    dev::init_grad(G);
+    dev::stage_spline(spl, vw->spline);
+    dev::stage_perm2(perm, vslot, MCB_OW_OCT);
+    __syncthreads();
+    const McbOct *oct = vslot->oct;
+    const int64_t per = (int64_t)J.nqx * J.nqz;
+    for (int w = blockIdx.x * kWG + threadIdx.x; w < nf * kWin; w += gridDim.x * kWG) {
+        const int f = w / kWin, j = w - f * kWin;
+        const int64_t i = fix[f], ly = i >= per ? 1 : 0, col = i - ly * per;
+        Cell c;
+        cell_start(W, J.q0x + (int)(col % J.nqx), J.qys[ly], J.q0z + (int)(col / J.nqx), c);
+        WinCell r;
+        r.ok = 0;
+        r.tied = r.single = 0;
+        if (c.idx0 - j >= 0) {
+            c.k = j;
+            int x, y, z;
+            cell_point(W, c, x, y, z);
+            const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, perm, G, oct, x >> 2, z >> 2);
+            dev::Column cl;
+            dev::column_finish<MODE>(vw, spl, n.t, n.h, n.c, n.e, n.w, cl);
+            const float v[6] = {(float)n.t, (float)n.h, (float)n.c, (float)n.e, dev::ow_depth_raw<MODE>(vw, cl, y), (float)n.w};
+            if (cell_targets([&](int q) { return v[q]; }, r.tg)) {
+                Found fd;
+                const bool listed = cell_search(W, r.tg, fd);
+                if (!listed) {
+                    fd.leaf = rt_find(W.tree, W.ntree, r.tg);
+                    if (fd.leaf >= 0) {
+                        fd.d = pdist(W.tree[fd.leaf], r.tg);
+                        fd.biome = W.tree[fd.leaf].biome;
+                        fd.single = false;
+                        fd.tied = rt_tie(W.tree, W.ntree, r.tg, fd.d);
+                    }
+                }
+                if (fd.leaf >= 0) {
+                    r.ok = 1;
+                    r.leaf = fd.leaf;
+                    r.biome = fd.biome;
+                    r.d = fd.d;
+                    r.tied = fd.tied;
+                    r.single = fd.single;
+                }
+            }
+        }
+        win[w] = r;
+    }
+}
+
+// Fourth launch: each listed cell's chain from its window, as the state machine walks it (back while the
+// least distance is tied and the fill index is not 0, unresolved at the cap; then forward, keeping the
+// carried leaf where it is at the cell's least distance).  Cells beyond kWinCells, and chains that run past
+// the window, go to the second list for the state machine.
+__global__ __launch_bounds__(kWG) void k_van_chain(PWorld W, PJob J, const int32_t *__restrict__ fix, int32_t *__restrict__ cnt,
+                                                   const WinCell *__restrict__ win, int32_t *__restrict__ list2) {
+    const int nf = cnt[0];
+    const int64_t per = (int64_t)J.nqx * J.nqz;
+    for (int f = blockIdx.x * kWG + threadIdx.x; f < nf; f += gridDim.x * kWG) {
+        const int32_t i = fix[f];
+        if (f >= kWinCells) { list2[atomicAdd(cnt + 1, 1)] = i; continue; }
+        const int64_t ly = i >= per ? 1 : 0, col = i - ly * per;
+        Cell c;
+        cell_start(W, J.q0x + (int)(col % J.nqx), J.qys[ly], J.q0z + (int)(col / J.nqx), c);
+        const WinCell *R = win + (int64_t)f * kWin;
+        int n = -1, result = -2;                      // -2: past the window
+        for (int k = 0; k < kWin; k++) {
+            if (!R[k].ok) { result = -1; break; }
+            if (c.idx0 - k == 0 || !R[k].tied) { n = k; break; }
+            if (k == W.chain_cap) { result = -1; break; }
+        }
+        if (n < 0) {
+            if (result == -2) list2[atomicAdd(cnt + 1, 1)] = i;
+            else J.out[i] = 255;
+            continue;
+        }
+        int leaf = R[n].leaf;
+        if (n == 0) { J.out[i] = (uint8_t)R[0].biome; continue; }
+        for (int k = n - 1; k >= 0; k--) {
+            const WinCell &r = R[k];
+            leaf = !r.single && pdist(W.tree[leaf], r.tg) == r.d ? leaf : r.leaf;
+        }
+        J.out[i] = (uint8_t)W.tree[leaf].biome;
+    }
+}
+
+// the McbOct fields derived from the offsets, for the router's octaves (once per world)
+__global__ void k_router_slot(McbSlot *s) {
+    const int o = threadIdx.x;
+    if (o < MCB_OW_OCT) s->oct[o] = dev::make_oct(s->oct[o].xo, s->oct[o].yo, s->oct[o].zo);
 }
 
 // block grid read from a lattice of cells (MULTI) or chunks (END)
@@ -397,6 +673,10 @@ static int grid_blocks(int64_t n) {
 template <bool NEWF, bool PL>
 static hipError_t launch_t(const PWorld &W, const PJob &J, int spd, int lds, hipStream_t st) {
     const int64_t items = J.gen == GEN_LATTICE ? (int64_t)J.nqx * J.nqz : J.n;     // lattice: columns
+    if (W.vw) {
+        hipLaunchKernelGGL((k_cells_van<NEWF ? 1 : 0, true>), dim3(grid_blocks(items)), dim3(kWG), 0, st, W, J, W.vw, W.vslot);
+        return hipGetLastError();
+    }
     if (spd <= 6) hipLaunchKernelGGL((k_cells<NEWF, PL, 6>), dim3(grid_blocks(items)), dim3(kWG), lds, st, W, J);
     else hipLaunchKernelGGL((k_cells<NEWF, PL, kSplineDepth>), dim3(grid_blocks(items)), dim3(kWG), lds, st, W, J);
     return hipGetLastError();
@@ -406,6 +686,47 @@ hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds
     if (J.n <= 0) return hipSuccess;
     if (newf) return perm_lds ? launch_t<true, true>(W, J, W.spl_depth, lds, st) : launch_t<true, false>(W, J, W.spl_depth, lds, st);
     return perm_lds ? launch_t<false, true>(W, J, W.spl_depth, lds, st) : launch_t<false, false>(W, J, W.spl_depth, lds, st);
+}
+
+hipError_t launch_van_lattice(const PWorld &W, const PJob &J, bool newf, int32_t *tgt, int32_t *fix, WinCell *win,
+                              int32_t *cnt, hipStream_t st) {
+    const int64_t per = (int64_t)J.nqx * J.nqz;
+    if (per <= 0) return hipSuccess;
+    if (newf)
+        hipLaunchKernelGGL((k_van_columns<1, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
+    else
+        hipLaunchKernelGGL((k_van_columns<0, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
+    if (hipError_t e = hipGetLastError()) return e;
+    hipLaunchKernelGGL(k_van_search, dim3(grid_blocks(2 * per)), dim3(kWG), 0, st, W, J, (const int32_t *)tgt, fix, cnt);
+    if (hipError_t e = hipGetLastError()) return e;
+    // the listed cells' count stays on the device: the window and chain launches cover the most they can
+    // hold and return early past the count
+    const int wblocks = grid_blocks((int64_t)kWinCells * kWin);
+    if (newf)
+        hipLaunchKernelGGL((k_van_window<1, true>), dim3(wblocks), dim3(kWG), 0, st, W, J, W.vw, W.vslot, (const int32_t *)fix,
+                           (const int32_t *)cnt, win);
+    else
+        hipLaunchKernelGGL((k_van_window<0, true>), dim3(wblocks), dim3(kWG), 0, st, W, J, W.vw, W.vslot, (const int32_t *)fix,
+                           (const int32_t *)cnt, win);
+    if (hipError_t e = hipGetLastError()) return e;
+    int32_t *list2 = fix + 2 * per;
+    hipLaunchKernelGGL(k_van_chain, dim3(grid_blocks(kWinCells)), dim3(kWG), 0, st, W, J, (const int32_t *)fix, cnt,
+                       (const WinCell *)win, list2);
+    if (hipError_t e = hipGetLastError()) return e;
+    PJob L = J;
+    L.gen = GEN_LIST;
+    L.list = list2;
+    L.nlist = cnt + 1;
+    if (newf)
+        hipLaunchKernelGGL((k_cells_van<1, true>), dim3(64), dim3(kWG), 0, st, W, L, W.vw, W.vslot);
+    else
+        hipLaunchKernelGGL((k_cells_van<0, true>), dim3(64), dim3(kWG), 0, st, W, L, W.vw, W.vslot);
+    return hipGetLastError();
+}
+
+hipError_t finish_router_slot(McbSlot *d_slot, hipStream_t st) {
+    hipLaunchKernelGGL(k_router_slot, dim3(1), dim3(64), 0, st, d_slot);
+    return hipGetLastError();
 }
 
 hipError_t launch_zoom_lattice(const PWorld &W, const PJob &L, int x0, int z0, int nx, int nz, int step, int y, uint8_t *out,

@@ -121,6 +121,8 @@ def lib():
         L.mcgpu_biome_free.argtypes = [P]
         L.mcgpu_biome_grid.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, u8p, ip]
         L.mcgpu_biome_points.argtypes = [P, C.c_int, ip, u8p, ip]
+        L.mcb_plugin_router.argtypes = [P]
+        L.mcb_plugin_router_world.argtypes = [P]
         sp = C.POINTER(C.c_size_t)
         L.mcgpu_terrain_new.restype = P
         L.mcgpu_terrain_new.argtypes = [P, C.c_int, C.c_char_p, C.c_size_t]
@@ -147,6 +149,13 @@ class PluginError(RuntimeError):
     pass
 
 
+def router_world(world):
+    """Host-only check: would mcgpu_biome_new run this McgBiomeWorld (ctypes instance or address) through
+    the engine's column code?"""
+    addr = world if isinstance(world, int) else C.addressof(world)
+    return bool(lib().mcb_plugin_router_world(C.c_void_p(addr)))
+
+
 class BiomeWorld:
     """A world created with mcgpu_biome_new from a McgBiomeWorld (a ctypes instance or its address)."""
 
@@ -165,6 +174,11 @@ class BiomeWorld:
         if rc:
             raise PluginError(last_error())
         return out.reshape(nz, nx), bad.value
+
+    @property
+    def router(self):
+        """True when the world's program runs through the engine's fused column code (vanilla router)."""
+        return bool(lib().mcb_plugin_router(self.h))
 
     def points(self, xyz):
         xyz = np.ascontiguousarray(xyz, np.int32).reshape(-1, 3)

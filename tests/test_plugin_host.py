@@ -10,6 +10,7 @@ import pytest
 
 import mcgprog
 import mcgpu
+import oracle
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mcgpu.h")
 
@@ -123,3 +124,96 @@ def test_refuses_zero_gradient_period(van):
     P = mcgprog.Prog(False, van.prog)
     g = P.node(mcgpu.K["GRAD_REPEAT"], i0=1, i1=0, i2=0, f0=0.0, f1=1.0)
     _refused(mcgprog.world(P.build([g] * 6), van.world), "gradient period 0")
+
+
+# ---- recognition of the vanilla Overworld climate router (host only, no device call) ----
+def _vanilla_world(version, preset, seed=11):
+    o = oracle.Oracle(version, "overworld", preset)
+    ref = oracle.McgWorldRef.vanilla(o, seed)
+    W = C.cast(C.c_void_p(ref.addr), C.POINTER(mcgpu.McgBiomeWorld)).contents
+    return ref, W
+
+
+def _nodes_of(W, kind):
+    P = W.prog.contents
+    return [i for i in range(P.nnodes) if P.nodes[i].k == kind]
+
+
+@pytest.mark.parametrize("version", ["26.1", "26.2", "26.3"])
+@pytest.mark.parametrize("preset", ["overworld", "large_biomes", "amplified"])
+def test_router_recognises_vanilla_overworld(version, preset):
+    ref, W = _vanilla_world(version, preset)
+    assert mcgpu.router_world(ref.addr)
+
+
+@pytest.mark.parametrize("dim", ["nether", "end"])
+def test_router_not_for_other_dimensions(dim):
+    V = mcgprog.vanilla("26.3", dim, 3)
+    assert not mcgpu.router_world(V.addr)
+
+
+def _perturb_new(W, what):
+    """One change to a 26.3 vanilla program that the engine's column code does not compute."""
+    P = W.prog.contents
+    K = mcgpu.K
+    if what == "shift_scale":
+        i = [i for i in _nodes_of(W, K["CMUL"]) if P.nodes[i].f0 == 4.0][0]
+        P.nodes[i].f0 = 4.5
+    elif what == "alpha":
+        P.nodes[_nodes_of(W, K["CTX_ALPHA"])[0]].f0 = 0.5
+    elif what == "second_freq":
+        L = P.layers[P.noise[3].l0 + 3]                 # continentalness, level 1, second stack
+        L.a = L.a * (1.0 + 2.0 ** -40)
+    elif what == "smeared":
+        P.layers[P.noise[1].l0].kind = 1
+    elif what == "y_scale":
+        P.nodes[_nodes_of(W, K["NOISE_XZ"])[2]].d1 = 0.5
+    elif what == "slice":
+        n = P.nodes[_nodes_of(W, K["NOISE_XZ"])[0]]
+        n.cm, n.cx = 1, 0
+    elif what == "levels":
+        P.noise[4].n0 -= 2                               # erosion with one level fewer
+    elif what == "depth_axis":
+        P.nodes[_nodes_of(W, K["GRAD_CLAMP"])[0]].i0 = 0
+    elif what == "ridge_root":
+        P.root[5] = P.root[2]                            # ridges = continentalness
+    else:
+        raise ValueError(what)
+
+
+@pytest.mark.parametrize("what", ["shift_scale", "alpha", "second_freq", "smeared", "y_scale", "slice", "levels",
+                                  "depth_axis", "ridge_root"])
+def test_router_refuses_other_graphs(what):
+    ref, W = _vanilla_world("26.3", "overworld")
+    assert mcgpu.router_world(ref.addr)
+    _perturb_new(W, what)
+    assert not mcgpu.router_world(ref.addr)
+
+
+def test_router_refuses_other_old_graphs():
+    ref, W = _vanilla_world("26.1", "overworld")
+    P = W.prog.contents
+    O = mcgpu.O
+    i = _nodes_of(W, O["SHIFTED_NOISE"])[1]
+    P.nodes[i].b = _nodes_of(W, O["SHIFT_A"])[0]         # y shifted by shift_x instead of 0
+    assert not mcgpu.router_world(ref.addr)
+    ref, W = _vanilla_world("26.2", "overworld")
+    P = W.prog.contents
+    P.layers[P.noise[2].l1 + 1].b *= 2.0                 # second PerlinNoise's value factor differs
+    assert not mcgpu.router_world(ref.addr)
+
+
+def test_router_maps_spline_coordinates():
+    """The spline's coordinate list permuted (and every spline node's coordinate renumbered to match) is
+    the same function; the recognition follows the list instead of assuming its order."""
+    ref, W = _vanilla_world("26.3", "overworld")
+    P = W.prog.contents
+    s = P.nodes[_nodes_of(W, mcgpu.K["SPLINE"])[0]]
+    perm = [2, 0, 3, 1]                                  # new position of each old coordinate
+    old = [P.arr[s.arr0 + j] for j in range(4)]
+    for j in range(4):
+        P.arr[s.arr0 + perm[j]] = old[j]
+    for i in range(P.nsp):
+        if not P.sp[i].is_const:
+            P.sp[i].coord = perm[P.sp[i].coord]
+    assert mcgpu.router_world(ref.addr)
