@@ -703,6 +703,24 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
 
 int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// A lattice or points job of a world with the vanilla router through its launches (launch_van_cells);
+// false when the world has no router or its cell table no locator, or the job's cells exceed 2^27 (the
+// launches keep up to 44 bytes per cell) or its buffers cannot be had: the caller runs the per-lane kernel
+// instead.
+bool van_run(GBiome *g, const PJob &J) {
+    const int64_t nc = van_cells(J);
+    if (!g->router || !g->W.clut || !g->W.cent1 || nc > ((int64_t)1 << 27)) return false;
+    const bool grown = grow(&g->d_tgt, &g->tgt_cap, (size_t)van_planes(J) * (size_t)nc) &&
+                       grow(&g->d_fix, &g->fix_cap, 2 * (size_t)nc) && grow(&g->d_win, &g->win_cap, (size_t)kWinCells * kWin);
+    hipError_t e = grown ? hipMemsetAsync(g->d_bad + 1, 0, 2 * sizeof(int32_t), g->st) : hipErrorOutOfMemory;
+    if (e == hipSuccess) e = launch_van_cells(g->W, J, g->newf, g->d_tgt, g->d_fix, g->d_win, g->d_bad + 1, g->st);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;                          // the per-lane kernel runs the job instead
+    }
+    return true;
+}
+
 bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsigned char *out, int *nunres) {
     const int64_t n = (int64_t)nx * nz;
     if (!grow(&g->d_out, &g->out_cap, (size_t)n)) return false;
@@ -728,24 +746,14 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
         J.qys[1] = clampi(py + 1, lo, hi);
         J.out = g->d_lat;
         J.nbad = nullptr;                    // counted per output point by the zoom pass
-        // the router's three launches keep 36 bytes per column (targets and fix list): up to 2^27 columns
-        if (g->router && g->W.clut && g->W.cent1 && nax * naz <= ((int64_t)1 << 27)) {
-            const size_t per = (size_t)(nax * naz);
-            if (!grow(&g->d_tgt, &g->tgt_cap, 7 * per) || !grow(&g->d_fix, &g->fix_cap, 4 * per) ||
-                !grow(&g->d_win, &g->win_cap, (size_t)kWinCells * kWin))
-                return false;
-            HCHK(hipMemsetAsync(g->d_bad + 1, 0, 2 * sizeof(int32_t), g->st));
-            HCHK(launch_van_lattice(g->W, J, g->newf, g->d_tgt, g->d_fix, g->d_win, g->d_bad + 1, g->st));
-        } else {
-            HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
-        }
+        if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
         HCHK(launch_zoom_lattice(g->W, J, x0, z0, nx, nz, step, y, g->d_out, g->d_bad, g->st));
     } else {
         J.gen = GEN_GRIDPTS;
         J.n = n;
         J.x0 = x0; J.z0 = z0; J.nx = nx; J.step = step; J.y = y;
         J.out = g->d_out;
-        HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
     }
     int32_t bad = 0;
     HCHK(hipMemcpyAsync(out, g->d_out, (size_t)n, hipMemcpyDeviceToHost, g->st));
@@ -770,7 +778,7 @@ bool run_points(GBiome *g, int n, const int *xyz, unsigned char *out, int *nunre
     J.xyz = g->d_xyz;
     J.out = g->d_out;
     J.nbad = g->d_bad;
-    HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
     int32_t bad = 0;
     HCHK(hipMemcpyAsync(out, g->d_out, (size_t)n, hipMemcpyDeviceToHost, g->st));
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));

@@ -118,10 +118,13 @@ struct PJob {
     int32_t cells;                 // chunks: quart cells per chunk (16 per quart layer), item = chunk * cells + cell
     uint8_t *out;
     int32_t *nbad;
-    // list: the lattice cells list[0 .. *nlist) (cell i = layer * nqx * nqz + column, as the lattice numbers
-    // them), each written to out[i]
+    // list: the cells list[0 .. *nlist), each written to out[i]: lattice cells (i = layer * nqx * nqz +
+    // column, as the lattice numbers them) when cq is null, else cell i at quart (cq[i], cq[nc + i],
+    // cq[2 * nc + i]) with nc = ncells
     const int32_t *list;
     const int32_t *nlist;
+    const int32_t *cq;
+    int64_t ncells;
 };
 
 // Whole chunks of a world description in libmcgen's stored form (world_chunk_biomes,
@@ -154,14 +157,15 @@ hipError_t finish_router_slot(McbSlot *d_slot, hipStream_t st);
 
 // launchers (mcg_plugin.hip); lds = bytes of dynamic LDS, perm_lds = permutation tables staged in LDS
 hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st);
-// The lattice of a world with the vanilla router (W.vw): the quantized climate of every column into tgt
-// (7 * columns int32); the table search of every cell at high occupancy, listing the cells it cannot settle
-// (ties to follow along the fill order, targets beyond the table) in fix / cnt[0]; for the first
-// kWinCells of those, every cell of their chain window (the cell and kWin - 1 cells back in fill order)
-// evaluated in parallel into win, then each chain resolved from its window; chains the window cannot
-// settle (uncapped ones that run past it) and listed cells beyond kWinCells through the per-lane state
-// machine (list fix + 2 * columns, length cnt[1]).  tgt holds 7 * nqx * nqz values, fix 4 * nqx * nqz, win
-// kWinCells * kWin records; cnt[0..1] must be zero on entry.
+// Cells of a world with the vanilla router (W.vw) in launches: the quantized climate of every cell into
+// tgt (planes of ncells int32: the six tree targets, then for points the cell's quart x, y, z); the table
+// search of every cell at high occupancy, listing the cells it cannot settle (ties to follow along the fill
+// order, targets beyond the table) in fix / cnt[0]; for the first kWinCells of those, every cell of their
+// chain window (the cell and kWin - 1 cells back in fill order) evaluated in parallel into win, then each
+// chain resolved from its window; chains the window cannot settle (uncapped ones that run past it) and
+// listed cells beyond kWinCells through the per-lane state machine (list fix + ncells, length cnt[1]).
+// J.gen: GEN_LATTICE (ncells = 2 * nqx * nqz; tgt holds 6 planes), GEN_POINTS or GEN_GRIDPTS (ncells =
+// J.n; 9 planes).  fix holds 2 * ncells, win kWinCells * kWin records; cnt[0..1] must be zero on entry.
 constexpr int kWin = kChainCap + 1;
 constexpr int kWinCells = 32768;
 struct WinCell {
@@ -172,8 +176,10 @@ struct WinCell {
     int32_t pad2;
 };
 static_assert(sizeof(WinCell) == 72, "WinCell layout");
-hipError_t launch_van_lattice(const PWorld &W, const PJob &J, bool newf, int32_t *tgt, int32_t *fix, WinCell *win,
-                              int32_t *cnt, hipStream_t st);
+int64_t van_cells(const PJob &J);
+int van_planes(const PJob &J);
+hipError_t launch_van_cells(const PWorld &W, const PJob &J, bool newf, int32_t *tgt, int32_t *fix, WinCell *win, int32_t *cnt,
+                            hipStream_t st);
 // block grid from a lattice of cells (MULTI) or of chunks (END): out[iz*nx+ix]
 hipError_t launch_zoom_lattice(const PWorld &W, const PJob &L, int x0, int z0, int nx, int nz, int step, int y,
                                uint8_t *out, int32_t *nbad, hipStream_t st);

@@ -289,6 +289,19 @@ __device__ __forceinline__ void count_bad(int32_t *nbad, bool bad) {
     if (m && nbad && (int)(threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) atomicAdd(nbad, (int)__popcll(m));
 }
 
+// Quart coordinates of cell i of a router job: a lattice cell (layer * nqx * nqz + column), or a cell whose
+// coordinates the first launch stored in the planes cq
+__device__ __forceinline__ void list_cell(const PJob &J, int64_t i, int *q) {
+    if (J.cq) {
+        q[0] = J.cq[i]; q[1] = J.cq[J.ncells + i]; q[2] = J.cq[2 * J.ncells + i];
+        return;
+    }
+    const int64_t per = (int64_t)J.nqx * J.nqz, ly = i >= per ? 1 : 0, col = i - ly * per;
+    q[0] = J.q0x + (int)(col % J.nqx);
+    q[1] = J.qys[ly];
+    q[2] = J.q0z + (int)(col / J.nqx);
+}
+
 // The item loop of the cell kernels: each lane walks its items (cells of a lattice, points, grid points or
 // chunk cells; stride gridDim*256) and runs one climate evaluation per loop trip, eval(c, x, y, z, same)
 // (same: the lane's previous evaluation had this x and z), which also advances the lane's state machine; a
@@ -324,8 +337,10 @@ __device__ __forceinline__ void cells_items(const PWorld &W, const PJob &J, Eval
             have = it < nitems;
             if (!have) break;
             if (lst) {
-                const int64_t i = J.list[it], ly = i >= per ? 1 : 0, col = i - ly * per;
-                cell_start(W, J.q0x + (int)(col % J.nqx), J.qys[ly], J.q0z + (int)(col / J.nqx), c);
+                const int64_t i = J.list[it];
+                int q[3];
+                list_cell(J, i, q);
+                cell_start(W, q[0], q[1], q[2], c);
             } else if (J.gen == GEN_CHUNKS) {
                 const int ch = (int)(it / J.cells), r = (int)(it % J.cells);
                 cell_start(W, J.xyz[2 * ch] * 4 + (r & 3), W.min_qy + (r >> 4), J.xyz[2 * ch + 1] * 4 + ((r >> 2) & 3), c);
@@ -415,60 +430,104 @@ __global__ __launch_bounds__(kWG) void k_cells_van(PWorld W, PJob J, const McbWo
     });
 }
 
-// The router's lattice, first launch: per column the climate of both quart layers, quantized as
-// Climate.target does ((long)(v * 10000.0f)), as int32 in tgt[k * per + column] for k = temperature,
-// humidity, continentalness, erosion, weirdness, depth of layer 0, depth of layer 1.  A value whose target
-// is non-finite or beyond +-32767 (outside the cell locator) is stored as INT32_MIN: the column (or that
-// layer) goes to the state machine, which applies the plugin's rules to it.
+// First launch, lattice: per column the climate of both quart layers, quantized as Climate.target does
+// ((long)(v * 10000.0f)), as int32 in tgt[k * ncells + cell] for the tree dimensions k = temperature,
+// humidity, continentalness, erosion, depth, weirdness (cell = layer * per + column).  A target that is
+// non-finite or beyond +-32767 (outside the cell locator) is stored as kNoTarget: the cell goes to the
+// chain windows, which apply the plugin's rules to it.
 constexpr int32_t kNoTarget = INT32_MIN;
 __device__ __forceinline__ int32_t target16(float v) {
     const float f = v * 10000.0f;
     return f > -32769.0f && f < 32768.0f ? (int32_t)f : kNoTarget;   // false for NaN
 }
+// the router's LDS: doubled permutation tables, gradient table, offset spline
+template <int MODE>
+struct VanLds {
+    uint8_t perm[MCB_OW_OCT * OCT_BYTES];
+    typename dev::GT<MODE>::T G[256];
+    dev::SplLds spl;
+};
+template <int MODE>
+__device__ __forceinline__ void van_stage(VanLds<MODE> &L, const McbWorld *__restrict__ vw, const McbSlot *__restrict__ vslot) {
+    dev::init_grad(L.G);
+    dev::stage_spline(&L.spl, vw->spline);
+    dev::stage_perm2(L.perm, vslot, MCB_OW_OCT);
+    __syncthreads();
+}
+__device__ __forceinline__ void put_targets(int32_t *__restrict__ tgt, int64_t nc, int64_t cell, const int32_t *t5, int32_t dep) {
+    const bool ok = t5[0] != kNoTarget && t5[1] != kNoTarget && t5[2] != kNoTarget && t5[3] != kNoTarget && t5[4] != kNoTarget;
+    tgt[cell] = ok ? t5[0] : kNoTarget;
+    tgt[nc + cell] = t5[1];
+    tgt[2 * nc + cell] = t5[2];
+    tgt[3 * nc + cell] = t5[3];
+    tgt[4 * nc + cell] = dep;
+    tgt[5 * nc + cell] = t5[4];
+}
 template <int MODE, bool WRAP>
 __global__ __launch_bounds__(kWG) void k_van_columns(PJob J, const McbWorld *__restrict__ vw, const McbSlot *__restrict__ vslot,
                                                      int32_t *__restrict__ tgt) {
-    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_OW_OCT * OCT_BYTES];
-    __shared__ __attribute__((aligned(16))) typename dev::GT<MODE>::T G[256];
-    __shared__ dev::SplLds spl[1];
-    dev::init_grad(G);
-    dev::stage_spline(spl, vw->spline);
-    dev::stage_perm2(perm, vslot, MCB_OW_OCT);
-    __syncthreads();
+    __shared__ __attribute__((aligned(16))) VanLds<MODE> L;
+    van_stage<MODE>(L, vw, vslot);
     const McbOct *oct = vslot->oct;
     const int per = J.nqx * J.nqz;
+    const int64_t nc = 2 * (int64_t)per;
     for (int col = blockIdx.x * kWG + threadIdx.x; col < per; col += gridDim.x * kWG) {
         const int qx = J.q0x + col % J.nqx, qz = J.q0z + col / J.nqx;
-        const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, perm, G, oct, qx, qz);
+        const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, L.perm, L.G, oct, qx, qz);
         dev::Column c;
-        dev::column_finish<MODE>(vw, spl, n.t, n.h, n.c, n.e, n.w, c);
+        dev::column_finish<MODE>(vw, &L.spl, n.t, n.h, n.c, n.e, n.w, c);
         const int32_t t5[5] = {target16((float)n.t), target16((float)n.h), target16((float)n.c), target16((float)n.e),
                                target16((float)n.w)};
-        const bool ok = t5[0] != kNoTarget && t5[1] != kNoTarget && t5[2] != kNoTarget && t5[3] != kNoTarget &&
-                        t5[4] != kNoTarget;
-#pragma unroll
-        for (int k = 0; k < 5; k++) tgt[(int64_t)k * per + col] = ok ? t5[k] : kNoTarget;
-        tgt[(int64_t)5 * per + col] = target16(dev::ow_depth_raw<MODE>(vw, c, J.qys[0] * 4));
-        tgt[(int64_t)6 * per + col] = target16(dev::ow_depth_raw<MODE>(vw, c, J.qys[1] * 4));
+        put_targets(tgt, nc, col, t5, target16(dev::ow_depth_raw<MODE>(vw, c, J.qys[0] * 4)));
+        put_targets(tgt, nc, per + col, t5, target16(dev::ow_depth_raw<MODE>(vw, c, J.qys[1] * 4)));
+    }
+}
+// First launch, points (GEN_POINTS: blocks xyz; GEN_GRIDPTS: the grid's blocks): BiomeManager zoom to a
+// quart cell, its y clamped into the world, then that cell's climate; the six targets as above and the
+// cell's quart x, y, z in planes 6-8.
+template <int MODE, bool WRAP>
+__global__ __launch_bounds__(kWG) void k_van_points(PWorld W, PJob J, const McbWorld *__restrict__ vw,
+                                                    const McbSlot *__restrict__ vslot, int32_t *__restrict__ tgt) {
+    __shared__ __attribute__((aligned(16))) VanLds<MODE> L;
+    van_stage<MODE>(L, vw, vslot);
+    const McbOct *oct = vslot->oct;
+    const int64_t nc = J.n;
+    for (int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x; it < nc; it += (int64_t)gridDim.x * kWG) {
+        int x, y, z;
+        if (J.gen == GEN_POINTS) {
+            x = J.xyz[3 * it]; y = J.xyz[3 * it + 1]; z = J.xyz[3 * it + 2];
+        } else {
+            x = J.x0 + (int)(it % J.nx) * J.step; y = J.y; z = J.z0 + (int)(it / J.nx) * J.step;
+        }
+        int q[3];
+        dev::zoom_quart(W.zoom_seed, x, y, z, q);
+        const int qy = clamp_qy(W, q[1]);
+        const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, L.perm, L.G, oct, q[0], q[2]);
+        dev::Column c;
+        dev::column_finish<MODE>(vw, &L.spl, n.t, n.h, n.c, n.e, n.w, c);
+        const int32_t t5[5] = {target16((float)n.t), target16((float)n.h), target16((float)n.c), target16((float)n.e),
+                               target16((float)n.w)};
+        put_targets(tgt, nc, it, t5, target16(dev::ow_depth_raw<MODE>(vw, c, qy * 4)));
+        tgt[6 * nc + it] = q[0];
+        tgt[7 * nc + it] = qy;
+        tgt[8 * nc + it] = q[2];
     }
 }
 
-// Second launch: one lane per cell (i = layer * per + column), the locator staged in LDS.  A cell with one
-// listed leaf, or an untied least distance, or a tie at its chunk's first fill cell (which takes the first
-// leaf), is settled here; the others (ties to follow along the fill order, unlisted cells, targets beyond
-// the locator) are written to fix for the state machine.  Small and memory-bound: many waves per SIMD
-// hide the entry and record latencies.
+// Second launch: one lane per cell, the locator staged in LDS.  A cell with one listed leaf, or an untied
+// least distance, or a tie at its chunk's first fill cell (which takes the first leaf), is settled here;
+// the others (ties to follow along the fill order, unlisted cells, targets beyond the locator) are written
+// to fix for the chain windows.  Small and memory-bound: many waves per SIMD hide the entry and record
+// latencies.
 __global__ __launch_bounds__(kWG) void k_van_search(PWorld W, PJob J, const int32_t *__restrict__ tgt,
-                                                    int32_t *__restrict__ fix, int32_t *__restrict__ nfix) {
+                                                    int32_t *__restrict__ fix, int32_t *__restrict__ cnt) {
     __shared__ uint32_t lut[6 * 256];
     for (int i = threadIdx.x; i < 6 * 256; i += kWG) lut[i] = W.clut[i];
     __syncthreads();
-    const int per = J.nqx * J.nqz;
-    for (int i = blockIdx.x * kWG + threadIdx.x; i < 2 * per; i += gridDim.x * kWG) {
-        const int ly = i >= per ? 1 : 0, col = i - ly * per;
+    const int64_t nc = J.ncells;
+    for (int64_t i = (int64_t)blockIdx.x * kWG + threadIdx.x; i < nc; i += (int64_t)gridDim.x * kWG) {
         // tree dimensions: temperature, humidity, continentalness, erosion, depth, weirdness
-        const int64_t P = per;
-        const int t[6] = {tgt[col], tgt[P + col], tgt[2 * P + col], tgt[3 * P + col], tgt[(5 + ly) * P + col], tgt[4 * P + col]};
+        const int t[6] = {tgt[i], tgt[nc + i], tgt[2 * nc + i], tgt[3 * nc + i], tgt[4 * nc + i], tgt[5 * nc + i]};
         int b = -1;
         if (t[0] != kNoTarget && t[4] != kNoTarget) {
             uint32_t idx = 0;
@@ -479,14 +538,14 @@ __global__ __launch_bounds__(kWG) void k_van_search(PWorld W, PJob J, const int3
                 idx += (e & 0xFFFFFu) + ((u & 255u) >= (e >> 20) ? (uint32_t)W.cstride[k] : 0u);
             }
             const uint32_t e = W.cent1[idx];
-            const int cnt = (int)(e & 0xFF);
-            if (cnt == 1) {
+            const int cnt1 = (int)(e & 0xFF);
+            if (cnt1 == 1) {
                 b = (int)(e >> 24);
-            } else if (cnt > 1) {
+            } else if (cnt1 > 1) {
                 const PRec *R = W.crec + (e >> 8);
                 double best = __builtin_inf();
                 int bl = 0, nt = 0;
-                for (int q = 0; q < cnt; q++) {
+                for (int q = 0; q < cnt1; q++) {
                     const PRec r = R[q];
                     const double dq = rdist_d(r, t);
                     if (dq < best) { best = dq; bl = r.leaf; nt = 1; }
@@ -494,15 +553,19 @@ __global__ __launch_bounds__(kWG) void k_van_search(PWorld W, PJob J, const int3
                 }
                 // a tie is settled here only at the chunk's first fill cell (index 0: lowest quart layer,
                 // x and z multiples of 4), where the chain does not walk back
-                const int qx = J.q0x + col % J.nqx, qz = J.q0z + col / J.nqx;
-                if (nt < 2 || (J.qys[ly] == W.min_qy && (qx & 3) == 0 && (qz & 3) == 0)) b = (bl >> 16) & 0xFF;
+                if (nt >= 2) {
+                    int q[3];
+                    list_cell(J, i, q);
+                    if (q[1] == W.min_qy && (q[0] & 3) == 0 && (q[2] & 3) == 0) nt = 1;
+                }
+                if (nt < 2) b = (bl >> 16) & 0xFF;
             }
         }
         if (b >= 0) {
             J.out[i] = (uint8_t)b;
         } else {
             J.out[i] = 255;
-            fix[atomicAdd(nfix, 1)] = i;
+            fix[atomicAdd(cnt, 1)] = (int32_t)i;
         }
     }
 }
@@ -514,22 +577,17 @@ template <int MODE, bool WRAP>
 __global__ __launch_bounds__(kWG) void k_van_window(PWorld W, PJob J, const McbWorld *__restrict__ vw,
                                                     const McbSlot *__restrict__ vslot, const int32_t *__restrict__ fix,
                                                     const int32_t *__restrict__ cnt, WinCell *__restrict__ win) {
-    __shared__ __attribute__((aligned(16))) uint8_t perm[MCB_OW_OCT * OCT_BYTES];
-    __shared__ __attribute__((aligned(16))) typename dev::GT<MODE>::T G[256];
-    __shared__ dev::SplLds spl[1];
+    __shared__ __attribute__((aligned(16))) VanLds<MODE> L;
     const int nf = min(cnt[0], kWinCells);
     if ((int64_t)blockIdx.x * kWG >= (int64_t)nf * kWin) return;        // the whole workgroup has nothing to do
-    dev::init_grad(G);
-    dev::stage_spline(spl, vw->spline);
-    dev::stage_perm2(perm, vslot, MCB_OW_OCT);
-    __syncthreads();
+    van_stage<MODE>(L, vw, vslot);
     const McbOct *oct = vslot->oct;
-    const int64_t per = (int64_t)J.nqx * J.nqz;
     for (int w = blockIdx.x * kWG + threadIdx.x; w < nf * kWin; w += gridDim.x * kWG) {
         const int f = w / kWin, j = w - f * kWin;
-        const int64_t i = fix[f], ly = i >= per ? 1 : 0, col = i - ly * per;
+        int q[3];
+        list_cell(J, fix[f], q);
         Cell c;
-        cell_start(W, J.q0x + (int)(col % J.nqx), J.qys[ly], J.q0z + (int)(col / J.nqx), c);
+        cell_start(W, q[0], q[1], q[2], c);
         WinCell r;
         r.ok = 0;
         r.tied = r.single = 0;
@@ -537,11 +595,11 @@ __global__ __launch_bounds__(kWG) void k_van_window(PWorld W, PJob J, const McbW
             c.k = j;
             int x, y, z;
             cell_point(W, c, x, y, z);
-            const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, perm, G, oct, x >> 2, z >> 2);
+            const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, L.perm, L.G, oct, x >> 2, z >> 2);
             dev::Column cl;
-            dev::column_finish<MODE>(vw, spl, n.t, n.h, n.c, n.e, n.w, cl);
+            dev::column_finish<MODE>(vw, &L.spl, n.t, n.h, n.c, n.e, n.w, cl);
             const float v[6] = {(float)n.t, (float)n.h, (float)n.c, (float)n.e, dev::ow_depth_raw<MODE>(vw, cl, y), (float)n.w};
-            if (cell_targets([&](int q) { return v[q]; }, r.tg)) {
+            if (cell_targets([&](int k) { return v[k]; }, r.tg)) {
                 Found fd;
                 const bool listed = cell_search(W, r.tg, fd);
                 if (!listed) {
@@ -574,13 +632,13 @@ __global__ __launch_bounds__(kWG) void k_van_window(PWorld W, PJob J, const McbW
 __global__ __launch_bounds__(kWG) void k_van_chain(PWorld W, PJob J, const int32_t *__restrict__ fix, int32_t *__restrict__ cnt,
                                                    const WinCell *__restrict__ win, int32_t *__restrict__ list2) {
     const int nf = cnt[0];
-    const int64_t per = (int64_t)J.nqx * J.nqz;
     for (int f = blockIdx.x * kWG + threadIdx.x; f < nf; f += gridDim.x * kWG) {
         const int32_t i = fix[f];
         if (f >= kWinCells) { list2[atomicAdd(cnt + 1, 1)] = i; continue; }
-        const int64_t ly = i >= per ? 1 : 0, col = i - ly * per;
+        int q[3];
+        list_cell(J, i, q);
         Cell c;
-        cell_start(W, J.q0x + (int)(col % J.nqx), J.qys[ly], J.q0z + (int)(col / J.nqx), c);
+        cell_start(W, q[0], q[1], q[2], c);
         const WinCell *R = win + (int64_t)f * kWin;
         int n = -1, result = -2;                      // -2: past the window
         for (int k = 0; k < kWin; k++) {
@@ -589,8 +647,12 @@ __global__ __launch_bounds__(kWG) void k_van_chain(PWorld W, PJob J, const int32
             if (k == W.chain_cap) { result = -1; break; }
         }
         if (n < 0) {
-            if (result == -2) list2[atomicAdd(cnt + 1, 1)] = i;
-            else J.out[i] = 255;
+            if (result == -2) {
+                list2[atomicAdd(cnt + 1, 1)] = i;
+            } else {
+                J.out[i] = 255;
+                if (J.nbad) atomicAdd(J.nbad, 1);
+            }
             continue;
         }
         int leaf = R[n].leaf;
@@ -688,16 +750,26 @@ hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds
     return perm_lds ? launch_t<false, true>(W, J, W.spl_depth, lds, st) : launch_t<false, false>(W, J, W.spl_depth, lds, st);
 }
 
-hipError_t launch_van_lattice(const PWorld &W, const PJob &J, bool newf, int32_t *tgt, int32_t *fix, WinCell *win,
-                              int32_t *cnt, hipStream_t st) {
-    const int64_t per = (int64_t)J.nqx * J.nqz;
-    if (per <= 0) return hipSuccess;
-    if (newf)
-        hipLaunchKernelGGL((k_van_columns<1, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
-    else
-        hipLaunchKernelGGL((k_van_columns<0, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
+int64_t van_cells(const PJob &J) { return J.gen == GEN_LATTICE ? 2 * (int64_t)J.nqx * J.nqz : J.n; }
+int van_planes(const PJob &J) { return J.gen == GEN_LATTICE ? 6 : 9; }
+
+hipError_t launch_van_cells(const PWorld &W, const PJob &J0, bool newf, int32_t *tgt, int32_t *fix, WinCell *win, int32_t *cnt,
+                            hipStream_t st) {
+    PJob J = J0;
+    J.ncells = van_cells(J0);
+    if (J.ncells <= 0) return hipSuccess;
+    const bool lat = J.gen == GEN_LATTICE;
+    J.cq = lat ? nullptr : tgt + 6 * J.ncells;
+    if (lat) {
+        const int64_t per = J.ncells / 2;
+        if (newf) hipLaunchKernelGGL((k_van_columns<1, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
+        else hipLaunchKernelGGL((k_van_columns<0, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
+    } else {
+        if (newf) hipLaunchKernelGGL((k_van_points<1, true>), dim3(grid_blocks(J.n)), dim3(kWG), 0, st, W, J, W.vw, W.vslot, tgt);
+        else hipLaunchKernelGGL((k_van_points<0, true>), dim3(grid_blocks(J.n)), dim3(kWG), 0, st, W, J, W.vw, W.vslot, tgt);
+    }
     if (hipError_t e = hipGetLastError()) return e;
-    hipLaunchKernelGGL(k_van_search, dim3(grid_blocks(2 * per)), dim3(kWG), 0, st, W, J, (const int32_t *)tgt, fix, cnt);
+    hipLaunchKernelGGL(k_van_search, dim3(grid_blocks(J.ncells)), dim3(kWG), 0, st, W, J, (const int32_t *)tgt, fix, cnt);
     if (hipError_t e = hipGetLastError()) return e;
     // the listed cells' count stays on the device: the window and chain launches cover the most they can
     // hold and return early past the count
@@ -709,7 +781,7 @@ hipError_t launch_van_lattice(const PWorld &W, const PJob &J, bool newf, int32_t
         hipLaunchKernelGGL((k_van_window<0, true>), dim3(wblocks), dim3(kWG), 0, st, W, J, W.vw, W.vslot, (const int32_t *)fix,
                            (const int32_t *)cnt, win);
     if (hipError_t e = hipGetLastError()) return e;
-    int32_t *list2 = fix + 2 * per;
+    int32_t *list2 = fix + J.ncells;
     hipLaunchKernelGGL(k_van_chain, dim3(grid_blocks(kWinCells)), dim3(kWG), 0, st, W, J, (const int32_t *)fix, cnt,
                        (const WinCell *)win, list2);
     if (hipError_t e = hipGetLastError()) return e;
@@ -717,10 +789,8 @@ hipError_t launch_van_lattice(const PWorld &W, const PJob &J, bool newf, int32_t
     L.gen = GEN_LIST;
     L.list = list2;
     L.nlist = cnt + 1;
-    if (newf)
-        hipLaunchKernelGGL((k_cells_van<1, true>), dim3(64), dim3(kWG), 0, st, W, L, W.vw, W.vslot);
-    else
-        hipLaunchKernelGGL((k_cells_van<0, true>), dim3(64), dim3(kWG), 0, st, W, L, W.vw, W.vslot);
+    if (newf) hipLaunchKernelGGL((k_cells_van<1, true>), dim3(64), dim3(kWG), 0, st, W, L, W.vw, W.vslot);
+    else hipLaunchKernelGGL((k_cells_van<0, true>), dim3(64), dim3(kWG), 0, st, W, L, W.vw, W.vslot);
     return hipGetLastError();
 }
 

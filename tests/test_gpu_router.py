@@ -89,6 +89,25 @@ def test_router_lattice_large_grid_equals_interpreter():
     assert ba == bb and np.array_equal(ga, gb), int((ga != gb).sum())
 
 
+@pytest.mark.parametrize("version", ["26.3", "26.1"])
+def test_router_points_path_large_grid_equals_interpreter(version):
+    """A step-4 grid (one zoomed cell per block: the per-point launches, the shape of libmcgen's biome-map
+    benchmark) of 1024 x 768 on both paths, and points with ties near the world's bottom."""
+    ref, W = _world(version, "overworld", 12345)
+    A = mcgpu.BiomeWorld(W)
+    B = _interpreted(W)
+    print("step-4 grid", version, flush=True)
+    ga, ba = A.grid(-2048, -1536, 1024, 768, 4, 64)
+    gb, bb = B.grid(-2048, -1536, 1024, 768, 4, 64)
+    assert ba == bb and np.array_equal(ga, gb), int((ga != gb).sum())
+    rng = np.random.default_rng(31)
+    xyz = np.stack([rng.integers(-300_000, 300_000, 200_000), rng.integers(-70, -50, 200_000),
+                    rng.integers(-300_000, 300_000, 200_000)], 1).astype(np.int32)
+    pa, qa = A.points(xyz)
+    pb, qb = B.points(xyz)
+    assert qa == qb and np.array_equal(pa, pb), int((pa != pb).sum())
+
+
 def test_router_with_permuted_spline_coordinates():
     ref, W = _world("26.3", "overworld", 99)
     P = W.prog.contents
