@@ -136,11 +136,42 @@ int attach_cells(McbEngine *e, const std::vector<McbNode> &tree) {
             uniform = (recs[first + q].leaf & 0xFF) == (recs[first].leaf & 0xFF);
         dent[c] = uniform ? (uint32_t)(recs[first].leaf & 0xFF) << 8 | 1u : en;
     }
+    // the union over each five-dimension cell's depth cells (ascending pre-order, so a strict < still keeps
+    // the first least-distance leaf): the layers of one column share those five dimensions, and one scan
+    // of the union serves every layer.  A union that would exceed 255 leaves, or that meets a tree-walk
+    // entry, gets count 0.
+    const int nd = n[4];
+    const size_t n5 = ct->entry.size() / (size_t)nd;
+    std::vector<uint32_t> uent(n5);
+    std::vector<McbNode> urecs;
+    std::vector<uint16_t> merged;
+    for (size_t c5 = 0; c5 < n5; c5++) {
+        merged.clear();
+        bool ok = true;
+        for (int d = 0; d < nd && ok; d++) {
+            const uint32_t en = ct->entry[c5 * nd + d], cnt = en & 255u, first = en >> 8;
+            ok = cnt > 0;
+            for (uint32_t q = 0; q < cnt; q++) merged.push_back(ct->list[first + q]);
+        }
+        std::sort(merged.begin(), merged.end());
+        merged.erase(std::unique(merged.begin(), merged.end()), merged.end());
+        if (!ok || merged.size() > 255 || urecs.size() >= (1u << 24)) {
+            uent[c5] = 0;
+            continue;
+        }
+        uent[c5] = (uint32_t)urecs.size() << 8 | (uint32_t)merged.size();
+        for (uint16_t l : merged) urecs.push_back(tree[l]);
+    }
+    if (urecs.empty()) urecs.resize(1);
     if (hipMalloc((void **)&e->w.d_cells, dent.size() * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc((void **)&e->w.d_cell_recs, recs.size() * sizeof(McbNode)) != hipSuccess)
+        hipMalloc((void **)&e->w.d_cell_recs, recs.size() * sizeof(McbNode)) != hipSuccess ||
+        hipMalloc((void **)&e->w.d_cell_u, uent.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void **)&e->w.d_cell_urecs, urecs.size() * sizeof(McbNode)) != hipSuccess)
         return fail(MCB_E_NOMEM, "cell table allocation");
     MCB_HIPCHK(hipMemcpy(e->w.d_cells, dent.data(), dent.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     MCB_HIPCHK(hipMemcpy(e->w.d_cell_recs, recs.data(), recs.size() * sizeof(McbNode), hipMemcpyHostToDevice));
+    MCB_HIPCHK(hipMemcpy(e->w.d_cell_u, uent.data(), uent.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    MCB_HIPCHK(hipMemcpy(e->w.d_cell_urecs, urecs.data(), urecs.size() * sizeof(McbNode), hipMemcpyHostToDevice));
     e->cell_entries = ct->entry.size();
     e->cell_list_len = ct->list.size();
     e->cells = ct;
@@ -259,6 +290,8 @@ void mcb_close(McbEngine *e) {
     (void)hipFree(e->w.d_tree);
     (void)hipFree(e->w.d_cells);
     (void)hipFree(e->w.d_cell_recs);
+    (void)hipFree(e->w.d_cell_u);
+    (void)hipFree(e->w.d_cell_urecs);
     (void)hipFree(e->d_slots);
     (void)hipFree(e->d_ident);
     (void)hipFree(e->d_seeds);

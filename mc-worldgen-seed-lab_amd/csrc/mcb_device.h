@@ -542,16 +542,28 @@ __device__ __forceinline__ int cell_best(const McbNode *__restrict__ recs, uint3
     }
     return biome;
 }
-// All layers of a column at once: the entries are loaded together, then candidate q of every layer in
-// the same step, so the loads of different layers overlap.  False when some layer must walk the tree.
+// All layers of a column at once: the entries are loaded together; a layer whose entry names one biome is
+// settled by it.  The others scan the union list of the column's five-dimension cell (every leaf any of
+// its depth cells lists, in ascending pre-order): each record is read once, whole (two 16-byte loads), its
+// five-dimension distance summed once, and only the depth term added per layer.  A leaf of the union that
+// a layer's own list leaves out can never be that layer's (distance, index) minimum, so the strict < over
+// the ascending union keeps the list's answer.  Without a union (count 0) each layer scans its own list.
+// False when some layer must walk the tree.
+__device__ __forceinline__ void load_node(const McbNode *__restrict__ recs, uint32_t i, McbNode &nd) {
+    const uint4 *r = reinterpret_cast<const uint4 *>(recs + i);
+    const uint4 lo = r[0], hi = r[1];
+    __builtin_memcpy(reinterpret_cast<uint4 *>(&nd), &lo, 16);
+    __builtin_memcpy(reinterpret_cast<uint4 *>(&nd) + 1, &hi, 16);
+}
 template <int MAXL>
 __device__ __forceinline__ bool cell_layers(const McbWorld *__restrict__ W, const uint32_t *clut,
                                             const uint32_t *__restrict__ cells, const McbNode *__restrict__ recs,
+                                            const uint32_t *__restrict__ ucells, const McbNode *__restrict__ urecs,
                                             const int *t5, const int *dep, int ny, int *b) {
     const uint32_t c5 = cell_base(W, clut, t5);
     uint32_t e[MAXL];
     int cnt[MAXL], mx = 0;
-    bool ok = true;
+    bool ok = true, open = false;
 #pragma unroll
     for (int j = 0; j < MAXL; j++) e[j] = j < ny ? cells[c5 + cell_depth(clut, dep[j])] : 1u;
 #pragma unroll
@@ -559,21 +571,53 @@ __device__ __forceinline__ bool cell_layers(const McbWorld *__restrict__ W, cons
         cnt[j] = (int)(e[j] & 255u);
         ok = ok && cnt[j] != 0;
         if (cnt[j] == 1) b[j] = (int)((e[j] >> 8) & 255u);
+        open = open || cnt[j] > 1;
         mx = max(mx, cnt[j] == 1 ? 0 : cnt[j]);
     }
     if (!ok) return false;
+    if (!open) return true;
     double bd[MAXL];
 #pragma unroll
     for (int j = 0; j < MAXL; j++) bd[j] = __builtin_inf();
+    const uint32_t u = ucells[c5 / (uint32_t)W->cell_stride[5]];
+    const int un = (int)(u & 255u);
+    if (un) {
+        for (int q = 0; q < un; q++) {
+            McbNode nd;
+            load_node(urecs, (u >> 8) + q, nd);
+            double d5 = (double)nd.d6;
+            d5 = dim_sq(t5[0], nd.lo[0], nd.hi[0], d5);
+            d5 = dim_sq(t5[1], nd.lo[1], nd.hi[1], d5);
+            d5 = dim_sq(t5[2], nd.lo[2], nd.hi[2], d5);
+            d5 = dim_sq(t5[3], nd.lo[3], nd.hi[3], d5);
+            d5 = dim_sq(t5[4], nd.lo[5], nd.hi[5], d5);
+#pragma unroll
+            for (int j = 0; j < MAXL; j++) {
+                if (cnt[j] > 1) {
+                    const double d = dim_sq(dep[j], nd.lo[4], nd.hi[4], d5);
+                    if (d < bd[j]) {
+                        bd[j] = d;
+                        b[j] = nd.leaf & 0xFF;
+                    }
+                }
+            }
+        }
+        return true;
+    }
+    // Step q reads record q of every layer at once (a layer with no record q re-reads record 0 and ignores
+    // it), and only then compares, so the loads of all layers are in flight together.  Record 0 exists
+    // whenever some entry lists two or more leaves (mx > 0).
     for (int q = 0; q < mx; q++) {
+        McbNode nd[MAXL];
+#pragma unroll
+        for (int j = 0; j < MAXL; j++) load_node(recs, q < cnt[j] && cnt[j] > 1 ? (e[j] >> 8) + q : 0u, nd[j]);
 #pragma unroll
         for (int j = 0; j < MAXL; j++) {
             if (q < cnt[j] && cnt[j] > 1) {
-                const McbNode nd = recs[(e[j] >> 8) + q];
-                const double d = leaf_dist5(nd, t5, dep[j]);
+                const double d = leaf_dist5(nd[j], t5, dep[j]);
                 if (d < bd[j]) {
                     bd[j] = d;
-                    b[j] = nd.leaf & 0xFF;
+                    b[j] = nd[j].leaf & 0xFF;
                 }
             }
         }

@@ -64,6 +64,10 @@ struct DevWorld {
     McbNode *d_tree = nullptr;
     uint32_t *d_cells = nullptr;      // Overworld cell table (null elsewhere): entries, see cell_best
     McbNode *d_cell_recs = nullptr;   // the listed leaves, copied in list order
+    // per cell of the five dimensions other than depth: the union of the lists of its depth cells,
+    // (first << 8) | count into d_cell_urecs (count 0: no union, each layer scans its own list)
+    uint32_t *d_cell_u = nullptr;
+    McbNode *d_cell_urecs = nullptr;
 
     double cells_build_ms = 0.0;
     int version = 0, dim = 0;

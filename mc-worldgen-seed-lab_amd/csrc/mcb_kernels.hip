@@ -120,6 +120,7 @@ __global__ __launch_bounds__(64) void k_seed_end(const McbWorld *__restrict__ W,
 template <int MODE, int DIM, int MAXL, bool WRAP>
 __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, const McbNode *__restrict__ tree,
                                               const uint32_t *__restrict__ cells, const McbNode *__restrict__ crecs,
+                                              const uint32_t *__restrict__ ucells, const McbNode *__restrict__ urecs,
                                               const McbSlot *__restrict__ slots, const McbRequest *__restrict__ reqs,
                                               const McbTile *__restrict__ tiles, int ntiles, int tile_w, int cpl,
                                               uint8_t *__restrict__ out) {
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(256) void k_grid(const McbWorld *__restrict__ W, co
                     dep[j] = j < ny ? ow_depth<MODE>(W, col, R.y[j]) : 0;
                     in = in && cell_in_range(dep[j]);
                 }
-                if (in) in = cell_layers<MAXL>(W, clut, cells, crecs, t5, dep, ny, b);
+                if (in) in = cell_layers<MAXL>(W, clut, cells, crecs, ucells, urecs, t5, dep, ny, b);
                 if (!in) tree_search_layers<MAXL>(tree, tree_n, t5, dep, ny, cand, b);
 #pragma unroll
                 for (int j = 0; j < MAXL; j++)
@@ -497,7 +498,7 @@ hipError_t grid_launch3(const DevWorld &w, const McbSlot *slots, const McbReques
                            cpl, out);
     else
         hipLaunchKernelGGL((k_grid<MODE, DIM, MAXL, WRAP>), dim3(blocks), dim3(256), 0, st, w.d_world, w.d_tree, w.d_cells,
-                           w.d_cell_recs, slots, reqs, tiles, ntiles, tile_w, cpl, out);
+                           w.d_cell_recs, w.d_cell_u, w.d_cell_urecs, slots, reqs, tiles, ntiles, tile_w, cpl, out);
     return hipGetLastError();
 }
 // The 26.3 (float) grid kernel has a variant without PerlinNoise.wrap's range test, for launches whose
