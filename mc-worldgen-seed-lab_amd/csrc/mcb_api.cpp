@@ -71,18 +71,43 @@ int mcb::ensure_ident(McbEngine *e, int n) {
 
 // Cell tables depend only on the tree (not on the seed or preset): one build per distinct tree (and
 // dominance rule) per process, shared by every engine, context world and plugin world that loads it.
-std::shared_ptr<const CellTable> mcb::cells_for_tree(const std::vector<McbNode> &tree, std::string &err, bool strict) {
+// The key hashes the tree eight bytes at a time (McbNode is 32 bytes), FNV-1a style over 64-bit words.
+// max_cells > 0: on a miss, a tree whose breakpoint grid would exceed that many cells gets no table
+// (remembered as such).
+std::shared_ptr<const CellTable> mcb::cells_for_tree(const std::vector<McbNode> &tree, std::string &err, bool strict,
+                                                     double max_cells) {
     static std::mutex mu;
-    static std::map<uint64_t, std::shared_ptr<const CellTable>> cache;
+    static std::map<std::pair<uint64_t, uint64_t>, std::shared_ptr<const CellTable>> cache;
+    static_assert(sizeof(McbNode) % 8 == 0, "McbNode words");
     uint64_t h = 1469598103934665603ULL ^ (strict ? 0x5bd1e995ULL : 0);
-    const uint8_t *p = reinterpret_cast<const uint8_t *>(tree.data());
-    for (size_t i = 0; i < tree.size() * sizeof(McbNode); i++) h = (h ^ p[i]) * 1099511628211ULL;
+    const uint64_t *p = reinterpret_cast<const uint64_t *>(tree.data());
+    for (size_t i = 0; i < tree.size() * sizeof(McbNode) / 8; i++) h = (h ^ p[i]) * 1099511628211ULL;
+    const std::pair<uint64_t, uint64_t> key(h, (uint64_t)tree.size() << 1 | (strict ? 1 : 0));
     std::lock_guard<std::mutex> lk(mu);
-    auto it = cache.find(h);
-    if (it != cache.end()) return it->second;
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        if (!it->second) err = "cell table over the size limit";
+        return it->second;
+    }
+    if (max_cells > 0) {
+        std::vector<int32_t> ends[6];
+        for (const McbNode &n : tree)
+            if (n.leaf & 0x8000)
+                for (int k = 0; k < 6; k++) { ends[k].push_back(n.lo[k]); ends[k].push_back(n.hi[k]); }
+        double cells = 1.0;
+        for (int k = 0; k < 6; k++) {
+            std::sort(ends[k].begin(), ends[k].end());
+            cells *= (double)(std::unique(ends[k].begin(), ends[k].end()) - ends[k].begin()) + 1.0;
+        }
+        if (cells > max_cells) {
+            cache[key] = nullptr;
+            err = "cell table over the size limit";
+            return nullptr;
+        }
+    }
     auto ct = std::make_shared<CellTable>();
     if (build_cells(tree, *ct, err, strict) != 0) return nullptr;
-    cache[h] = ct;
+    cache[key] = ct;
     return ct;
 }
 

@@ -41,7 +41,8 @@ struct CellTable {
 // strict: a leaf is left out of a cell only when another leaf is strictly nearer everywhere in it, so the
 // list also holds every leaf that can tie at the least distance (the plugin's tie test needs them)
 int build_cells(const std::vector<McbNode> &tree, CellTable &ct, std::string &err, bool strict = false);
-std::shared_ptr<const CellTable> cells_for_tree(const std::vector<McbNode> &tree, std::string &err, bool strict = false);
+std::shared_ptr<const CellTable> cells_for_tree(const std::vector<McbNode> &tree, std::string &err, bool strict = false,
+                                               double max_cells = 0.0);
 
 // One entry of a biome parameter list (params/<V>/<dim>.tsv): quantized intervals of temperature,
 // humidity, continentalness, erosion, depth, weirdness and the offset (lo = hi), and the biome id.

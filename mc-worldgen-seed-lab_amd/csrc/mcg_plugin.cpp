@@ -386,7 +386,19 @@ bool convert_tree(const McgTreeNode *T, int n, int root, std::vector<PNode> &out
     return true;
 }
 
+// Device resources a world can borrow instead of owning: a stream and grow-only buffers, kept per device
+// for the process.  Worlds made for one call (chunk_biomes) borrow them; creating a stream costs about a
+// millisecond, several times the chunk kernels themselves.
+struct Loan {
+    hipStream_t st = nullptr;
+    char *d_data = nullptr;
+    uint8_t *d_out = nullptr;
+    int32_t *d_xyz = nullptr, *d_bad = nullptr;
+    size_t data_cap = 0, out_cap = 0, xyz_cap = 0;
+};
+
 struct GBiome {
+    Loan *loan = nullptr;                   // borrowed stream and buffers (returned on release), or owned
     PWorld W;
     bool newf = false, perm_lds = false;
     bool router = false;                    // the vanilla router runs through the engine's column code
@@ -396,15 +408,19 @@ struct GBiome {
     char *d_data = nullptr;                 // program and tree, one allocation
     uint8_t *d_out = nullptr, *d_lat = nullptr;
     int32_t *d_xyz = nullptr, *d_bad = nullptr;
-    int32_t *d_tgt = nullptr, *d_fix = nullptr;   // the router's lattice: column targets, cells to follow along their chains
-    WinCell *d_win = nullptr;                     // and their chain windows
-    size_t out_cap = 0, lat_cap = 0, xyz_cap = 0, tgt_cap = 0, fix_cap = 0, win_cap = 0;
+    size_t out_cap = 0, lat_cap = 0, xyz_cap = 0;
     void release() {
+        if (loan) {                         // hand the (possibly grown) buffers back
+            if (st) (void)hipStreamSynchronize(st);
+            loan->d_out = d_out; loan->out_cap = out_cap;
+            loan->d_xyz = d_xyz; loan->xyz_cap = xyz_cap;
+            (void)hipFree(d_lat);
+            st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr;
+            return;
+        }
         if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
         (void)hipFree(d_data); (void)hipFree(d_out); (void)hipFree(d_lat); (void)hipFree(d_xyz); (void)hipFree(d_bad);
-        (void)hipFree(d_tgt); (void)hipFree(d_fix); (void)hipFree(d_win);
-        st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr; d_tgt = d_fix = nullptr;
-        d_win = nullptr;
+        st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr;
     }
 };
 
@@ -438,29 +454,22 @@ struct HostWorld {
 std::shared_ptr<const mcb::CellTable> plugin_cells(const std::vector<PNode> &T) {
     if (T.size() >= 65535) return nullptr;
     std::vector<McbNode> t16(T.size());
-    std::vector<int32_t> ends[6];
     for (size_t i = 0; i < T.size(); i++) {
         const PNode &p = T[i];
         McbNode &m = t16[i];
+        memset(&m, 0, sizeof m);
         for (int k = 0; k < 6; k++) {
             if (p.lo[k] < -32768 || p.lo[k] > 32767 || p.hi[k] < -32768 || p.hi[k] > 32767) return nullptr;
             m.lo[k] = (int16_t)p.lo[k];
             m.hi[k] = (int16_t)p.hi[k];
-            if (p.leaf) { ends[k].push_back(p.lo[k]); ends[k].push_back(p.hi[k]); }
         }
         if (p.d6 > 0xFFFFFFFFull) return nullptr;
         m.d6 = (uint32_t)p.d6;
         m.escape = (uint16_t)p.escape;
         m.leaf = p.leaf ? (uint16_t)(0x8000 | (p.biome & 0xFF)) : 0;
     }
-    double cells = 1.0;
-    for (int k = 0; k < 6; k++) {
-        std::sort(ends[k].begin(), ends[k].end());
-        cells *= (double)(std::unique(ends[k].begin(), ends[k].end()) - ends[k].begin()) + 1.0;
-    }
-    if (cells > (double)(1 << 22)) return nullptr;
     std::string err;
-    return mcb::cells_for_tree(t16, err, true);
+    return mcb::cells_for_tree(t16, err, true, (double)(1 << 22));
 }
 bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
     if (W->source != MCG_BS_MULTI && W->source != MCG_BS_END && W->source != MCG_BS_FIXED) {
@@ -622,8 +631,17 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     for (int k = 0; k < 6; k++) P.root_slot[k] = -1;
     g->source = W->source;
     g->newf = W->newf != 0;
-    HCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
-    HCHK(hipMalloc((void **)&g->d_bad, 4 * sizeof(int32_t)));      // [0] unresolved count, [1..2] the router's list lengths
+    if (Loan *L = g->loan) {
+        if (!L->st) HCHK(hipStreamCreateWithFlags(&L->st, hipStreamNonBlocking));
+        if (!L->d_bad) HCHK(hipMalloc((void **)&L->d_bad, 4 * sizeof(int32_t)));
+        g->st = L->st;
+        g->d_bad = L->d_bad;
+        g->d_out = L->d_out; g->out_cap = L->out_cap;
+        g->d_xyz = L->d_xyz; g->xyz_cap = L->xyz_cap;
+    } else {
+        HCHK(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
+        HCHK(hipMalloc((void **)&g->d_bad, 4 * sizeof(int32_t)));  // [0] unresolved count, [1..2] the router's list lengths
+    }
     if (W->source == MCG_BS_FIXED) return true;
     const McgProg *p = H.p;
     const Compiled &C = H.C;
@@ -655,9 +673,18 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
     if (sh)
         for (int i = 9; i < 13; i++) parts[i].bytes = 0;
     for (auto &q : parts) { q.off = total; total += (q.bytes + 255) & ~(size_t)255; }
-    HCHK(hipMalloc((void **)&g->d_data, std::max<size_t>(total, 256)));
+    if (Loan *L = g->loan) {
+        if (!grow(&L->d_data, &L->data_cap, std::max<size_t>(total, 256))) return false;
+        g->d_data = L->d_data;
+    } else {
+        HCHK(hipMalloc((void **)&g->d_data, std::max<size_t>(total, 256)));
+    }
+    // the parts packed into one host image and uploaded with one copy (one synchronous copy per part cost
+    // about 20 us each)
+    std::vector<char> image(total);
     for (auto &q : parts)
-        if (q.bytes) HCHK(hipMemcpy(g->d_data + q.off, q.src, q.bytes, hipMemcpyHostToDevice));
+        if (q.bytes) memcpy(image.data() + q.off, q.src, q.bytes);
+    if (total) HCHK(hipMemcpy(g->d_data, image.data(), total, hipMemcpyHostToDevice));
     auto at = [&](int i) { return (const void *)(g->d_data + parts[i].off); };
     P.ins = (const PIns *)at(0);
     P.noise = (const McgNoise *)at(1);
@@ -703,6 +730,19 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
 
 int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// The router launches' working buffers (column targets, cells to follow along their chains, their chain
+// windows): one grow-only set per device for the process.  Entries are serialised by the entry lock and
+// each call waits for its stream before returning, so calls never overlap on them.
+struct VanScratch {
+    int32_t *tgt = nullptr, *fix = nullptr;
+    WinCell *win = nullptr;
+    size_t tgt_cap = 0, fix_cap = 0, win_cap = 0;
+};
+VanScratch &van_scratch() {
+    static std::map<int, VanScratch> per_device;
+    return per_device[binding().device()];
+}
+
 // A lattice or points job of a world with the vanilla router through its launches (launch_van_cells);
 // false when the world has no router or its cell table no locator, or the job's cells exceed 2^27 (the
 // launches keep up to 44 bytes per cell) or its buffers cannot be had: the caller runs the per-lane kernel
@@ -710,10 +750,11 @@ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 bool van_run(GBiome *g, const PJob &J) {
     const int64_t nc = van_cells(J);
     if (!g->router || !g->W.clut || !g->W.cent1 || nc > ((int64_t)1 << 27)) return false;
-    const bool grown = grow(&g->d_tgt, &g->tgt_cap, (size_t)van_planes(J) * (size_t)nc) &&
-                       grow(&g->d_fix, &g->fix_cap, 2 * (size_t)nc) && grow(&g->d_win, &g->win_cap, (size_t)kWinCells * kWin);
+    VanScratch &S = van_scratch();
+    const bool grown = grow(&S.tgt, &S.tgt_cap, (size_t)van_planes(J) * (size_t)nc) && grow(&S.fix, &S.fix_cap, 2 * (size_t)nc) &&
+                       grow(&S.win, &S.win_cap, (size_t)kWinCells * kWin);
     hipError_t e = grown ? hipMemsetAsync(g->d_bad + 1, 0, 2 * sizeof(int32_t), g->st) : hipErrorOutOfMemory;
-    if (e == hipSuccess) e = launch_van_cells(g->W, J, g->newf, g->d_tgt, g->d_fix, g->d_win, g->d_bad + 1, g->st);
+    if (e == hipSuccess) e = launch_van_cells(g->W, J, g->newf, S.tgt, S.fix, S.win, g->d_bad + 1, g->st);
     if (e != hipSuccess) {
         (void)hipGetLastError();
         return false;                          // the per-lane kernel runs the job instead
@@ -801,7 +842,7 @@ bool run_chunks(GBiome *g, int n, const int32_t *cxz, uint8_t *out, int *nunres)
     J.cells = cells;
     J.out = g->d_out;
     J.nbad = g->d_bad;
-    HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
     int32_t bad = 0;
     HCHK(hipMemcpyAsync(out, g->d_out, (size_t)items, hipMemcpyDeviceToHost, g->st));
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
@@ -842,6 +883,8 @@ int mcb::plug::chunk_biomes(const McgBiomeWorld *W, int n, const int32_t *cxz, u
     bool ok = false;
     HostWorld H;
     std::unique_ptr<GBiome> g(new GBiome());
+    static std::map<int, Loan> loans;         // guarded by the entry gate
+    if (gate.ready) g->loan = &loans[binding().device()];
     if (n < 0 || (n && (!cxz || !out))) plugin_error("bad arguments");
     else if (gate.ready && prepare_world(W, H) && build_world(g.get(), W, H)) {
         g->W.chain_cap = 1 << 30;            // a chunk's fill order is finite: the walk ends at its first cell

@@ -125,6 +125,7 @@ struct PJob {
     const int32_t *nlist;
     const int32_t *cq;
     int64_t ncells;
+    int32_t geom;                  // router jobs: how cell i is placed (the job's own gen; GEN_LIST keeps it)
 };
 
 // Whole chunks of a world description in libmcgen's stored form (world_chunk_biomes,
@@ -164,8 +165,9 @@ hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds
 // chain window (the cell and kWin - 1 cells back in fill order) evaluated in parallel into win, then each
 // chain resolved from its window; chains the window cannot settle (uncapped ones that run past it) and
 // listed cells beyond kWinCells through the per-lane state machine (list fix + ncells, length cnt[1]).
-// J.gen: GEN_LATTICE (ncells = 2 * nqx * nqz; tgt holds 6 planes), GEN_POINTS or GEN_GRIDPTS (ncells =
-// J.n; 9 planes).  fix holds 2 * ncells, win kWinCells * kWin records; cnt[0..1] must be zero on entry.
+// J.gen: GEN_LATTICE (ncells = 2 * nqx * nqz; tgt holds 6 planes), GEN_CHUNKS (ncells = J.n, whole chunks;
+// 6 planes), GEN_POINTS or GEN_GRIDPTS (ncells = J.n; 9 planes).  fix holds 2 * ncells, win kWinCells *
+// kWin records; cnt[0..1] must be zero on entry.
 constexpr int kWin = kChainCap + 1;
 constexpr int kWinCells = 32768;
 struct WinCell {

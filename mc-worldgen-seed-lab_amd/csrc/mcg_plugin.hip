@@ -289,9 +289,18 @@ __device__ __forceinline__ void count_bad(int32_t *nbad, bool bad) {
     if (m && nbad && (int)(threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) atomicAdd(nbad, (int)__popcll(m));
 }
 
-// Quart coordinates of cell i of a router job: a lattice cell (layer * nqx * nqz + column), or a cell whose
-// coordinates the first launch stored in the planes cq
-__device__ __forceinline__ void list_cell(const PJob &J, int64_t i, int *q) {
+// Quart coordinates of cell i of a router job: a chunk cell (chunk * cells + (qy * 4 + qz) * 4 + qx), a
+// lattice cell (layer * nqx * nqz + column), or a cell whose coordinates the first launch stored in the
+// planes cq
+__device__ __forceinline__ void list_cell(const PWorld &W, const PJob &J, int64_t i, int *q) {
+    if (J.geom == GEN_CHUNKS) {
+        const int64_t ch = i / J.cells;
+        const int r = (int)(i - ch * J.cells);
+        q[0] = J.xyz[2 * ch] * 4 + (r & 3);
+        q[1] = W.min_qy + (r >> 4);
+        q[2] = J.xyz[2 * ch + 1] * 4 + ((r >> 2) & 3);
+        return;
+    }
     if (J.cq) {
         q[0] = J.cq[i]; q[1] = J.cq[J.ncells + i]; q[2] = J.cq[2 * J.ncells + i];
         return;
@@ -339,7 +348,7 @@ __device__ __forceinline__ void cells_items(const PWorld &W, const PJob &J, Eval
             if (lst) {
                 const int64_t i = J.list[it];
                 int q[3];
-                list_cell(J, i, q);
+                list_cell(W, J, i, q);
                 cell_start(W, q[0], q[1], q[2], c);
             } else if (J.gen == GEN_CHUNKS) {
                 const int ch = (int)(it / J.cells), r = (int)(it % J.cells);
@@ -482,6 +491,29 @@ __global__ __launch_bounds__(kWG) void k_van_columns(PJob J, const McbWorld *__r
         put_targets(tgt, nc, per + col, t5, target16(dev::ow_depth_raw<MODE>(vw, c, J.qys[1] * 4)));
     }
 }
+// First launch, chunks (libmcgen's stored chunk biomes): a lane per chunk column (16 per chunk), its climate
+// evaluated once, then the depth target of each of the world's qh quart layers; cell = chunk * cells +
+// layer * 16 + (qz & 3) * 4 + (qx & 3), the chunk output layout.
+template <int MODE, bool WRAP>
+__global__ __launch_bounds__(kWG) void k_van_chunks(PWorld W, PJob J, const McbWorld *__restrict__ vw,
+                                                    const McbSlot *__restrict__ vslot, int32_t *__restrict__ tgt) {
+    __shared__ __attribute__((aligned(16))) VanLds<MODE> L;
+    van_stage<MODE>(L, vw, vslot);
+    const McbOct *oct = vslot->oct;
+    const int64_t nc = J.n, nchunks = J.n / J.cells;
+    for (int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x; it < nchunks * 16; it += (int64_t)gridDim.x * kWG) {
+        const int64_t ch = it >> 4;
+        const int k = (int)(it & 15);
+        const int qx = J.xyz[2 * ch] * 4 + (k & 3), qz = J.xyz[2 * ch + 1] * 4 + (k >> 2);
+        const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, L.perm, L.G, oct, qx, qz);
+        dev::Column c;
+        dev::column_finish<MODE>(vw, &L.spl, n.t, n.h, n.c, n.e, n.w, c);
+        const int32_t t5[5] = {target16((float)n.t), target16((float)n.h), target16((float)n.c), target16((float)n.e),
+                               target16((float)n.w)};
+        for (int ly = 0; ly < W.qh; ly++)
+            put_targets(tgt, nc, ch * J.cells + ly * 16 + k, t5, target16(dev::ow_depth_raw<MODE>(vw, c, (W.min_qy + ly) * 4)));
+    }
+}
 // First launch, points (GEN_POINTS: blocks xyz; GEN_GRIDPTS: the grid's blocks): BiomeManager zoom to a
 // quart cell, its y clamped into the world, then that cell's climate; the six targets as above and the
 // cell's quart x, y, z in planes 6-8.
@@ -555,7 +587,7 @@ __global__ __launch_bounds__(kWG) void k_van_search(PWorld W, PJob J, const int3
                 // x and z multiples of 4), where the chain does not walk back
                 if (nt >= 2) {
                     int q[3];
-                    list_cell(J, i, q);
+                    list_cell(W, J, i, q);
                     if (q[1] == W.min_qy && (q[0] & 3) == 0 && (q[2] & 3) == 0) nt = 1;
                 }
                 if (nt < 2) b = (bl >> 16) & 0xFF;
@@ -585,7 +617,7 @@ __global__ __launch_bounds__(kWG) void k_van_window(PWorld W, PJob J, const McbW
     for (int w = blockIdx.x * kWG + threadIdx.x; w < nf * kWin; w += gridDim.x * kWG) {
         const int f = w / kWin, j = w - f * kWin;
         int q[3];
-        list_cell(J, fix[f], q);
+        list_cell(W, J, fix[f], q);
         Cell c;
         cell_start(W, q[0], q[1], q[2], c);
         WinCell r;
@@ -636,7 +668,7 @@ __global__ __launch_bounds__(kWG) void k_van_chain(PWorld W, PJob J, const int32
         const int32_t i = fix[f];
         if (f >= kWinCells) { list2[atomicAdd(cnt + 1, 1)] = i; continue; }
         int q[3];
-        list_cell(J, i, q);
+        list_cell(W, J, i, q);
         Cell c;
         cell_start(W, q[0], q[1], q[2], c);
         const WinCell *R = win + (int64_t)f * kWin;
@@ -751,16 +783,21 @@ hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds
 }
 
 int64_t van_cells(const PJob &J) { return J.gen == GEN_LATTICE ? 2 * (int64_t)J.nqx * J.nqz : J.n; }
-int van_planes(const PJob &J) { return J.gen == GEN_LATTICE ? 6 : 9; }
+int van_planes(const PJob &J) { return J.gen == GEN_LATTICE || J.gen == GEN_CHUNKS ? 6 : 9; }
 
 hipError_t launch_van_cells(const PWorld &W, const PJob &J0, bool newf, int32_t *tgt, int32_t *fix, WinCell *win, int32_t *cnt,
                             hipStream_t st) {
     PJob J = J0;
+    J.geom = J0.gen;
     J.ncells = van_cells(J0);
     if (J.ncells <= 0) return hipSuccess;
-    const bool lat = J.gen == GEN_LATTICE;
-    J.cq = lat ? nullptr : tgt + 6 * J.ncells;
-    if (lat) {
+    const bool lat = J.gen == GEN_LATTICE, chk = J.gen == GEN_CHUNKS;
+    J.cq = lat || chk ? nullptr : tgt + 6 * J.ncells;
+    if (chk) {
+        const int64_t lanes = J.n / J.cells * 16;
+        if (newf) hipLaunchKernelGGL((k_van_chunks<1, true>), dim3(grid_blocks(lanes)), dim3(kWG), 0, st, W, J, W.vw, W.vslot, tgt);
+        else hipLaunchKernelGGL((k_van_chunks<0, true>), dim3(grid_blocks(lanes)), dim3(kWG), 0, st, W, J, W.vw, W.vslot, tgt);
+    } else if (lat) {
         const int64_t per = J.ncells / 2;
         if (newf) hipLaunchKernelGGL((k_van_columns<1, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
         else hipLaunchKernelGGL((k_van_columns<0, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);

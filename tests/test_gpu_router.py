@@ -136,3 +136,21 @@ def test_perturbed_world_is_interpreted():
     ga, ba = A.grid(1000, -3000, 90, 80, 1, 64)
     gw, bw = ref.grid(1000, -3000, 90, 80, 1, 64)
     assert ba == bw and np.array_equal(ga, gw)
+
+
+@pytest.mark.parametrize("version", ["26.3", "26.1"])
+def test_router_chunks_equal_interpreter(version):
+    """Whole chunks (mcb_chunk_biomes: every quart layer, ties along the fill order with no cap) through
+    the router launches and through the interpreter."""
+    import mcbiome
+    eng = mcbiome.Engine(version, "overworld")
+    xs, zs = np.meshgrid(np.arange(24), np.arange(20))
+    chunks = np.stack([-4000 + xs.ravel(), 9000 + zs.ravel()], 1).astype(np.int32)
+    print("chunks", version, len(chunks), flush=True)
+    a = eng.chunk_biomes(77, chunks)
+    os.environ["MCB_PLUGIN_INTERPRET"] = "1"
+    try:
+        b = eng.chunk_biomes(77, chunks)
+    finally:
+        del os.environ["MCB_PLUGIN_INTERPRET"]
+    assert np.array_equal(a, b), int((a != b).sum())
