@@ -564,6 +564,8 @@ __device__ __forceinline__ bool cell_layers(const McbWorld *__restrict__ W, cons
     uint32_t e[MAXL];
     int cnt[MAXL], mx = 0;
     bool ok = true, open = false;
+    // the union entry is read with the layers' entries, so both are in flight together
+    const uint32_t u = ucells[c5 / (uint32_t)W->cell_stride[5]];
 #pragma unroll
     for (int j = 0; j < MAXL; j++) e[j] = j < ny ? cells[c5 + cell_depth(clut, dep[j])] : 1u;
 #pragma unroll
@@ -579,12 +581,14 @@ __device__ __forceinline__ bool cell_layers(const McbWorld *__restrict__ W, cons
     double bd[MAXL];
 #pragma unroll
     for (int j = 0; j < MAXL; j++) bd[j] = __builtin_inf();
-    const uint32_t u = ucells[c5 / (uint32_t)W->cell_stride[5]];
     const int un = (int)(u & 255u);
     if (un) {
+        // record q + 1 is requested before record q is compared, so its load overlaps the arithmetic
+        McbNode next;
+        load_node(urecs, u >> 8, next);
         for (int q = 0; q < un; q++) {
-            McbNode nd;
-            load_node(urecs, (u >> 8) + q, nd);
+            const McbNode nd = next;
+            if (q + 1 < un) load_node(urecs, (u >> 8) + q + 1, next);
             double d5 = (double)nd.d6;
             d5 = dim_sq(t5[0], nd.lo[0], nd.hi[0], d5);
             d5 = dim_sq(t5[1], nd.lo[1], nd.hi[1], d5);
