@@ -49,8 +49,8 @@ PER_SAMPLE = {"26.3": {"f32": 3}, "26.1": {"f64": 3}}
 PEAK = {"f32": 157.3, "f64": 78.6}     # MI355X published vector peaks, TFLOP/s (FMA counted as 2)
 # HBM bytes per launch of the 26.3 grid kernel on this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (KiB per dispatch) scaled by the same counters on a 256 MiB device copy (FETCH x2.0, WRITE
-# x1.0): profiles/k_grid_batched_4096x64x64x4.md.  (62,929 x 2.0 + 65,634) KiB.
-TRAFFIC_26_3 = (62929 * 2 + 65634) * 1024
+# x1.0): profiles/k_grid_batched_4096x64x64x4.md.  (50,827 x 2.0 + 65,536) KiB.
+TRAFFIC_26_3 = (50827 * 2 + 65536) * 1024
 
 
 def splitmix64_stream(k, n):
