@@ -457,7 +457,7 @@ int mcb_region(McbEngine *e, const McbRegion *req, uint8_t *out_host) {
     int rc = MCB_OK;
     if (hipMalloc((void **)&d, (size_t)bytes) != hipSuccess) rc = fail(MCB_E_NOMEM, "output");
     if (!rc) rc = mcb_plan_run(p, d, nullptr);
-    if (!rc && hipMemcpy(out_host, d, (size_t)bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(MCB_E_DEVICE, "copy back");
+    if (!rc && mcb::copy_to_host(out_host, d, (size_t)bytes, nullptr) != hipSuccess) rc = fail(MCB_E_DEVICE, "copy back");
     (void)hipFree(d);
     mcb_plan_free(p);
     return rc;

@@ -797,8 +797,8 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
         if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
     }
     int32_t bad = 0;
-    HCHK(hipMemcpyAsync(out, g->d_out, (size_t)n, hipMemcpyDeviceToHost, g->st));
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
+    HCHK(mcb::copy_to_host(out, g->d_out, (size_t)n, g->st));
     HCHK(hipStreamSynchronize(g->st));
     if (nunres) *nunres = bad;
     return true;
@@ -821,8 +821,8 @@ bool run_points(GBiome *g, int n, const int *xyz, unsigned char *out, int *nunre
     J.nbad = g->d_bad;
     if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
     int32_t bad = 0;
-    HCHK(hipMemcpyAsync(out, g->d_out, (size_t)n, hipMemcpyDeviceToHost, g->st));
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
+    HCHK(mcb::copy_to_host(out, g->d_out, (size_t)n, g->st));
     HCHK(hipStreamSynchronize(g->st));
     if (nunres) *nunres = bad;
     return true;
@@ -844,8 +844,8 @@ bool run_chunks(GBiome *g, int n, const int32_t *cxz, uint8_t *out, int *nunres)
     J.nbad = g->d_bad;
     if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
     int32_t bad = 0;
-    HCHK(hipMemcpyAsync(out, g->d_out, (size_t)items, hipMemcpyDeviceToHost, g->st));
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
+    HCHK(mcb::copy_to_host(out, g->d_out, (size_t)items, g->st));
     HCHK(hipStreamSynchronize(g->st));
     if (nunres) *nunres = bad;
     return true;

@@ -796,11 +796,11 @@ bool terrain_batch(Terrain *g, int n, const int *cx, const int *cz, McgBeardFn b
         V.lo = W.ore_lo; V.hi = W.ore_hi; V.nmin = W.nmin; V.nh = W.nh;
         if (!hchk(launch_tveins(g->gapw, g->gap_lds, V, B, g->d_veins, g->st), "vein launch")) return false;
     }
-    if (out_dens && !hchk(hipMemcpyAsync(out_dens, g->d_arena + g->vals[g->density].off, 4 * nd * (size_t)n, hipMemcpyDeviceToHost, g->st),
+    if (out_dens && !hchk(mcb::copy_to_host(out_dens, g->d_arena + g->vals[g->density].off, 4 * nd * (size_t)n, g->st),
                           "density copy"))
         return false;
     if (nveins > 0 && out_veins &&
-        !hchk(hipMemcpyAsync(out_veins, g->d_veins, 2 * nd * (size_t)n, hipMemcpyDeviceToHost, g->st), "vein copy"))
+        !hchk(mcb::copy_to_host(out_veins, g->d_veins, 2 * nd * (size_t)n, g->st), "vein copy"))
         return false;
     if (want_cells) {
         std::vector<uint32_t> masks(g->splops.size() * (size_t)g->Bmax, 0u);
@@ -864,7 +864,7 @@ bool terrain_batch(Terrain *g, int n, const int *cx, const int *cz, McgBeardFn b
         } else if (!hchk(hipMemsetAsync(g->d_stage, 0, 4 * (size_t)n * g->cell_floats, g->st), "staging clear")) {
             return false;
         }
-        if (!hchk(hipMemcpyAsync(out_cells, g->d_stage, 4 * (size_t)n * g->cell_floats, hipMemcpyDeviceToHost, g->st), "cell copy"))
+        if (!hchk(mcb::copy_to_host(out_cells, g->d_stage, 4 * (size_t)n * g->cell_floats, g->st), "cell copy"))
             return false;
     }
     return hchk(hipStreamSynchronize(g->st), "batch");
