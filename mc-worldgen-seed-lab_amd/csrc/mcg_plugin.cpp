@@ -402,6 +402,7 @@ struct GBiome {
     PWorld W;
     bool newf = false, perm_lds = false;
     bool router = false;                    // the vanilla router runs through the engine's column code
+    int end_oct = -1;                       // End: the islands octave of the End table (-1: none)
     int lds = 0;
     int source = 0;
     hipStream_t st = nullptr;
@@ -445,6 +446,7 @@ struct HostWorld {
     // the vanilla Overworld router: the engine's constants and seed layout (null: interpret)
     std::unique_ptr<McbWorld> vw;
     std::unique_ptr<McbSlot> vslot;
+    int end_oct = -1;                               // End: the islands nodes' octave (-1: no End table)
 };
 
 // The cell table is built over the engine's compact node form, so it is used only when that form holds
@@ -486,16 +488,22 @@ bool prepare_world(const McgBiomeWorld *W, HostWorld &H) {
     if (W->source == MCG_BS_MULTI)
         for (int k = 0; k < 6; k++)
             if (H.C.root_slot[k] < 0) { plugin_error("no climate field %d", k); return false; }
+    // MCB_PLUGIN_INTERPRET=1: interpret every program with no End table (comparisons of the paths)
+    const char *force = getenv("MCB_PLUGIN_INTERPRET");
+    const bool interpret = force && force[0] == '1';
     if (W->source == MCG_BS_MULTI) {
         if (!convert_tree(W->tree, W->ntree, W->tree_root, H.tree)) return false;
         H.cells = plugin_cells(H.tree);
-        // MCB_PLUGIN_INTERPRET=1: interpret every program (comparisons of the two paths)
-        const char *force = getenv("MCB_PLUGIN_INTERPRET");
-        if (!(force && force[0] == '1')) {
+        if (!interpret) {
             H.vw.reset(new McbWorld());
             H.vslot.reset(new McbSlot());
             if (!vanilla_router(W, *H.vw, *H.vslot)) { H.vw.reset(); H.vslot.reset(); }
         }
+    } else if (!interpret) {
+        // the End table serves the islands nodes of the first one's octave (others compute their windows)
+        for (int i = 0; i < p->nnodes && H.end_oct < 0; i++)
+            if (p->nodes[i].k == (p->old ? MCGO_END_ISLANDS : MCGK_END) && p->nodes[i].oct >= 0 && p->nodes[i].oct < p->noct)
+                H.end_oct = p->nodes[i].oct;
     }
     const int vbytes = H.C.nslot * kWG * (W->newf ? 4 : 8);
     if (grad_lds_bytes(W->newf != 0) + vbytes > kMaxLdsBytes) {
@@ -708,6 +716,7 @@ bool build_world(GBiome *g, const McgBiomeWorld *W, const HostWorld &H) {
             for (int k = 0; k < 6; k++) P.cstride[k] = sh->cstride[k];
         }
     }
+    g->end_oct = H.end_oct;
     if (H.vw) {
         P.vw = (const McbWorld *)at(13);
         P.vslot = (const McbSlot *)at(14);
@@ -779,7 +788,10 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
     // the lattice path indexes by cell, so every block coordinate must be formed without int overflow
     const bool fits = (int64_t)x0 - 2 >= INT32_MIN && (int64_t)z0 - 2 >= INT32_MIN && xmax <= INT32_MAX && zmax <= INT32_MAX;
     if (g->source != MCG_BS_FIXED && fits && nlat <= n && nlat < (1ll << 31)) {
-        if (!grow(&g->d_lat, &g->lat_cap, (size_t)nlat)) return false;
+        // End: the islands heights of the lattice's chunks after its bytes (the table holds one float per chunk)
+        const bool etab = g->source == MCG_BS_END && g->end_oct >= 0;
+        const size_t tab_off = ((size_t)nlat + 255) & ~(size_t)255;
+        if (!grow(&g->d_lat, &g->lat_cap, etab ? tab_off + 4 * (size_t)(nlat / 2) : (size_t)nlat)) return false;
         J.gen = GEN_LATTICE;
         J.n = nlat;
         J.q0x = (int32_t)ax0; J.q0z = (int32_t)az0; J.nqx = (int32_t)nax; J.nqz = (int32_t)naz;
@@ -787,7 +799,16 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
         J.qys[1] = clampi(py + 1, lo, hi);
         J.out = g->d_lat;
         J.nbad = nullptr;                    // counted per output point by the zoom pass
-        if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        if (etab) {
+            PWorld W = g->W;
+            W.etab = (const float *)(g->d_lat + tab_off);
+            W.etab_oct = g->end_oct;
+            W.etab_x0 = J.q0x; W.etab_z0 = J.q0z; W.etab_nx = J.nqx; W.etab_nz = J.nqz;
+            HCHK(launch_end_table(g->W, g->end_oct, J.q0x, J.q0z, J.nqx, J.nqz, !g->newf, (float *)(g->d_lat + tab_off), g->st));
+            HCHK(launch_cells(W, J, g->newf, g->perm_lds, g->lds, g->st));
+        } else if (!van_run(g, J)) {
+            HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        }
         HCHK(launch_zoom_lattice(g->W, J, x0, z0, nx, nz, step, y, g->d_out, g->d_bad, g->st));
     } else {
         J.gen = GEN_GRIDPTS;

@@ -102,6 +102,11 @@ struct PWorld {
     // octaves in the engine's seed layout; null: the program is interpreted
     const McbWorld *vw;
     const McbSlot *vslot;
+    // End islands heights (end_height_j) of a lattice of chunks, filled by k_end_table before the launch:
+    // chunk (etab_x0 + i % etab_nx, etab_z0 + i / etab_nx) at etab[i], for the islands nodes of octave
+    // etab_oct; null: every islands node computes its window
+    const float *etab;
+    int32_t etab_oct, etab_x0, etab_z0, etab_nx, etab_nz;
 };
 
 // work description of one launch
@@ -158,6 +163,9 @@ hipError_t finish_router_slot(McbSlot *d_slot, hipStream_t st);
 
 // launchers (mcg_plugin.hip); lds = bytes of dynamic LDS, perm_lds = permutation tables staged in LDS
 hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st);
+// End islands heights of the chunks (x0 + i % nx, z0 + i / nx), i < nx * nz, from octave oct's simplex
+// table, into out[i] (26.1/26.2 form when old)
+hipError_t launch_end_table(const PWorld &W, int oct, int x0, int z0, int nx, int nz, bool old, float *out, hipStream_t st);
 // Cells of a world with the vanilla router (W.vw) in launches: the quantized climate of every cell into
 // tgt (planes of ncells int32: the six tree targets, then for points the cell's quart x, y, z); the table
 // search of every cell at high occupancy, listing the cells it cannot settle (ties to follow along the fill

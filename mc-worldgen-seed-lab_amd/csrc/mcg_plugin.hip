@@ -776,6 +776,71 @@ static hipError_t launch_t(const PWorld &W, const PJob &J, int spd, int lds, hip
     return hipGetLastError();
 }
 
+// End islands heights (end_height_j) of a lattice of chunks, one workgroup per tile of 16 x 16 chunks.  A
+// chunk c reads section s = 2c + 1, whose window is the 25 x 25 cells around s / 2 (C division).  The tile
+// tests each cell of the union of its chunks' windows once (at most 40 x 40 cells; the island test is a 2-D
+// simplex) and lists the islands with their sizes in LDS; each chunk then takes Java's max over the listed
+// islands inside its own window.  That max propagates NaN from either side and prefers +0 to -0 in either
+// order, so the order of the list does not change the result: each value is end_height_j's.  The per-chunk
+// loop evaluates up to 625 simplex cells; the tile about 6 per chunk.
+template <bool OLD>
+__global__ __launch_bounds__(256) void k_end_table(const McgOct *__restrict__ octs, int oct, int x0, int z0, int nx, int nz,
+                                                   float *__restrict__ out) {
+    constexpr int kTile = 16, kWin = kTile + 24, kCells = kWin * kWin;
+    __shared__ uint8_t P[256];
+    __shared__ int32_t itx[kCells], itz[kCells];
+    __shared__ float isz[kCells];
+    __shared__ int nisl;
+    const int tiles_x = (nx + kTile - 1) / kTile;
+    const int tx0 = (int)(blockIdx.x % (unsigned)tiles_x) * kTile, tz0 = (int)(blockIdx.x / (unsigned)tiles_x) * kTile;
+    P[threadIdx.x] = octs[oct].p[threadIdx.x];
+    if (threadIdx.x == 0) nisl = 0;
+    // window centres are monotonic in the chunk, so the union of the tile's windows is a rectangle
+    auto centre = [](int c) { return (2 * c + 1) / 2; };
+    const int wx0 = centre(x0 + tx0) - 12, wx1 = centre(x0 + min(tx0 + kTile, nx) - 1) + 12;
+    const int wz0 = centre(z0 + tz0) - 12, wz1 = centre(z0 + min(tz0 + kTile, nz) - 1) + 12;
+    const int wnx = wx1 - wx0 + 1, wn = wnx * (wz1 - wz0 + 1);
+    __syncthreads();
+    for (int i = threadIdx.x; i < wn; i += 256) {
+        const int64_t tx = wx0 + i % wnx, tz = wz0 + i / wnx;
+        if (tx * tx + tz * tz <= 4096LL) continue;
+        const double nv = dev::simplex2(P, (double)tx, (double)tz);
+        if (OLD ? !(nv < (double)-0.9f) : !((float)nv < -0.9f)) continue;
+        const int k = atomicAdd(&nisl, 1);
+        itx[k] = (int)tx;
+        itz[k] = (int)tz;
+        isz[k] = fmodf(fabsf((float)tx) * 3439.0f + fabsf((float)tz) * 147.0f, 13.0f) + 9.0f;
+    }
+    __syncthreads();
+    const int lx = (int)threadIdx.x % kTile, lz = (int)threadIdx.x / kTile;
+    if (tx0 + lx >= nx || tz0 + lz >= nz) return;
+    const int sx = 2 * (x0 + tx0 + lx) + 1, sz = 2 * (z0 + tz0 + lz) + 1;
+    const int cx = sx / 2, cz = sz / 2, subx = sx % 2, subz = sz % 2;
+    float doffs = -100.0f;
+    if (OLD) {
+        const int32_t sq = (int32_t)((uint32_t)sx * (uint32_t)sx + (uint32_t)sz * (uint32_t)sz);
+        doffs = jclampf(100.0f - (float)sqrt((double)(float)sq) * 8.0f, -100.0f, 80.0f);
+    }
+    const int n = nisl;
+    for (int k = 0; k < n; k++) {
+        const int xo = itx[k] - cx, zo = itz[k] - cz;
+        if (xo < -12 || xo > 12 || zo < -12 || zo > 12) continue;
+        const float xd = (float)(subx - xo * 2), zd = (float)(subz - zo * 2);
+        const float nd = jclampf(100.0f - (float)sqrt((double)(xd * xd + zd * zd)) * isz[k], -100.0f, 80.0f);
+        doffs = jmaxf(doffs, nd);
+    }
+    out[(int64_t)(tz0 + lz) * nx + tx0 + lx] = doffs;
+}
+
+hipError_t launch_end_table(const PWorld &W, int oct, int x0, int z0, int nx, int nz, bool old, float *out, hipStream_t st) {
+    if (nx <= 0 || nz <= 0) return hipSuccess;
+    const int64_t tiles = (int64_t)((nx + 15) / 16) * ((nz + 15) / 16);
+    if (tiles > INT32_MAX) return hipErrorInvalidValue;
+    if (old) hipLaunchKernelGGL(k_end_table<true>, dim3((unsigned)tiles), dim3(256), 0, st, W.octs, oct, x0, z0, nx, nz, out);
+    else hipLaunchKernelGGL(k_end_table<false>, dim3((unsigned)tiles), dim3(256), 0, st, W.octs, oct, x0, z0, nx, nz, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_cells(const PWorld &W, const PJob &J, bool newf, bool perm_lds, int lds, hipStream_t st) {
     if (J.n <= 0) return hipSuccess;
     if (newf) return perm_lds ? launch_t<true, true>(W, J, W.spl_depth, lds, st) : launch_t<true, false>(W, J, W.spl_depth, lds, st);

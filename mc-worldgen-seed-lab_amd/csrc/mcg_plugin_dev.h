@@ -254,6 +254,19 @@ __device__ float end_height_j(const uint8_t *P, int sx, int sz) {
     return doffs;
 }
 
+// The islands node at block (bx, bz): the height of section (bx / 8, bz / 8) from the End table when that
+// section is a table chunk's (section 2c + 1 of chunk c) and the node reads the table's octave, else the
+// window computed here.
+template <bool OLD>
+__device__ __forceinline__ float end_height_at(const PWorld &W, const uint8_t *P, int oct, int bx, int bz) {
+    const int sx = bx / 8, sz = bz / 8;
+    if (W.etab && oct == W.etab_oct && (sx & 1) && (sz & 1)) {
+        const int ix = ((sx - 1) >> 1) - W.etab_x0, iz = ((sz - 1) >> 1) - W.etab_z0;
+        if ((unsigned)ix < (unsigned)W.etab_nx && (unsigned)iz < (unsigned)W.etab_nz) return W.etab[(int64_t)iz * W.etab_nx + ix];
+    }
+    return end_height_j<OLD>(P, sx, sz);
+}
+
 // ---------------------------------------------------------------------------------------------------
 // spline (mcgpu_dev.cuh:246-289, df_new.c:707-728): bounded compile-time recursion, one call site per
 // level; coordinates are the spline node's coordinate slots
@@ -364,7 +377,7 @@ __device__ void eval_new(const PWorld &W, const uint8_t *lp, float *vals, int ox
             break;
         }
         case MCGK_SHIFT_B: r = ns_get<PL>(W, lp, h.ns, bz * h.d0, bx * h.d0, 0.0) * 4.0f; break;
-        case MCGK_END: r = (end_height_j<false>(perm_of<true, PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0f) / 128.0f; break;
+        case MCGK_END: r = (end_height_at<false>(W, perm_of<true, PL>(W, lp, s.oct), s.oct, bx, bz) - 8.0f) / 128.0f; break;
         case MCGK_DIST: r = dist_value(s.i3, (float)(s.i0 - bx), (float)(s.i1 - by), (float)(s.i2 - bz)); break;
         case MCGK_GRAD_CLAMP: case MCGK_GRAD_REPEAT: case MCGK_GRAD_MIRROR:
             r = grad_value(s, s.i0 == 0 ? bx : (s.i0 == 1 ? by : bz));
@@ -460,7 +473,7 @@ __device__ void eval_old(const PWorld &W, const uint8_t *lp, double *vals, int b
         case MCGO_SHIFT_B: r = old_normal<PL>(W, lp, h.ns, (double)bz * h.d0, (double)bx * h.d0, 0.0 * h.d0) * 4.0; break;
         case MCGO_SHIFT: r = old_normal<PL>(W, lp, h.ns, (double)bx * h.d0, (double)by * h.d1, (double)bz * h.d0) * 4.0; break;
         case MCGO_END_ISLANDS:
-            r = ((double)end_height_j<true>(perm_of<false, PL>(W, lp, s.oct), bx / 8, bz / 8) - 8.0) / 128.0;
+            r = ((double)end_height_at<true>(W, perm_of<false, PL>(W, lp, s.oct), s.oct, bx, bz) - 8.0) / 128.0;
             break;
         case MCGO_Y_GRADIENT: r = clamped_map((double)by, (double)s.i0, (double)s.i1, h.d0, h.d1); break;
         case MCGO_WEIRD_SCALED: {
