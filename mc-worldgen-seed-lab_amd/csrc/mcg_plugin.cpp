@@ -409,19 +409,22 @@ struct GBiome {
     char *d_data = nullptr;                 // program and tree, one allocation
     uint8_t *d_out = nullptr, *d_lat = nullptr;
     int32_t *d_xyz = nullptr, *d_bad = nullptr;
-    size_t out_cap = 0, lat_cap = 0, xyz_cap = 0;
+    float *d_etab = nullptr;                // End islands table (launch_end_table)
+    size_t out_cap = 0, lat_cap = 0, xyz_cap = 0, etab_cap = 0;
     void release() {
         if (loan) {                         // hand the (possibly grown) buffers back
             if (st) (void)hipStreamSynchronize(st);
             loan->d_out = d_out; loan->out_cap = out_cap;
             loan->d_xyz = d_xyz; loan->xyz_cap = xyz_cap;
             (void)hipFree(d_lat);
-            st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr;
+            (void)hipFree(d_etab);
+            st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr; d_etab = nullptr;
             return;
         }
         if (st) { (void)hipStreamSynchronize(st); (void)hipStreamDestroy(st); }
         (void)hipFree(d_data); (void)hipFree(d_out); (void)hipFree(d_lat); (void)hipFree(d_xyz); (void)hipFree(d_bad);
-        st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr;
+        (void)hipFree(d_etab);
+        st = nullptr; d_data = nullptr; d_out = d_lat = nullptr; d_xyz = d_bad = nullptr; d_etab = nullptr;
     }
 };
 
@@ -771,6 +774,24 @@ bool van_run(GBiome *g, const PJob &J) {
     return true;
 }
 
+// The device world of an End job: with the islands table over chunks [cx0, cx0 + ncx) x [cz0, cz0 + ncz)
+// (every chunk the job's items can evaluate) when the table costs less than the windows the items would
+// compute (about 6 simplex cells per table chunk against up to 625 per item); else g->W as it is.
+bool end_world(GBiome *g, int64_t cx0, int64_t cz0, int64_t ncx, int64_t ncz, int64_t items, PWorld &W) {
+    W = g->W;
+    if (g->source != MCG_BS_END || g->end_oct < 0 || ncx <= 0 || ncz <= 0) return true;
+    const int64_t area = ncx * ncz;
+    if (area > 64 * items + 65536 || area >= (1ll << 31) || cx0 < -(1ll << 28) || cz0 < -(1ll << 28) ||
+        cx0 + ncx > (1ll << 28) || cz0 + ncz > (1ll << 28))
+        return true;
+    if (!grow(&g->d_etab, &g->etab_cap, (size_t)area)) return false;
+    HCHK(launch_end_table(g->W, g->end_oct, (int)cx0, (int)cz0, (int)ncx, (int)ncz, !g->newf, g->d_etab, g->st));
+    W.etab = g->d_etab;
+    W.etab_oct = g->end_oct;
+    W.etab_x0 = (int32_t)cx0; W.etab_z0 = (int32_t)cz0; W.etab_nx = (int32_t)ncx; W.etab_nz = (int32_t)ncz;
+    return true;
+}
+
 bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsigned char *out, int *nunres) {
     const int64_t n = (int64_t)nx * nz;
     if (!grow(&g->d_out, &g->out_cap, (size_t)n)) return false;
@@ -788,10 +809,7 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
     // the lattice path indexes by cell, so every block coordinate must be formed without int overflow
     const bool fits = (int64_t)x0 - 2 >= INT32_MIN && (int64_t)z0 - 2 >= INT32_MIN && xmax <= INT32_MAX && zmax <= INT32_MAX;
     if (g->source != MCG_BS_FIXED && fits && nlat <= n && nlat < (1ll << 31)) {
-        // End: the islands heights of the lattice's chunks after its bytes (the table holds one float per chunk)
-        const bool etab = g->source == MCG_BS_END && g->end_oct >= 0;
-        const size_t tab_off = ((size_t)nlat + 255) & ~(size_t)255;
-        if (!grow(&g->d_lat, &g->lat_cap, etab ? tab_off + 4 * (size_t)(nlat / 2) : (size_t)nlat)) return false;
+        if (!grow(&g->d_lat, &g->lat_cap, (size_t)nlat)) return false;
         J.gen = GEN_LATTICE;
         J.n = nlat;
         J.q0x = (int32_t)ax0; J.q0z = (int32_t)az0; J.nqx = (int32_t)nax; J.nqz = (int32_t)naz;
@@ -799,12 +817,9 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
         J.qys[1] = clampi(py + 1, lo, hi);
         J.out = g->d_lat;
         J.nbad = nullptr;                    // counted per output point by the zoom pass
-        if (etab) {
-            PWorld W = g->W;
-            W.etab = (const float *)(g->d_lat + tab_off);
-            W.etab_oct = g->end_oct;
-            W.etab_x0 = J.q0x; W.etab_z0 = J.q0z; W.etab_nx = J.nqx; W.etab_nz = J.nqz;
-            HCHK(launch_end_table(g->W, g->end_oct, J.q0x, J.q0z, J.nqx, J.nqz, !g->newf, (float *)(g->d_lat + tab_off), g->st));
+        if (g->source == MCG_BS_END) {              // the lattice is the table's rectangle of chunks
+            PWorld W;
+            if (!end_world(g, ax0, az0, nax, naz, nlat, W)) return false;
             HCHK(launch_cells(W, J, g->newf, g->perm_lds, g->lds, g->st));
         } else if (!van_run(g, J)) {
             HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
@@ -815,7 +830,13 @@ bool run_grid(GBiome *g, int x0, int z0, int nx, int nz, int step, int y, unsign
         J.n = n;
         J.x0 = x0; J.z0 = z0; J.nx = nx; J.step = step; J.y = y;
         J.out = g->d_out;
-        if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        if (g->source == MCG_BS_END) {              // the chunks of the zoomed quarts, as for the lattice
+            PWorld W;
+            if (!end_world(g, ax0, az0, nax, naz, n, W)) return false;
+            HCHK(launch_cells(W, J, g->newf, g->perm_lds, g->lds, g->st));
+        } else if (!van_run(g, J)) {
+            HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+        }
     }
     int32_t bad = 0;
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
@@ -840,7 +861,20 @@ bool run_points(GBiome *g, int n, const int *xyz, unsigned char *out, int *nunre
     J.xyz = g->d_xyz;
     J.out = g->d_out;
     J.nbad = g->d_bad;
-    if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    if (g->source == MCG_BS_END) {
+        // the chunks of the quarts the zoom can pick: [(x - 2) >> 2, ((x - 2) >> 2) + 1] >> 2
+        int64_t xa = INT64_MAX, xb = INT64_MIN, za = INT64_MAX, zb = INT64_MIN;
+        for (int i = 0; i < n; i++) {
+            const int64_t qx = ((int64_t)xyz[3 * i] - 2) >> 2, qz = ((int64_t)xyz[3 * i + 2] - 2) >> 2;
+            xa = std::min(xa, qx >> 2); xb = std::max(xb, (qx + 1) >> 2);
+            za = std::min(za, qz >> 2); zb = std::max(zb, (qz + 1) >> 2);
+        }
+        PWorld W;
+        if (!end_world(g, xa, za, xb - xa + 1, zb - za + 1, n, W)) return false;
+        HCHK(launch_cells(W, J, g->newf, g->perm_lds, g->lds, g->st));
+    } else if (!van_run(g, J)) {
+        HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    }
     int32_t bad = 0;
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
     HCHK(mcb::copy_to_host(out, g->d_out, (size_t)n, g->st));
@@ -863,7 +897,18 @@ bool run_chunks(GBiome *g, int n, const int32_t *cxz, uint8_t *out, int *nunres)
     J.cells = cells;
     J.out = g->d_out;
     J.nbad = g->d_bad;
-    if (!van_run(g, J)) HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    if (g->source == MCG_BS_END && n > 0) {    // every cell of a chunk evaluates at its chunk's section
+        int64_t xa = INT64_MAX, xb = INT64_MIN, za = INT64_MAX, zb = INT64_MIN;
+        for (int i = 0; i < n; i++) {
+            xa = std::min<int64_t>(xa, cxz[2 * i]); xb = std::max<int64_t>(xb, cxz[2 * i]);
+            za = std::min<int64_t>(za, cxz[2 * i + 1]); zb = std::max<int64_t>(zb, cxz[2 * i + 1]);
+        }
+        PWorld W;
+        if (!end_world(g, xa, za, xb - xa + 1, zb - za + 1, items, W)) return false;
+        HCHK(launch_cells(W, J, g->newf, g->perm_lds, g->lds, g->st));
+    } else if (!van_run(g, J)) {
+        HCHK(launch_cells(g->W, J, g->newf, g->perm_lds, g->lds, g->st));
+    }
     int32_t bad = 0;
     HCHK(hipMemcpyAsync(&bad, g->d_bad, sizeof bad, hipMemcpyDeviceToHost, g->st));
     HCHK(mcb::copy_to_host(out, g->d_out, (size_t)items, g->st));

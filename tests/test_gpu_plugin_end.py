@@ -72,3 +72,49 @@ def test_end_table_large_grid():
     ga, _ = A.grid(-4096, -4096, 2048, 2048, 4, 64)
     gb, _ = B.grid(-4096, -4096, 2048, 2048, 4, 64)
     assert np.array_equal(ga, gb), int((ga != gb).sum())
+
+
+@pytest.mark.parametrize("version", ["26.1", "26.3"])
+def test_end_table_points_and_sparse_grids(version):
+    """Points and grids sampled more sparsely than one block per chunk (the per-point path) read the table
+    over the chunks their zoomed quarts can reach when it is small enough; scattered points compute their
+    windows."""
+    ref, A, B = _worlds(version, 4242)
+    rng = np.random.default_rng(7)
+    clustered = np.stack([rng.integers(-3000, 3000, 5000), rng.integers(0, 256, 5000),
+                          rng.integers(-2500, 2500, 5000)], 1).astype(np.int32)
+    scattered = np.stack([rng.integers(-2**30, 2**30, 300), rng.integers(0, 256, 300),
+                          rng.integers(-2**30, 2**30, 300)], 1).astype(np.int32)
+    for name, xyz in (("clustered", clustered), ("scattered", scattered)):
+        print("end points", version, name, flush=True)
+        pa, qa = A.points(xyz)
+        pb, qb = B.points(xyz)
+        pw, qw = ref.points(xyz)
+        assert qa == qb == qw == 0
+        assert np.array_equal(pa, pb) and np.array_equal(pa, pw), (int((pa != pb).sum()), int((pa != pw).sum()))
+    for step, x0, z0, nx, nz in [(16, -3001, 1999, 300, 200), (64, 2**24 + 3, -(2**24) - 9, 150, 100)]:
+        print("end sparse grid", version, step, flush=True)
+        ga, _ = A.grid(x0, z0, nx, nz, step, 64)
+        gb, _ = B.grid(x0, z0, nx, nz, step, 64)
+        gw, _ = ref.grid(x0, z0, nx, nz, step, 64)
+        assert np.array_equal(ga, gb) and np.array_equal(ga, gw), (int((ga != gb).sum()), int((ga != gw).sum()))
+
+
+@pytest.mark.parametrize("version", ["26.2", "26.3"])
+def test_end_table_chunk_biomes(version):
+    """Whole End chunks (mcb_chunk_biomes): every quart cell of a chunk evaluates at the chunk's section,
+    read from the table over the listed chunks' rectangle."""
+    import mcbiome
+    eng = mcbiome.Engine(version, "the_end")
+    xs, zs = np.meshgrid(np.arange(40), np.arange(30))
+    near = np.stack([-70 + xs.ravel(), -10 + zs.ravel()], 1).astype(np.int32)       # the ring of the centre
+    far = np.stack([3 * 2**20 + xs.ravel(), -(2**21) + zs.ravel()], 1).astype(np.int32)
+    for chunks in (near, far, np.concatenate([near[:50], far[:50]])):
+        print("end chunks", version, len(chunks), flush=True)
+        a = eng.chunk_biomes(99, chunks)
+        os.environ["MCB_PLUGIN_INTERPRET"] = "1"
+        try:
+            b = eng.chunk_biomes(99, chunks)
+        finally:
+            del os.environ["MCB_PLUGIN_INTERPRET"]
+        assert np.array_equal(a, b), int((a != b).sum())
