@@ -972,8 +972,7 @@ bool mcb::plug::plugin_fail(const char *fmt, ...) {
     return false;
 }
 
-bool mcb::plug::point_program(const McgProg *p, PWorld &P, char **d_data, int *lds) {
-    *d_data = nullptr;
+bool mcb::plug::point_program(const McgProg *p, PWorld &P, char **d_data, size_t *cap, int *lds) {
     memset(&P, 0, sizeof P);
     if (!p || p->old) return plugin_error("the point program must be a 26.3+ (float) program");
     Compiled C;
@@ -988,9 +987,18 @@ bool mcb::plug::point_program(const McgProg *p, PWorld &P, char **d_data, int *l
                     {p->thr, (size_t)p->nthr * sizeof(double), 0}};
     size_t total = 0;
     for (auto &q : parts) { q.off = total; total += (q.bytes + 255) & ~(size_t)255; }
-    HCHK(hipMalloc((void **)d_data, std::max<size_t>(total, 256)));
+    total = std::max<size_t>(total, 256);
+    if (!*d_data || *cap < total) {
+        (void)hipFree(*d_data);
+        *d_data = nullptr;
+        *cap = 0;
+        HCHK(hipMalloc((void **)d_data, total));
+        *cap = total;
+    }
+    std::vector<char> image(total);
     for (auto &q : parts)
-        if (q.bytes) HCHK(hipMemcpy(*d_data + q.off, q.src, q.bytes, hipMemcpyHostToDevice));
+        if (q.bytes) memcpy(image.data() + q.off, q.src, q.bytes);
+    HCHK(hipMemcpy(*d_data, image.data(), total, hipMemcpyHostToDevice));
     auto at = [&](int i) { return (const void *)(*d_data + parts[i].off); };
     P.ins = (const PIns *)at(0);
     P.noise = (const McgNoise *)at(1);

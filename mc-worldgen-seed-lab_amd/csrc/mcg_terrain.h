@@ -123,9 +123,10 @@ hipError_t launch_tveins(const PWorld &G, int lds, const TVeins &V, const TBatch
 hipError_t launch_tgather(const TGather *items, int nitems, const float *arena, float *dst, hipStream_t st);
 
 // point program of a 26.3+ McgProg on the device (the ore-vein gap functions): compiled as the biome
-// worlds' programs are, uploaded into one allocation (*d_data, freed by the caller).  lds = dynamic LDS
-// bytes of an evaluation (gradient table and value slots).  false with the plugin error set on failure.
-bool point_program(const McgProg *p, PWorld &W, char **d_data, int *lds);
+// worlds' programs are, uploaded with one copy into *d_data (kept when its *cap bytes suffice, else
+// allocated again; freed by the caller).  lds = dynamic LDS bytes of an evaluation (gradient table and
+// value slots).  false with the plugin error set on failure.
+bool point_program(const McgProg *p, PWorld &W, char **d_data, size_t *cap, int *lds);
 
 }  // namespace plug
 }  // namespace mcb

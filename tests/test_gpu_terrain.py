@@ -144,3 +144,40 @@ def test_engine_terrain_world_matches_restatement():
         assert same_bits(G.batch(cx, cz)[0], dens)
         G.close()
         tw.close()
+
+
+def test_worlds_reuse_device_kits():
+    """Terrain worlds made one after another take over the stream and buffers of freed ones on the device
+    (grown when a world needs more, kept when it needs less): the kinds world (veins, cache cells) and the
+    engine's stand-in world in turn with batch sizes 2, 16, 1 and 8, two of them alive at once, each batch
+    equal to the restatement."""
+    import mcbiome
+    eng = mcbiome.Engine("26.3", "overworld")
+    R, W, r, p = kinds_setup()
+    chunks = CHUNKS[:4]
+    cx, cz = [c[0] for c in chunks], [c[1] for c in chunks]
+    want_kinds = [T.chunk(W, x, z) for x, z in chunks]
+    tw = eng.terrain_world(77)
+    want_eng = [T.chunk(tw.world, x, z)[0] for x, z in chunks]
+
+    def check_kinds(G):
+        for lo in range(0, len(chunks), G.max_chunks):
+            part = slice(lo, lo + G.max_chunks)
+            dens, veins, _, _ = G.batch(cx[part], cz[part])
+            for i, (od, ov, _) in enumerate(want_kinds[part]):
+                assert same_bits(dens[i], od) and np.array_equal(veins[i], ov), (G.max_chunks, lo + i)
+
+    def check_eng(G):
+        for lo in range(0, len(chunks), G.max_chunks):
+            part = slice(lo, lo + G.max_chunks)
+            dens, _, _, _ = G.batch(cx[part], cz[part])
+            for i, od in enumerate(want_eng[part]):
+                assert same_bits(dens[i], od), (G.max_chunks, lo + i)
+
+    for k, mc in enumerate((2, 16, 1, 8)):
+        print("terrain: kit reuse, max_chunks %d" % mc, flush=True)
+        A = mcgpu.TerrainWorld(W, max_chunks=mc)
+        B = mcgpu.TerrainWorld(tw.world, max_chunks=17 - mc)
+        (check_kinds(A), check_eng(B)) if k % 2 == 0 else (check_eng(B), check_kinds(A))
+        (A.close(), B.close()) if k % 2 == 0 else (B.close(), A.close())
+    tw.close()
