@@ -704,19 +704,22 @@ __global__ void k_router_slot(McbSlot *s) {
 }
 
 // block grid read from a lattice of cells (MULTI) or chunks (END)
-// Grid zoom by tiles of 64 x 4 samples: the workgroup first computes the offsets of every lattice point
-// its samples' corners touch (each once, into LDS), then each lane picks its corner from them (the same
-// operations in the same order as the per-block form).  A tile whose lattice exceeds kZoomPts (large
-// steps) computes each block's corners directly.
-constexpr int kZoomTX = 64, kZoomTZ = 4, kZoomPts = 1024;
+// Grid zoom by tiles of 64 x zoom_rows(step) samples: the workgroup first computes the offsets of every
+// lattice point its samples' corners touch (each once, into LDS), then each lane picks the corners of its
+// blocks from them (the same operations in the same order as the per-block form).  A tile whose lattice
+// exceeds kZoomPts (large steps) computes each block's corners directly.  Step 1 takes 32 rows and step 2
+// 16 (8 and 4 blocks per lane: a tile's barriers and lattice points serve more blocks; per 4096^2 grid
+// 4 rows took 142 us, 16 rows 104 us, 32 rows 97 us), larger steps 4 rows so that their lattice fits.
+constexpr int kZoomTX = 64, kZoomPts = 1024;
+__host__ __device__ __forceinline__ int zoom_rows(int step) { return step == 1 ? 32 : (step == 2 ? 16 : 4); }
 __global__ __launch_bounds__(kWG) void k_zoom_tiles(PWorld W, PJob L, int x0, int z0, int nx, int nz, int step, int y,
                                                     uint8_t *out, int32_t *nbad) {
     __shared__ dev::ZFid F[kZoomPts];
-    const int ntx = (nx + kZoomTX - 1) / kZoomTX;
-    const int64_t ntiles = (int64_t)ntx * ((nz + kZoomTZ - 1) / kZoomTZ);
+    const int ntx = (nx + kZoomTX - 1) / kZoomTX, tz = zoom_rows(step);
+    const int64_t ntiles = (int64_t)ntx * ((nz + tz - 1) / tz);
     for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int ix0 = (int)(t % ntx) * kZoomTX, iz0 = (int)(t / ntx) * kZoomTZ;
-        const int ix1 = min(ix0 + kZoomTX, nx) - 1, iz1 = min(iz0 + kZoomTZ, nz) - 1;
+        const int ix0 = (int)(t % ntx) * kZoomTX, iz0 = (int)(t / ntx) * tz;
+        const int ix1 = min(ix0 + kZoomTX, nx) - 1, iz1 = min(iz0 + tz, nz) - 1;
         // lattice box of the tile: corners px..px+1 of every block, px = (x - 2) >> 2 (step > 0)
         const int pxl = (x0 + ix0 * step - 2) >> 2, pxh = ((x0 + ix1 * step - 2) >> 2) + 1;
         const int pzl = (z0 + iz0 * step - 2) >> 2, pzh = ((z0 + iz1 * step - 2) >> 2) + 1;
@@ -730,10 +733,10 @@ __global__ __launch_bounds__(kWG) void k_zoom_tiles(PWorld W, PJob L, int x0, in
                 F[i] = dev::zoom_fiddles(W.zoom_seed, pxl + u, pyl + w, pzl + v);
             }
         __syncthreads();
-        const int ix = ix0 + (int)threadIdx.x % kZoomTX, iz = iz0 + (int)threadIdx.x / kZoomTX;
-        const bool live = ix <= ix1 && iz <= iz1;
+        for (int rr = 0; rr < tz; rr += kWG / kZoomTX) {          // the same trip count in every lane
         bool bad = false;
-        if (live) {
+        const int ix = ix0 + (int)threadIdx.x % kZoomTX, iz = iz0 + rr + (int)threadIdx.x / kZoomTX;
+        if (ix <= ix1 && iz <= iz1) {
             const int x = x0 + ix * step, z = z0 + iz * step;
             int q[3];
             if (shared)
@@ -756,6 +759,7 @@ __global__ __launch_bounds__(kWG) void k_zoom_tiles(PWorld W, PJob L, int x0, in
             out[(int64_t)iz * nx + ix] = (uint8_t)b;
         }
         count_bad(nbad, bad);
+        }
     }
 }
 
@@ -905,7 +909,7 @@ hipError_t launch_zoom_lattice(const PWorld &W, const PJob &L, int x0, int z0, i
                                int32_t *nbad, hipStream_t st) {
     const int64_t n = (int64_t)nx * nz;
     if (n <= 0) return hipSuccess;
-    const int64_t ntiles = (int64_t)((nx + kZoomTX - 1) / kZoomTX) * ((nz + kZoomTZ - 1) / kZoomTZ);
+    const int64_t ntiles = (int64_t)((nx + kZoomTX - 1) / kZoomTX) * ((nz + zoom_rows(step) - 1) / zoom_rows(step));
     const int blocks = (int)(ntiles < 8192 ? ntiles : 8192);
     hipLaunchKernelGGL(k_zoom_tiles, dim3(blocks), dim3(kWG), 0, st, W, L, x0, z0, nx, nz, step, y, out, nbad);
     return hipGetLastError();
