@@ -9,6 +9,7 @@
 // Waits are short polls (a block's copy takes tens of microseconds, and sleeping on a futex or on the
 // device's completion signal costs about as much).  Staging (kStage pinned blocks of kBlock bytes, from
 // hipHostMalloc, per device) and the threads are kept for the life of the process; calls are serialised.
+// Destinations the device reaches directly (pinned host memory, device memory) take one plain copy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -129,14 +130,25 @@ hipError_t wait_event(hipEvent_t ev) {
     }
 }
 
+// dst is memory the device reaches directly (pinned host memory, such as mcgpu_host_alloc's slots, or
+// device memory): one DMA copies there at full rate, so it is not staged
+bool direct_target(const void *dst) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, dst) != hipSuccess) {
+        (void)hipGetLastError();                 // pageable memory the runtime does not know
+        return false;
+    }
+    return a.type == hipMemoryTypeHost || a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
 }  // namespace
 
 hipError_t mcb::copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t st) {
     if (bytes == 0) return hipSuccess;
     std::lock_guard<std::mutex> lk(g_mu);
-    Staging *S = bytes < kDirect ? nullptr : staging();
+    Staging *S = bytes < kDirect || direct_target(dst) ? nullptr : staging();
     if (!S) {
-        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st);
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, st);
         return e == hipSuccess ? hipStreamSynchronize(st) : e;
     }
     const size_t nblk = (bytes + kBlock - 1) / kBlock;

@@ -142,7 +142,8 @@ void lower_region(const McbRegion &r, int slot, int tile_w, int cpl, Lowered &L)
 
 // dst[0..bytes) (host memory, pageable or not) = src[0..bytes) (device memory), after the work queued on st:
 // blocks staged through pinned buffers on st while host threads copy the previous block out; returns when
-// the bytes are in dst.  Copies below 1 MiB are one hipMemcpyAsync + stream synchronisation.
+// the bytes are in dst.  Copies below 1 MiB, and copies into pinned host or device memory, are one
+// hipMemcpyAsync + stream synchronisation.
 hipError_t copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t st);
 
 }  // namespace mcb

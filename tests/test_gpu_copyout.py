@@ -1,8 +1,9 @@
 """Results copied back to the caller's memory through the staged copy (blocks of pinned memory, host
 threads moving each block out while the next one travels): sizes that are not multiples of a block or of
 a page, a destination that starts off a page boundary, sizes around the 1 MiB plain-copy threshold, and
-back-to-back calls of different sizes.  Each output is checked against the per-point paths, whose results
-are small copies, and bytes around an unaligned destination must stay untouched."""
+back-to-back calls of different sizes, and a pinned destination (one direct copy).  Each output is
+checked against the per-point paths, whose results are small copies, and bytes around an unaligned
+destination must stay untouched."""
 import ctypes as C
 
 import numpy as np
@@ -92,3 +93,24 @@ def test_region_and_chunks_against_points():
     sel = rng.integers(0, len(chunks), 40)
     sub = eng.chunk_biomes(seed, chunks[sel])
     assert np.array_equal(c[sel], sub)
+
+
+def test_pinned_destination(world):
+    """A destination in pinned host memory (mcgpu_host_alloc, as libmcgen's terrain service uses) takes one
+    direct copy; the bytes equal those of a pageable destination, and the bytes around it stay untouched."""
+    ref, w = world
+    nx, nz, off = 2500, 1700, 64
+    n = nx * nz
+    L = mcgpu.lib()
+    p = L.mcgpu_host_alloc(n + 2 * off)
+    assert p
+    try:
+        buf = np.ctypeslib.as_array(C.cast(C.c_void_p(p), C.POINTER(C.c_uint8)), shape=(n + 2 * off,))
+        buf[:] = 0x5A
+        print("pinned grid", nx, nz, flush=True)
+        bad = _grid_into(w, buf, off, 900, -1200, nx, nz, 1, 64)
+        a, bad_a = w.grid(900, -1200, nx, nz, 1, 64)
+        assert bad == bad_a and np.array_equal(buf[off:off + n].reshape(nz, nx), a)
+        assert (buf[:off] == 0x5A).all() and (buf[off + n:] == 0x5A).all()
+    finally:
+        L.mcgpu_host_free(p)
