@@ -157,15 +157,20 @@ hipError_t mcb::copy_to_host(void *dst, const void *src, size_t bytes, hipStream
         hipError_t e = hipMemcpyAsync(S->buf[k % kStage], (const char *)src + off, len, hipMemcpyDeviceToHost, st);
         return e == hipSuccess ? hipEventRecord(S->ev[k % kStage], st) : e;
     };
+    // on an error, copies already queued into the staging blocks finish before the next call reuses them
+    auto fail = [&](hipError_t e) {
+        (void)hipStreamSynchronize(st);
+        return e;
+    };
     for (size_t k = 0; k < std::min(nblk, (size_t)kStage); k++)
-        if (hipError_t e = issue(k)) return e;
+        if (hipError_t e = issue(k)) return fail(e);
     pool().run_job((char *)dst, nullptr, bytes);
     for (size_t k = 0; k < nblk; k++) {
-        if (hipError_t e = wait_event(S->ev[k % kStage])) return e;
+        if (hipError_t e = wait_event(S->ev[k % kStage])) return fail(e);
         const size_t off = k * kBlock, len = std::min(kBlock, bytes - off);
         pool().run_job((char *)dst + off, (const char *)S->buf[k % kStage], len);
         if (k + kStage < nblk)
-            if (hipError_t e = issue(k + kStage)) return e;
+            if (hipError_t e = issue(k + kStage)) return fail(e);
     }
     return hipSuccess;
 }
