@@ -192,7 +192,7 @@ int mcb_search(McbEngine *e, const McbSeedSource *src, const McbObs *obs, int no
         }
         if (ow) {
             MCB_HIPCHK(hipMemset(e->d_next, 0, sizeof(uint64_t)));
-            MCB_HIPCHK(launch_search_ow(e->w, A, ncu * 6, nullptr));
+            MCB_HIPCHK(launch_search_ow(e->w, A, ncu * 2, nullptr));      // two resident workgroups per CU
         } else {
             MCB_HIPCHK(launch_seed_gen(A, (int)n1, e->d_seeds, nullptr));
             MCB_HIPCHK(launch_seed_init(e->w, e->d_seeds, e->d_ident, (int)n1, e->d_slots, nullptr));

@@ -78,10 +78,16 @@ __device__ __forceinline__ void step_level(int s, int n, int &k, int &st) {
     else { st = s >= n; k = st ? s - n : s; }
 }
 
+// Four waves per workgroup: each lane's 256-byte Fisher-Yates tile takes 16 KB of LDS per wave, and the
+// workgroup's shared tables (gradients, spline, cell locator: 14-16 KB) are staged once for four waves,
+// so two workgroups fit a CU (8 waves; with two waves per workgroup only three fit: 6 waves).  The
+// shuffle's LDS reads and writes are latency-bound, and the extra waves hide it: 2.25e8 -> 2.81e8
+// candidates/s.  (Drawing the next index while a step's reads are in flight measured no faster.)
+constexpr int kSearchWG = 256;
 template <int MODE>
-__global__ __launch_bounds__(128) void k_search_ow(SearchArgs A) {
+__global__ __launch_bounds__(kSearchWG) void k_search_ow(SearchArgs A) {
     typedef typename R<MODE>::T T_;
-    __shared__ uint32_t tiles[2][64 * 64];
+    __shared__ uint32_t tiles[kSearchWG / 64][64 * 64];
     __shared__ __attribute__((aligned(16))) typename GT<MODE>::T G[256];
     __shared__ SplLds spl[1];
     __shared__ uint32_t clut[MCB_CELL_LUT];
@@ -302,8 +308,8 @@ __global__ void k_seed_gen(SearchArgs A, int n, int64_t *out) {
 
 hipError_t launch_search_ow(const DevWorld &w, const SearchArgs &A, int blocks, hipStream_t st) {
     if (w.dim != MCB_OVERWORLD) return hipErrorInvalidValue;
-    if (w.host.fmt_new) hipLaunchKernelGGL(k_search_ow<1>, dim3(blocks), dim3(128), 0, st, A);
-    else hipLaunchKernelGGL(k_search_ow<0>, dim3(blocks), dim3(128), 0, st, A);
+    if (w.host.fmt_new) hipLaunchKernelGGL(k_search_ow<1>, dim3(blocks), dim3(kSearchWG), 0, st, A);
+    else hipLaunchKernelGGL(k_search_ow<0>, dim3(blocks), dim3(kSearchWG), 0, st, A);
     return hipGetLastError();
 }
 
