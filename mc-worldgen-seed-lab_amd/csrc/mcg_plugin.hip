@@ -450,6 +450,11 @@ __device__ __forceinline__ int32_t target16(float v) {
     return f > -32769.0f && f < 32768.0f ? (int32_t)f : kNoTarget;   // false for NaN
 }
 // the router's LDS: doubled permutation tables, gradient table, offset spline
+// Workgroup size of the column and point launches.  Their LDS tables (31-33 KB) bound a CU to five
+// workgroups: eight waves per workgroup instead of four lift the bound from 5 waves per SIMD to the
+// registers' 5-7 (libmcgen's 2048^2 step-4 map 1.13 -> 1.08 ms).  The chunk launch keeps kWG: its 65,536
+// lanes would make fewer workgroups than CUs.
+constexpr int kVanWG = 512;
 template <int MODE>
 struct VanLds {
     uint8_t perm[MCB_OW_OCT * OCT_BYTES];
@@ -473,14 +478,14 @@ __device__ __forceinline__ void put_targets(int32_t *__restrict__ tgt, int64_t n
     tgt[5 * nc + cell] = t5[4];
 }
 template <int MODE, bool WRAP>
-__global__ __launch_bounds__(kWG) void k_van_columns(PJob J, const McbWorld *__restrict__ vw, const McbSlot *__restrict__ vslot,
+__global__ __launch_bounds__(kVanWG) void k_van_columns(PJob J, const McbWorld *__restrict__ vw, const McbSlot *__restrict__ vslot,
                                                      int32_t *__restrict__ tgt) {
     __shared__ __attribute__((aligned(16))) VanLds<MODE> L;
     van_stage<MODE>(L, vw, vslot);
     const McbOct *oct = vslot->oct;
     const int per = J.nqx * J.nqz;
     const int64_t nc = 2 * (int64_t)per;
-    for (int col = blockIdx.x * kWG + threadIdx.x; col < per; col += gridDim.x * kWG) {
+    for (int col = blockIdx.x * kVanWG + threadIdx.x; col < per; col += gridDim.x * kVanWG) {
         const int qx = J.q0x + col % J.nqx, qz = J.q0z + col / J.nqx;
         const dev::Noises5<MODE> n = dev::ow_noises<MODE, const uint8_t *, WRAP>(vw, L.perm, L.G, oct, qx, qz);
         dev::Column c;
@@ -518,13 +523,13 @@ __global__ __launch_bounds__(kWG) void k_van_chunks(PWorld W, PJob J, const McbW
 // quart cell, its y clamped into the world, then that cell's climate; the six targets as above and the
 // cell's quart x, y, z in planes 6-8.
 template <int MODE, bool WRAP>
-__global__ __launch_bounds__(kWG) void k_van_points(PWorld W, PJob J, const McbWorld *__restrict__ vw,
+__global__ __launch_bounds__(kVanWG) void k_van_points(PWorld W, PJob J, const McbWorld *__restrict__ vw,
                                                     const McbSlot *__restrict__ vslot, int32_t *__restrict__ tgt) {
     __shared__ __attribute__((aligned(16))) VanLds<MODE> L;
     van_stage<MODE>(L, vw, vslot);
     const McbOct *oct = vslot->oct;
     const int64_t nc = J.n;
-    for (int64_t it = (int64_t)blockIdx.x * kWG + threadIdx.x; it < nc; it += (int64_t)gridDim.x * kWG) {
+    for (int64_t it = (int64_t)blockIdx.x * kVanWG + threadIdx.x; it < nc; it += (int64_t)gridDim.x * kVanWG) {
         int x, y, z;
         if (J.gen == GEN_POINTS) {
             x = J.xyz[3 * it]; y = J.xyz[3 * it + 1]; z = J.xyz[3 * it + 2];
@@ -763,8 +768,8 @@ __global__ __launch_bounds__(kWG) void k_zoom_tiles(PWorld W, PJob L, int x0, in
     }
 }
 
-static int grid_blocks(int64_t n) {
-    const int64_t b = (n + kWG - 1) / kWG;
+static int grid_blocks(int64_t n, int wg = kWG) {
+    const int64_t b = (n + wg - 1) / wg;
     return (int)(b < 4096 ? (b > 0 ? b : 1) : 4096);
 }
 
@@ -868,11 +873,11 @@ hipError_t launch_van_cells(const PWorld &W, const PJob &J0, bool newf, int32_t 
         else hipLaunchKernelGGL((k_van_chunks<0, true>), dim3(grid_blocks(lanes)), dim3(kWG), 0, st, W, J, W.vw, W.vslot, tgt);
     } else if (lat) {
         const int64_t per = J.ncells / 2;
-        if (newf) hipLaunchKernelGGL((k_van_columns<1, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
-        else hipLaunchKernelGGL((k_van_columns<0, true>), dim3(grid_blocks(per)), dim3(kWG), 0, st, J, W.vw, W.vslot, tgt);
+        if (newf) hipLaunchKernelGGL((k_van_columns<1, true>), dim3(grid_blocks(per, kVanWG)), dim3(kVanWG), 0, st, J, W.vw, W.vslot, tgt);
+        else hipLaunchKernelGGL((k_van_columns<0, true>), dim3(grid_blocks(per, kVanWG)), dim3(kVanWG), 0, st, J, W.vw, W.vslot, tgt);
     } else {
-        if (newf) hipLaunchKernelGGL((k_van_points<1, true>), dim3(grid_blocks(J.n)), dim3(kWG), 0, st, W, J, W.vw, W.vslot, tgt);
-        else hipLaunchKernelGGL((k_van_points<0, true>), dim3(grid_blocks(J.n)), dim3(kWG), 0, st, W, J, W.vw, W.vslot, tgt);
+        if (newf) hipLaunchKernelGGL((k_van_points<1, true>), dim3(grid_blocks(J.n, kVanWG)), dim3(kVanWG), 0, st, W, J, W.vw, W.vslot, tgt);
+        else hipLaunchKernelGGL((k_van_points<0, true>), dim3(grid_blocks(J.n, kVanWG)), dim3(kVanWG), 0, st, W, J, W.vw, W.vslot, tgt);
     }
     if (hipError_t e = hipGetLastError()) return e;
     hipLaunchKernelGGL(k_van_search, dim3(grid_blocks(J.ncells)), dim3(kWG), 0, st, W, J, (const int32_t *)tgt, fix, cnt);
