@@ -311,6 +311,7 @@ int mcb_open(McbEngine **out, int version, int dim, const char *preset, const ch
 
 void mcb_close(McbEngine *e) {
     if (!e) return;
+    if (e->region_cache && e->region_free) e->region_free(e->region_cache);
     (void)hipFree(e->w.d_world);
     (void)hipFree(e->w.d_tree);
     (void)hipFree(e->w.d_cells);

@@ -153,6 +153,11 @@ struct McbEngine {
     mcb::DevWorld w;
     int device = 0;
     std::mutex mu;
+    // one-off regions (mcb_region): a plan group rebuilt per call in grow-only buffers (mcb_plan.cpp),
+    // guarded by region_mu; freed by region_free at close
+    std::mutex region_mu;
+    void *region_cache = nullptr;
+    void (*region_free)(void *) = nullptr;
     // per-call scratch (grow-only), guarded by mu
     McbSlot *d_slots = nullptr;
     size_t slots_cap = 0;
