@@ -294,8 +294,8 @@ __device__ __forceinline__ typename R<MODE>::T octave_term(PA P, const typename 
     } else {
         const double Xs = st ? X * INPUT_FACTOR : X, Ys = st ? Y * INPUT_FACTOR : Y, f = N.freq[k];
         // the octave's own wrap is the identity on an already wrapped input (|x| <= 2^24): left out
-        const double v = ZC ? oct_zc<0, PA, false>(P, G, o, wrapc(Xs * f), wrapc(Ys * f))
-                            : oct_yc<0, PA, false>(P, G, o, wrapc(Xs * f), wrapc(Ys * f));
+        const double v = ZC ? oct_zc<0, PA, false>(P, G, o, wrapc<WRAP>(Xs * f), wrapc<WRAP>(Ys * f))
+                            : oct_yc<0, PA, false>(P, G, o, wrapc<WRAP>(Xs * f), wrapc<WRAP>(Ys * f));
         return N.amp_d[k] * v * N.vf_d[k];
     }
 }

@@ -135,8 +135,9 @@ struct Lowered {
 int check_region(const McbRegion &r);
 // True when no octave input of the quart-grid requests' climate columns can reach +-2^24 (so PerlinNoise.wrap,
 // engine/mc_noise.h:86-89, is the identity on every one): the largest quart coordinate, widened by the most
-// the offset noise can shift it (4 x twice the summed amplitudes of both stacks; every octave value is below
-// 2 in magnitude), times the largest frequency of any octave, stays below 2^23.
+// the offset noise can shift it (4 x twice the summed amplitudes of both stacks, 26.1/26.2's amplitudes
+// times their level and NormalNoise value factors; every octave value is below 2 in magnitude), times the
+// largest frequency of any octave (and both stacks' input factor), stays below 2^23.
 bool wrap_free(const McbWorld &w, const std::vector<McbRequest> &reqs);
 void lower_region(const McbRegion &r, int slot, int tile_w, int cpl, Lowered &L);
 

@@ -501,14 +501,14 @@ hipError_t grid_launch3(const DevWorld &w, const McbSlot *slots, const McbReques
                            w.d_cell_recs, w.d_cell_u, w.d_cell_urecs, slots, reqs, tiles, ntiles, tile_w, cpl, out);
     return hipGetLastError();
 }
-// The 26.3 (float) grid kernel has a variant without PerlinNoise.wrap's range test, for launches whose
-// octave inputs provably stay inside +-2^24 (wrap_free); the test and its branch cost about 9% of the
-// kernel (26.3 grid 2.84 -> 2.59 ms).  The 26.1/26.2 kernel keeps one form: without the test it measured
-// slower (3.74 -> 3.88 ms).
+// The grid kernels have a variant without PerlinNoise.wrap's range test, for launches whose octave inputs
+// provably stay inside +-2^24 (wrap_free); the test and its branch cost about 9% of the 26.3 kernel
+// (26.3 grid 2.84 -> 2.59 ms).  26.1/26.2 take it up to two layers per column; at four the register
+// allocator gives it 132 VGPRs (3 waves per SIMD) and it measured slower (3.52 -> 3.69 ms).
 template <int MODE, int DIM, int MAXL>
 hipError_t grid_launch2(const DevWorld &w, const McbSlot *slots, const McbRequest *reqs, const McbTile *tiles, int ntiles,
                         uint8_t *out, int tile_w, int cpl, int blocks, bool nowrap, hipStream_t st) {
-    if constexpr (MODE == 1 && DIM != MCB_END)
+    if constexpr (DIM != MCB_END && (MODE == 1 || MAXL <= 2))
         if (nowrap) return grid_launch3<MODE, DIM, MAXL, false>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
     return grid_launch3<MODE, DIM, MAXL, true>(w, slots, reqs, tiles, ntiles, out, tile_w, cpl, blocks, st);
 }

@@ -32,7 +32,12 @@ bool wrap_free(const McbWorld &w, const std::vector<McbRequest> &reqs) {
     for (int i = 0; i < w.n_noise; i++)
         for (int k = 0; k < w.noise[i].nlev; k++) {
             fmax = std::max(fmax, w.noise[i].freq[k] * 1.0181268882175227 * 1.0181268882175227);
-            if (w.dim == MCB_OVERWORLD && i == 0) shift += 4.0 * 2.0 * 2.0 * std::fabs((double)w.noise[0].amp_f[k]);
+            if (w.dim != MCB_OVERWORLD || i != 0) continue;
+            // the offset noise's value: 26.3 sums amp_f * octave over both stacks; 26.1/26.2 sum
+            // amp_d * octave * vf_d per stack, then (v1 + v2) * valueFactor
+            const McbNoise &N = w.noise[0];
+            const double amp = w.fmt_new ? std::fabs((double)N.amp_f[k]) : std::fabs(N.amp_d[k] * N.vf_d[k] * N.value_factor);
+            shift += 4.0 * 2.0 * 2.0 * amp;
         }
     double q = 0.0;
     for (const McbRequest &r : reqs) {
@@ -41,7 +46,7 @@ bool wrap_free(const McbWorld &w, const std::vector<McbRequest> &reqs) {
         q = std::max(q, std::fabs((double)r.z0));
         q = std::max(q, std::fabs((double)r.z0 + r.nz - 1));
     }
-    return w.fmt_new && (q + shift + 1.0) * fmax < 8388608.0;
+    return (q + shift + 1.0) * fmax < 8388608.0;
 }
 
 static void add_tiles(std::vector<McbTile> &tiles, int req, int nx, int nz, int tw, int th) {
@@ -448,7 +453,8 @@ int mcb_plan_shape(const McbPlan *p, int group, int *tile_w, int *cpl, int *bloc
     if (tile_w) *tile_w = G.tile_w;
     if (cpl) *cpl = G.cpl;
     if (blocks) *blocks = G.blocks_arg;
-    if (wrap_free_grid) *wrap_free_grid = G.nowrap_g ? 1 : 0;
+    // the grid launch takes the wrap-free kernel: 26.3, or 26.1/26.2 up to two layers (launch_grid)
+    if (wrap_free_grid) *wrap_free_grid = G.nowrap_g && (G.e->w.host.fmt_new || G.max_ny_g <= 2) ? 1 : 0;
     return MCB_OK;
 }
 

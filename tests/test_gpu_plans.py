@@ -100,17 +100,21 @@ def test_empty_and_bad_inputs():
         eng.plan([mcbiome.make_region(1, 0, 0, 4, 4, list(range(9)))])
 
 
-@pytest.mark.parametrize("version", ["26.3", "26.1"])
-def test_wrap_free_kernel_choice_and_parity(version):
-    """The 26.3 grid runs without PerlinNoise.wrap's range test only when every octave input of the
-    launch provably stays inside +-2^24 (quart coordinates up to about 8e6 in 26.3); plans reaching
-    beyond keep the test.  Results equal the oracle on both sides of the threshold, in one plan and
-    mixed (one far request puts the whole launch on the tested form)."""
-    eng = mcbiome.Engine(version, "overworld")
-    orc = O.Oracle(version, "overworld")
+@pytest.mark.parametrize("version,dim", [("26.3", "overworld"), ("26.1", "overworld"), ("26.1", "the_nether")])
+def test_wrap_free_kernel_choice_and_parity(version, dim):
+    """A grid runs without PerlinNoise.wrap's range test only when every octave input of the launch
+    provably stays inside +-2^24 (quart coordinates up to about 8e6 in the Overworld, 26.3 and, up to two
+    layers per column, 26.1/26.2); plans reaching beyond keep the test.  Results equal the oracle on both
+    sides of the threshold, in one plan and mixed (one far request puts the whole launch on the tested
+    form)."""
+    eng = mcbiome.Engine(version, dim)
+    orc = O.Oracle(version, dim)
     near = [(11, -2**20, 2**20 - 40), (-5, 8_000_000 - 64, -8_000_000)]            # inside: no test needed
     far = [(77, 2**26 - 30, -(2**24) - 7), (1, -(2**27) + 5, 2**25 + 3)]           # wrap decides rounding
-    for cases, expect_free in ((near, version == "26.3"), (far, False), (near + far, False)):
+    # the Nether's highest octave frequency is 2^-6: its inputs reach 2^24 only near quart 2^29
+    edge = far if dim == "overworld" else [(3, 2**29 - 64, -(2**29) + 8)]
+    checks = [(near, True), (edge, False), (near + edge, False)] + ([] if dim == "overworld" else [(far, True)])
+    for cases, expect_free in checks:
         regs, off = [], 0
         for seed, x0, z0 in cases:
             regs.append(mcbiome.make_region(seed, x0, z0, 40, 24, [16, -4], False, off))
